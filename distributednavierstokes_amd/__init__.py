@@ -1,0 +1,11 @@
+"""MI355X (gfx950) single-device time-step hot path of DistributedNavierStokes.
+
+Finite-volume API (``fv``) and pseudo-spectral API (``spectral``) over the C
+ABI of ``libdnsrocm.so`` (include/dns_rocm.h).  Mutating functions of the
+reference (``step!``) are spelled with a trailing underscore (``step_``).
+"""
+from .problem import DNSError, FVProblem, SpectralProblem, tanhstretch  # noqa: F401
+from .fv import (  # noqa: F401
+    ROCmBackend, setup, scalarfield, vectorfield, velocityfield_, exchange_halo_, project_, momentum_, step_,
+    cfl_timestep, solve_, maxdiv, kinetic_energy, channelstats, channelprofiles, spectrumstats, energyspectrum, rk3,
+)

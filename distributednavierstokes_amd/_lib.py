@@ -1,0 +1,101 @@
+"""ctypes binding of libdnsrocm.so (the C ABI in include/dns_rocm.h).
+
+The library is built in-tree (``distributednavierstokes_amd/libdnsrocm.so``)
+by ``__graft_entry__.build()``; there is no fallback: if it is missing or
+cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+from .problem import DNSError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libdnsrocm.so")
+
+DNS_OK = 0
+DNS_FP64, DNS_MIXED = 0, 1
+LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
+FIELD = dict(ux=0, uy=1, uz=2, p=3, qx=4, qy=5, qz=6, delta=7)
+
+_dp = ct.POINTER(ct.c_double)
+
+
+class FVParams(ct.Structure):
+    _fields_ = [
+        ("n", ct.c_int32 * 3),
+        ("lims", ct.c_double * 6),
+        ("ywall", ct.c_int32),
+        ("yfaces", _dp),
+        ("visc", ct.c_double),
+        ("bodyforce", ct.c_double * 3),
+        ("ydiffusion", ct.c_int32),
+        ("precision", ct.c_int32),
+    ]
+
+
+class SpecParams(ct.Structure):
+    _fields_ = [
+        ("n", ct.c_int32),
+        ("kcut", ct.c_int32),
+        ("l", ct.c_double),
+        ("visc", ct.c_double),
+        ("precision", ct.c_int32),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "dns_last_error": (ct.c_char_p, []),
+    "dns_version": (ct.c_char_p, []),
+    "dns_fv_create": (ct.c_int, [ct.POINTER(FVParams), ct.POINTER(ct.c_void_p)]),
+    "dns_fv_destroy": (ct.c_int, [ct.c_void_p]),
+    "dns_fv_set_field": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.c_int32]),
+    "dns_fv_get_field": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.c_int32]),
+    "dns_fv_momentum": (ct.c_int, [ct.c_void_p, ct.c_double, ct.c_double]),
+    "dns_fv_project": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "dns_fv_step": (ct.c_int, [ct.c_void_p, ct.c_double]),
+    "dns_fv_cfl_timestep": (ct.c_int, [ct.c_void_p, ct.c_double, _dp]),
+    "dns_fv_maxdiv": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_fv_kinetic_energy": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_fv_channelstats_sample": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_fv_spectrum_sample": (ct.c_int, [ct.c_void_p, _dp, ct.c_int32]),
+    "dns_fv_sync": (ct.c_int, [ct.c_void_p]),
+    "dns_fv_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "dns_fv_timing_read": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.POINTER(ct.c_int64)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Loads libdnsrocm.so once; raises DNSError if it is absent."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise DNSError(f"HIP library not built: {LIBPATH} is missing (run __graft_entry__.build())")
+        L = ct.CDLL(LIBPATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc):
+    if rc != DNS_OK:
+        msg = lib().dns_last_error()
+        raise DNSError(msg.decode() if msg else f"libdnsrocm error {rc}")
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)

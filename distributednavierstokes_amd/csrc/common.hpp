@@ -1,0 +1,100 @@
+// Shared host/device helpers for the gfx950 kernels of libdnsrocm.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/dns_rocm.h"
+
+namespace dns {
+
+// ---------------------------------------------------------------- errors
+
+void set_error(const char *fmt, ...);
+
+#define DNS_HIP_CHECK(call)                                                             \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::dns::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                             __FILE__, __LINE__);                                       \
+            return DNS_EHIP;                                                            \
+        }                                                                               \
+    } while (0)
+
+#define DNS_TRY(call)                  \
+    do {                               \
+        int rc_ = (call);              \
+        if (rc_ != DNS_OK) return rc_; \
+    } while (0)
+
+// --------------------------------------------------------- complex types
+
+template <typename T> struct c2t;
+template <> struct c2t<double> { using type = double2; };
+template <> struct c2t<float> { using type = float2; };
+template <typename T> using cplx = typename c2t<T>::type;
+
+template <typename T> __host__ __device__ inline cplx<T> mk(T a, T b)
+{
+    cplx<T> r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename T> __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b)
+{
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T> __device__ inline cplx<T> cmulc(cplx<T> a, cplx<T> b)
+{
+    return mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T> __device__ inline cplx<T> cscale(cplx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T> __device__ inline cplx<T> cconj(cplx<T> a) { return mk<T>(a.x, -a.y); }
+// i * a
+template <typename T> __device__ inline cplx<T> cmuli(cplx<T> a) { return mk<T>(-a.y, a.x); }
+
+// ---------------------------------------------- compensated FP64 sums
+
+// Error-free transformation: s + e == a + b exactly.
+__device__ inline void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Double-double accumulator (hi, lo).
+struct dd {
+    double hi, lo;
+};
+__device__ inline dd dd_add(dd a, dd b)
+{
+    double s, e;
+    two_sum(a.hi, b.hi, s, e);
+    e += a.lo + b.lo;
+    double hi = s + e;
+    double lo = e - (hi - s);
+    return dd{hi, lo};
+}
+__device__ inline dd dd_add1(dd a, double b)
+{
+    double s, e;
+    two_sum(a.hi, b, s, e);
+    e += a.lo;
+    double hi = s + e;
+    double lo = e - (hi - s);
+    return dd{hi, lo};
+}
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace dns

@@ -1,0 +1,225 @@
+// Batched line FFTs staged in LDS for gfx950.
+//
+// A workgroup holds `nlines` lines of length n in LDS (line stride `ls`
+// complex elements) and transforms all of them in place with a mixed-radix
+// Stockham autosort schedule (radices 8, 4, 2, 3, 5):
+//
+//   pass with radix R, current sub-transform size Ns, m = n/R:
+//     for j in [0, m):  k = j mod Ns
+//       v[r] = x[j + r m] * w_n^(r k n/(Ns R))          r = 0..R-1
+//       V    = DFT_R(v)
+//       y[(j div Ns) Ns R + k + r Ns] = V[r]
+//
+// Every pass first reads all of its butterfly inputs into registers, waits
+// on a barrier and then writes the outputs back into the same buffer, so one
+// LDS buffer suffices (no ping-pong).  The caller guarantees
+// nlines * n <= EPT * blockDim.x, which bounds the butterflies per thread by
+// ceil(EPT / R) and keeps every register array statically indexed.
+#pragma once
+
+#include "common.hpp"
+
+namespace dns {
+
+constexpr int FFT_MAXPASS = 12;
+
+struct FFTDesc {
+    int n;
+    int npass;
+    int radix[FFT_MAXPASS];
+};
+
+// Factorises n into the supported radices (largest first: 8, 4, 2, then 3, 5).
+// Returns false when n has a prime factor other than 2, 3 and 5.
+inline bool make_fft_desc(int n, FFTDesc &d)
+{
+    d.n = n;
+    d.npass = 0;
+    int m = n;
+    if (n < 1) return false;
+    int p2 = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++p2;
+    }
+    while (p2 >= 3) {
+        d.radix[d.npass++] = 8;
+        p2 -= 3;
+    }
+    if (p2 == 2) d.radix[d.npass++] = 4;
+    if (p2 == 1) d.radix[d.npass++] = 2;
+    while (m % 3 == 0) {
+        d.radix[d.npass++] = 3;
+        m /= 3;
+    }
+    while (m % 5 == 0) {
+        d.radix[d.npass++] = 5;
+        m /= 5;
+    }
+    if (d.npass > FFT_MAXPASS) return false;
+    return m == 1;
+}
+
+// ------------------------------------------------------------ DFT kernels
+
+template <typename T, int R, bool INV> struct Dft;
+
+template <typename T, bool INV> struct Dft<T, 2, INV> {
+    __device__ static inline void run(cplx<T> *v)
+    {
+        cplx<T> a = cadd<T>(v[0], v[1]), b = csub<T>(v[0], v[1]);
+        v[0] = a;
+        v[1] = b;
+    }
+};
+
+// ±i rotation: forward multiplies by -i, inverse by +i.
+template <typename T, bool INV> __device__ inline cplx<T> rot_i(cplx<T> z)
+{
+    return INV ? mk<T>(-z.y, z.x) : mk<T>(z.y, -z.x);
+}
+
+template <typename T, bool INV> struct Dft<T, 4, INV> {
+    __device__ static inline void run(cplx<T> *v)
+    {
+        cplx<T> t0 = cadd<T>(v[0], v[2]), t1 = csub<T>(v[0], v[2]);
+        cplx<T> t2 = cadd<T>(v[1], v[3]), t3 = rot_i<T, INV>(csub<T>(v[1], v[3]));
+        v[0] = cadd<T>(t0, t2);
+        v[2] = csub<T>(t0, t2);
+        v[1] = cadd<T>(t1, t3);
+        v[3] = csub<T>(t1, t3);
+    }
+};
+
+template <typename T, bool INV> struct Dft<T, 8, INV> {
+    __device__ static inline void run(cplx<T> *v)
+    {
+        const T c = T(0.70710678118654752440);
+        cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<T, 4, INV>::run(e);
+        Dft<T, 4, INV>::run(o);
+        // o[q] *= w8^q, w8 = exp(∓2πi/8)
+        cplx<T> o1 = o[1], o3 = o[3];
+        if (INV) {
+            o[1] = mk<T>(c * (o1.x - o1.y), c * (o1.x + o1.y));
+            o[3] = mk<T>(-c * (o3.x + o3.y), c * (o3.x - o3.y));
+        } else {
+            o[1] = mk<T>(c * (o1.x + o1.y), c * (o1.y - o1.x));
+            o[3] = mk<T>(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+        }
+        o[2] = rot_i<T, INV>(o[2]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = cadd<T>(e[q], o[q]);
+            v[q + 4] = csub<T>(e[q], o[q]);
+        }
+    }
+};
+
+template <typename T, bool INV> struct Dft<T, 3, INV> {
+    __device__ static inline void run(cplx<T> *v)
+    {
+        const T h = T(0.86602540378443864676) * (INV ? T(1) : T(-1));
+        cplx<T> t1 = cadd<T>(v[1], v[2]);
+        cplx<T> t2 = mk<T>(v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y);
+        cplx<T> d = csub<T>(v[1], v[2]);
+        cplx<T> t3 = mk<T>(-h * d.y, h * d.x);  // i * h * (v1 - v2)
+        v[0] = cadd<T>(v[0], t1);
+        v[1] = cadd<T>(t2, t3);
+        v[2] = csub<T>(t2, t3);
+    }
+};
+
+template <typename T, bool INV> struct Dft<T, 5, INV> {
+    __device__ static inline void run(cplx<T> *v)
+    {
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T sg = INV ? T(1) : T(-1);
+        const T s1 = sg * T(0.95105651629515357212), s2 = sg * T(0.58778525229247312917);
+        cplx<T> a1 = cadd<T>(v[1], v[4]), a2 = cadd<T>(v[2], v[3]);
+        cplx<T> b1 = csub<T>(v[1], v[4]), b2 = csub<T>(v[2], v[3]);
+        cplx<T> x0 = v[0];
+        cplx<T> p1 = mk<T>(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+        cplx<T> p2 = mk<T>(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+        cplx<T> q1 = mk<T>(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+        cplx<T> q2 = mk<T>(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+        cplx<T> iq1 = cmuli<T>(q1), iq2 = cmuli<T>(q2);
+        v[0] = mk<T>(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+        v[1] = cadd<T>(p1, iq1);
+        v[4] = csub<T>(p1, iq1);
+        v[2] = cadd<T>(p2, iq2);
+        v[3] = csub<T>(p2, iq2);
+    }
+};
+
+// --------------------------------------------------------------- one pass
+
+template <typename T, int R, int EPT, bool INV>
+__device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns, const cplx<T> *__restrict__ tw,
+                                int tid, int nthr)
+{
+    constexpr int BPT = (EPT + R - 1) / R;
+    const int m = n / R;
+    const int total = nlines * m;
+    cplx<T> v[BPT][R];
+#pragma unroll
+    for (int s = 0; s < BPT; ++s) {
+        int t = tid + s * nthr;
+        if (t < total) {
+            int line = t / m;
+            int j = t - line * m;
+            const cplx<T> *src = buf + line * ls + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[s][r] = src[r * m];
+        }
+    }
+    __syncthreads();
+    const int tws = n / (Ns * R);
+#pragma unroll
+    for (int s = 0; s < BPT; ++s) {
+        int t = tid + s * nthr;
+        if (t < total) {
+            int line = t / m;
+            int j = t - line * m;
+            int jd = j / Ns;
+            int k = j - jd * Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cplx<T> w = tw[r * k * tws];
+                    v[s][r] = INV ? cmulc<T>(v[s][r], w) : cmul<T>(v[s][r], w);
+                }
+            }
+            Dft<T, R, INV>::run(v[s]);
+            cplx<T> *dst = buf + line * ls + jd * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * Ns] = v[s][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Transforms nlines lines of d.n points held in LDS at buf[line*ls + t].
+// tw[t] = exp(-2πi t / n) for t in [0, n).  Unnormalised in both directions.
+// Must be called by every thread of the block (contains barriers).
+template <typename T, int EPT, bool INV>
+__device__ inline void fft_lds(cplx<T> *buf, int nlines, int ls, const FFTDesc &d, const cplx<T> *__restrict__ tw)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    int Ns = 1;
+    for (int p = 0; p < d.npass; ++p) {
+        int R = d.radix[p];
+        switch (R) {
+            case 8: fft_pass<T, 8, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
+            case 4: fft_pass<T, 4, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
+            case 2: fft_pass<T, 2, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
+            case 3: fft_pass<T, 3, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
+            case 5: fft_pass<T, 5, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
+            default: break;
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace dns

@@ -1,0 +1,913 @@
+// Host side of the finite-volume path: problem set-up (grid metrics,
+// Poisson and CN coefficients, FFT plans), the RK3 stage orchestration and
+// the C ABI declared in include/dns_rocm.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "fv_kernels.hpp"
+
+namespace dns {
+
+static thread_local char g_errbuf[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_errbuf, sizeof(g_errbuf), fmt, ap);
+    va_end(ap);
+}
+
+const char *last_error() { return g_errbuf; }
+
+// Modified wavenumber of the periodic 3-point Laplacian,
+// modλ(k, n, Δ) = 2(cospi(2(k−1)/n) − 1)/Δ² (src/poisson.jl:12), k0 = k − 1.
+double modlam(int k0, int n, double d)
+{
+    const long double pi = 3.141592653589793238462643383279502884L;
+    int kk = std::min(k0, n - k0); // cos is even: keeps λ(k) == λ(n − k) exactly
+    double c = kk == 0 ? 1.0 : (double)cosl(pi * (2.0L * (long double)kk / (long double)n));
+    return 2.0 * (c - 1.0) / (d * d);
+}
+
+// tw[t] = exp(-2πi t/n), t = 0..len-1, rounded from long double.
+template <typename T> std::vector<cplx<T>> twiddles(int n, int len)
+{
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    std::vector<cplx<T>> tw(len);
+    for (int t = 0; t < len; ++t) {
+        long double a = twopi * (long double)t / (long double)n;
+        tw[t] = mk<T>((T)cosl(a), (T)(-sinl(a)));
+    }
+    return tw;
+}
+
+template <typename V> int dupload(V **dptr, const std::vector<V> &h)
+{
+    *dptr = nullptr;
+    size_t bytes = std::max<size_t>(1, h.size()) * sizeof(V);
+    DNS_HIP_CHECK(hipMalloc((void **)dptr, bytes));
+    if (!h.empty()) DNS_HIP_CHECK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+    return DNS_OK;
+}
+
+template <typename V> int dalloc(V **dptr, size_t count)
+{
+    *dptr = nullptr;
+    DNS_HIP_CHECK(hipMalloc((void **)dptr, std::max<size_t>(1, count) * sizeof(V)));
+    DNS_HIP_CHECK(hipMemset(*dptr, 0, std::max<size_t>(1, count) * sizeof(V)));
+    return DNS_OK;
+}
+
+// Per-family HIP-event timing of kernel launches on the handle's stream.
+struct LaunchTimer {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[8];
+    double ms[8] = {0};
+    long long cnt[8] = {0};
+    hipStream_t st = nullptr;
+    hipEvent_t cur = nullptr;
+    void begin(int fam)
+    {
+        (void)fam;
+        if (!on) return;
+        hipEventCreate(&cur);
+        hipEventRecord(cur, st);
+    }
+    void end(int fam)
+    {
+        if (!on) return;
+        hipEvent_t e;
+        hipEventCreate(&e);
+        hipEventRecord(e, st);
+        ev[fam].push_back({cur, e});
+        if (ev[fam].size() > 4096) flush();
+    }
+    void flush()
+    {
+        hipStreamSynchronize(st);
+        for (int f = 0; f < 8; ++f) {
+            for (auto &p : ev[f]) {
+                float t = 0;
+                hipEventElapsedTime(&t, p.first, p.second);
+                ms[f] += t;
+                cnt[f] += 1;
+                hipEventDestroy(p.first);
+                hipEventDestroy(p.second);
+            }
+            ev[f].clear();
+        }
+    }
+};
+
+enum { FAM_MOM = 0, FAM_X = 1, FAM_Z = 2, FAM_Y = 3, FAM_PGRAD = 4, FAM_CN = 5 };
+
+struct FVBase {
+    virtual ~FVBase() {}
+    virtual int set_field(int f, const double *h, int layout) = 0;
+    virtual int get_field(int f, double *h, int layout) = 0;
+    virtual int momentum(double A, double dt) = 0;
+    virtual int project(int psi) = 0;
+    virtual int step(double dt) = 0;
+    virtual int cfl(double cfl, double *dt) = 0;
+    virtual int maxdiv(double *out) = 0;
+    virtual int kinetic(double *out) = 0;
+    virtual int chstats(double *acc) = 0;
+    virtual int spectrum(double *E, int nbins) = 0;
+    virtual int sync() = 0;
+    LaunchTimer timer;
+};
+
+static const double RK_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+static const double RK_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+static const double RK_ALPHA[3] = {1.0 / 3.0, 5.0 / 12.0, 1.0 / 4.0};
+
+constexpr int X_EPT = 8, ZY_EPT = 16, NTHR = 256;
+constexpr int CH_NCHUNK = 16;
+constexpr int RED_BLOCKS = 1024;
+
+template <typename T> struct FVImpl : FVBase {
+    FVGeom g{};
+    int imexy = 0;
+    double visc = 0, f[3] = {0, 0, 0}, dx = 0, dz = 0, dymin = 0;
+    std::vector<double> dyh, dyuh;
+    hipStream_t st = nullptr;
+    T *u[3] = {}, *v[3] = {}, *q[3] = {}, *p = nullptr, *delta = nullptr;
+    cplx<T> *A = nullptr;
+    double2 *Dth = nullptr;
+    double *Dcn = nullptr;
+    T *dy_t = nullptr, *rdy_t = nullptr, *rdyu_t = nullptr;
+    double *dy_d = nullptr, *dyu_d = nullptr, *rdy_d = nullptr, *rdyu_d = nullptr;
+    double *lam1 = nullptr, *lam2 = nullptr, *lam3 = nullptr, *ay = nullptr, *cy = nullptr, *iptab = nullptr;
+    double *ydc[6] = {}, *cncoef = nullptr;
+    FFTDesc fdx{}, fdz{}, fdy{};
+    cplx<T> *twN = nullptr, *twx = nullptr, *twz = nullptr, *twy = nullptr;
+    int lpb = 1, tbz = 1, tby = 1;
+    size_t ldsx = 0, ldsz = 0, ldsy = 0;
+    double *red = nullptr, *redout = nullptr;
+    size_t redcap = 0;
+    double *pinned = nullptr;
+    int N = 0;
+
+    ~FVImpl() override
+    {
+        if (st) hipStreamSynchronize(st);
+        timer.flush();
+        void *ptrs[] = {u[0], u[1], u[2], v[0], v[1], v[2], q[0], q[1], q[2], p, delta, A, Dth, Dcn, dy_t, rdy_t,
+                        rdyu_t, dy_d, dyu_d, rdy_d, rdyu_d, lam1, lam2, lam3, ay, cy, iptab, ydc[0], ydc[1], ydc[2],
+                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout};
+        for (void *q_ : ptrs)
+            if (q_) hipFree(q_);
+        if (pinned) hipHostFree(pinned);
+        if (st) hipStreamDestroy(st);
+    }
+
+    int create(const dns_fv_params *P)
+    {
+        const int nx = P->n[0], ny = P->n[1], nz = P->n[2];
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = nz;
+        g.nxc = nx / 2 + 1;
+        g.wall = P->ywall ? 1 : 0;
+        imexy = P->ydiffusion ? 1 : 0;
+        visc = P->visc;
+        for (int c = 0; c < 3; ++c) f[c] = P->bodyforce[c];
+        N = nx * ny * nz;
+        dx = (P->lims[1] - P->lims[0]) / nx;
+        dz = (P->lims[5] - P->lims[4]) / nz;
+
+        // griddata (src/setup.jl:148-193): extended faces t = 0..ny+2 hold
+        // faces −1..ny+1; ghost faces mirror at walls or wrap periodically.
+        const double *yf = P->yfaces;
+        std::vector<double> yfe(ny + 3);
+        for (int j = 0; j <= ny; ++j) yfe[j + 1] = yf[j];
+        if (g.wall) {
+            yfe[0] = 2 * yf[0] - yf[1];
+            yfe[ny + 2] = 2 * yf[ny] - yf[ny - 1];
+        } else {
+            yfe[0] = yf[0] - (yf[ny] - yf[ny - 1]);
+            yfe[ny + 2] = yf[ny] + (yf[1] - yf[0]);
+        }
+        dyh.assign(ny + 2, 0.0);
+        dyuh.assign(ny + 2, 0.0);
+        std::vector<double> yce(ny + 2);
+        for (int c = 0; c < ny + 2; ++c) {
+            dyh[c] = yfe[c + 1] - yfe[c];
+            yce[c] = (yfe[c] + yfe[c + 1]) / 2;
+        }
+        for (int c = 0; c < ny + 1; ++c) dyuh[c] = yce[c + 1] - yce[c];
+        dyuh[ny + 1] = dyh[ny + 1];
+        dymin = *std::min_element(dyh.begin() + 1, dyh.begin() + ny + 1);
+        if (!g.wall) {
+            double mx = *std::max_element(dyh.begin() + 1, dyh.begin() + ny + 1);
+            if (!(mx - dymin < 1e-12 * mx)) {
+                set_error("periodic y requires a uniform grid (stretch = identity)");
+                return DNS_EINVAL;
+            }
+        }
+
+        DNS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        timer.st = st;
+        for (int c = 0; c < 3; ++c) {
+            DNS_TRY(dalloc(&u[c], N));
+            DNS_TRY(dalloc(&q[c], N));
+            if (!imexy) DNS_TRY(dalloc(&v[c], N));
+        }
+        DNS_TRY(dalloc(&p, N));
+        if (imexy) DNS_TRY(dalloc(&delta, N));
+        DNS_TRY(dalloc(&A, (size_t)g.nxc * ny * nz));
+
+        std::vector<T> dyt(ny + 2), rdyt(ny + 2), rdyut(ny + 2);
+        std::vector<double> rdyh(ny + 2), rdyuh(ny + 2);
+        for (int c = 0; c < ny + 2; ++c) {
+            rdyh[c] = 1.0 / dyh[c];
+            rdyuh[c] = 1.0 / dyuh[c];
+            dyt[c] = (T)dyh[c];
+            rdyt[c] = (T)rdyh[c];
+            rdyut[c] = (T)rdyuh[c];
+        }
+        DNS_TRY(dupload(&dy_t, dyt));
+        DNS_TRY(dupload(&rdy_t, rdyt));
+        DNS_TRY(dupload(&rdyu_t, rdyut));
+        DNS_TRY(dupload(&dy_d, dyh));
+        DNS_TRY(dupload(&dyu_d, dyuh));
+        DNS_TRY(dupload(&rdy_d, rdyh));
+        DNS_TRY(dupload(&rdyu_d, rdyuh));
+
+        // plan_poisson (src/poisson.jl:167-231)
+        std::vector<double> l1(g.nxc), l3(nz);
+        for (int k = 0; k < g.nxc; ++k) l1[k] = modlam(k, nx, dx);
+        for (int k = 0; k < nz; ++k) l3[k] = modlam(k, nz, dz);
+        DNS_TRY(dupload(&lam1, l1));
+        DNS_TRY(dupload(&lam3, l3));
+        if (!make_fft_desc(nx / 2, fdx) || !make_fft_desc(nz, fdz)) {
+            set_error("unsupported grid: nx/2 and nz must factor into 2, 3 and 5 (nx even)");
+            return DNS_EUNSUPPORTED;
+        }
+        if (nx % 2 != 0) {
+            set_error("unsupported grid: nx must be even");
+            return DNS_EUNSUPPORTED;
+        }
+        if (g.wall) {
+            std::vector<double> a(ny), b(ny), c(ny);
+            for (int j = 1; j <= ny; ++j) {
+                double aa = j == 1 ? 0.0 : 1 / (dyh[j] * dyuh[j - 1]);
+                double cc = j == ny ? 0.0 : 1 / (dyh[j] * dyuh[j]);
+                a[j - 1] = aa;
+                c[j - 1] = cc;
+                b[j - 1] = -(aa + cc);
+            }
+            DNS_TRY(dupload(&ay, a));
+            DNS_TRY(dupload(&cy, c));
+            // inverse pivots of every line (kx, |kz|) of D_y + λ
+            const int nzs = nz / 2 + 1;
+            std::vector<double> ip((size_t)g.nxc * ny * nzs);
+            for (int kz = 0; kz < nzs; ++kz)
+                for (int kx = 0; kx < g.nxc; ++kx) {
+                    double lam = l1[kx] + l3[kz];
+                    bool mean = lam == 0.0;
+                    double beta = mean ? 1.0 : b[0] + lam;
+                    double cp = (mean ? 0.0 : c[0]) / beta;
+                    ip[kx + (size_t)g.nxc * ((size_t)ny * kz)] = 1.0 / beta;
+                    for (int j = 1; j < ny; ++j) {
+                        beta = (b[j] + lam) - a[j] * cp;
+                        cp = c[j] / beta;
+                        ip[kx + (size_t)g.nxc * (j + (size_t)ny * kz)] = 1.0 / beta;
+                    }
+                }
+            DNS_TRY(dupload(&iptab, ip));
+            DNS_TRY(dalloc(&Dth, (size_t)g.nxc * ny * nz));
+        } else {
+            std::vector<double> l2(ny);
+            for (int k = 0; k < ny; ++k) l2[k] = modlam(k, ny, dyh[1]);
+            DNS_TRY(dupload(&lam2, l2));
+            if (!make_fft_desc(ny, fdy)) {
+                set_error("unsupported grid: periodic ny must factor into 2, 3 and 5");
+                return DNS_EUNSUPPORTED;
+            }
+        }
+        if (imexy) {
+            // ydiffusion_coefficients (src/ydiffusion.jl:31-51)
+            std::vector<double> at(ny), bt(ny), ct(ny), an(ny, 0.0), bn(ny, 0.0), cn(ny, 0.0);
+            for (int j = 1; j <= ny; ++j) {
+                double aa = 1 / (dyh[j] * dyuh[j - 1]);
+                double cc = 1 / (dyh[j] * dyuh[j]);
+                at[j - 1] = j == 1 ? 0.0 : aa;
+                ct[j - 1] = j == ny ? 0.0 : cc;
+                bt[j - 1] = j == 1 ? -(cc + 2 * aa) : j == ny ? -(aa + 2 * cc) : -(aa + cc);
+            }
+            for (int j = 1; j <= ny - 1; ++j) {
+                double aa = 1 / (dyuh[j] * dyh[j]);
+                double cc = 1 / (dyuh[j] * dyh[j + 1]);
+                an[j - 1] = j == 1 ? 0.0 : aa;
+                cn[j - 1] = j == ny - 1 ? 0.0 : cc;
+                bn[j - 1] = -(aa + cc);
+            }
+            std::vector<double> *sets[6] = {&at, &bt, &ct, &an, &bn, &cn};
+            for (int s = 0; s < 6; ++s) DNS_TRY(dupload(&ydc[s], *sets[s]));
+            DNS_TRY(dalloc(&cncoef, 4 * (size_t)ny));
+            DNS_TRY(dalloc(&Dcn, (size_t)N));
+        }
+
+        // FFT tiles
+        const int Nh = nx / 2;
+        if (Nh > X_EPT * NTHR || nz > ZY_EPT * NTHR || (!g.wall && ny > ZY_EPT * NTHR)) {
+            set_error("unsupported grid: FFT line longer than %d points", ZY_EPT * NTHR);
+            return DNS_EUNSUPPORTED;
+        }
+        int dev = 0, maxlds = 65536;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (maxlds <= 0) maxlds = 65536;
+        lpb = std::max(1, std::min(64, X_EPT * NTHR / std::max(1, Nh)));
+        ldsx = (size_t)lpb * Nh * sizeof(cplx<T>);
+        auto pick_tb = [&](int n) {
+            int tb = std::max(1, std::min(64, ZY_EPT * NTHR / n));
+            while (tb > 1 && (size_t)tb * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tb;
+            return std::min(tb, g.nxc);
+        };
+        tbz = pick_tb(nz);
+        ldsz = (size_t)tbz * (nz + 1) * sizeof(cplx<T>);
+        if (!g.wall) {
+            tby = pick_tb(ny);
+            ldsy = (size_t)tby * (ny + 1) * sizeof(cplx<T>);
+        }
+        if (ldsz > (size_t)maxlds || ldsy > (size_t)maxlds) {
+            set_error("FFT tile does not fit the %d-byte LDS limit", maxlds);
+            return DNS_EUNSUPPORTED;
+        }
+        DNS_TRY(dupload(&twN, twiddles<T>(std::max(1, Nh), std::max(1, Nh))));
+        DNS_TRY(dupload(&twx, twiddles<T>(nx, Nh + 1)));
+        DNS_TRY(dupload(&twz, twiddles<T>(nz, nz)));
+        if (!g.wall) DNS_TRY(dupload(&twy, twiddles<T>(ny, ny)));
+
+        redcap = std::max<size_t>({(size_t)RED_BLOCKS * 4, (size_t)16 * ny * CH_NCHUNK + 16});
+        DNS_TRY(dalloc(&red, redcap));
+        DNS_TRY(dalloc(&redout, (size_t)8 * ny + 64));
+        DNS_HIP_CHECK(hipHostMalloc((void **)&pinned, sizeof(double) * ((size_t)8 * ny + 64)));
+        return DNS_OK;
+    }
+
+    // ------------------------------------------------------------ fields
+    T *field_ptr(int fid)
+    {
+        switch (fid) {
+            case DNS_FIELD_UX: return u[0];
+            case DNS_FIELD_UY: return u[1];
+            case DNS_FIELD_UZ: return u[2];
+            case DNS_FIELD_P: return p;
+            case DNS_FIELD_QX: return q[0];
+            case DNS_FIELD_QY: return q[1];
+            case DNS_FIELD_QZ: return q[2];
+            case DNS_FIELD_DELTA: return delta ? delta : p;
+        }
+        return nullptr;
+    }
+
+    int set_field(int fid, const double *h, int layout) override
+    {
+        T *d = field_ptr(fid);
+        if (!d || !h) {
+            set_error("invalid field id %d or null pointer", fid);
+            return DNS_EINVAL;
+        }
+        const int nx = g.nx, ny = g.ny, nz = g.nz;
+        std::vector<T> s((size_t)N);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    size_t src = layout == DNS_LAYOUT_GHOSTED
+                                     ? (size_t)(i + 1) + (size_t)(nx + 2) * ((j + 1) + (size_t)(ny + 2) * (k + 1))
+                                     : (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+                    s[i + (size_t)nx * (j + (size_t)ny * k)] = (T)h[src];
+                }
+        // the top wall face of u.y is a boundary value (zero), src/halo.jl:31-33
+        if (fid == DNS_FIELD_UY && g.wall)
+            for (int k = 0; k < nz; ++k)
+                for (int i = 0; i < nx; ++i) s[i + (size_t)nx * ((ny - 1) + (size_t)ny * k)] = T(0);
+        DNS_HIP_CHECK(hipMemcpyAsync(d, s.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+
+    int get_field(int fid, double *h, int layout) override
+    {
+        T *d = field_ptr(fid);
+        if (!d || !h) {
+            set_error("invalid field id %d or null pointer", fid);
+            return DNS_EINVAL;
+        }
+        const int nx = g.nx, ny = g.ny, nz = g.nz;
+        std::vector<T> s((size_t)N);
+        DNS_HIP_CHECK(hipMemcpyAsync(s.data(), d, sizeof(T) * N, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        if (layout != DNS_LAYOUT_GHOSTED) {
+            for (size_t t = 0; t < (size_t)N; ++t) h[t] = (double)s[t];
+            return DNS_OK;
+        }
+        const int GX = nx + 2, GY = ny + 2, GZ = nz + 2;
+        auto G = [&](int i, int j, int k) { return (size_t)i + (size_t)GX * (j + (size_t)GY * k); };
+        std::fill(h, h + (size_t)GX * GY * GZ, 0.0);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) h[G(i + 1, j + 1, k + 1)] = (double)s[i + (size_t)nx * (j + (size_t)ny * k)];
+        if (fid >= DNS_FIELD_QX && fid <= DNS_FIELD_QZ) return DNS_OK; // registers carry no halos
+        int stag = fid == DNS_FIELD_UX ? 0 : fid == DNS_FIELD_UY ? 1 : fid == DNS_FIELD_UZ ? 2 : 3;
+        // exchange_halo! order: x, then y, then z; slabs span the ghosts
+        for (int k = 0; k < GZ; ++k)
+            for (int j = 0; j < GY; ++j) {
+                h[G(0, j, k)] = h[G(nx, j, k)];
+                h[G(nx + 1, j, k)] = h[G(1, j, k)];
+            }
+        for (int k = 0; k < GZ; ++k)
+            for (int i = 0; i < GX; ++i) {
+                if (!g.wall) {
+                    h[G(i, 0, k)] = h[G(i, ny, k)];
+                    h[G(i, ny + 1, k)] = h[G(i, 1, k)];
+                } else if (stag == 1) {
+                    h[G(i, 0, k)] = 0.0;
+                    h[G(i, ny, k)] = 0.0;
+                    h[G(i, ny + 1, k)] = -h[G(i, ny - 1, k)];
+                } else {
+                    double sg = stag == 3 ? 1.0 : -1.0;
+                    h[G(i, 0, k)] = sg * h[G(i, 1, k)];
+                    h[G(i, ny + 1, k)] = sg * h[G(i, ny, k)];
+                }
+            }
+        for (int j = 0; j < GY; ++j)
+            for (int i = 0; i < GX; ++i) {
+                h[G(i, j, 0)] = h[G(i, j, nz)];
+                h[G(i, j, nz + 1)] = h[G(i, j, 1)];
+            }
+        return DNS_OK;
+    }
+
+    // ---------------------------------------------------------- kernels
+    int launch_momentum(T **uu, T **vv, double Ac, double dt, double B, bool write_v)
+    {
+        MomentumArgs<T> a;
+        a.ux = uu[0];
+        a.uy = uu[1];
+        a.uz = uu[2];
+        a.qx = q[0];
+        a.qy = q[1];
+        a.qz = q[2];
+        a.vx = vv ? vv[0] : nullptr;
+        a.vy = vv ? vv[1] : nullptr;
+        a.vz = vv ? vv[2] : nullptr;
+        a.dy = dy_t;
+        a.rdy = rdy_t;
+        a.rdyu = rdyu_t;
+        a.A = (T)Ac;
+        a.dt = (T)dt;
+        a.nu = (T)visc;
+        a.fx = (T)f[0];
+        a.fy = (T)f[1];
+        a.fz = (T)f[2];
+        a.epsy = (T)(imexy ? 0.0 : 1.0);
+        a.B = (T)B;
+        a.rdx = (T)(1.0 / dx);
+        a.rdx2 = (T)(1.0 / (dx * dx));
+        a.rdz = (T)(1.0 / dz);
+        a.rdz2 = (T)(1.0 / (dz * dz));
+        dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
+        timer.begin(FAM_MOM);
+        const bool rq = Ac != 0.0;
+        if (write_v) {
+            if (rq) hipLaunchKernelGGL((k_momentum<T, true, true>), grd, blk, 0, st, a, g);
+            else hipLaunchKernelGGL((k_momentum<T, false, true>), grd, blk, 0, st, a, g);
+        } else {
+            if (rq) hipLaunchKernelGGL((k_momentum<T, true, false>), grd, blk, 0, st, a, g);
+            else hipLaunchKernelGGL((k_momentum<T, false, false>), grd, blk, 0, st, a, g);
+        }
+        timer.end(FAM_MOM);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    // poisson! of the divergence of uu, writing the pressure into psi
+    // (src/poisson.jl:147-161), without transposes: x pass, z pass, y solve,
+    // z pass, x pass.
+    int launch_poisson(T **uu, T *psi, int mode /*0 = divergence of uu*/)
+    {
+        XPassArgs<T> xa;
+        xa.ux = uu[0];
+        xa.uy = uu[1];
+        xa.uz = uu[2];
+        xa.A = A;
+        xa.rdy = rdy_t;
+        xa.rdx = (T)(1.0 / dx);
+        xa.rdz = (T)(1.0 / dz);
+        xa.tw = twN;
+        xa.twx = twx;
+        xa.lpb = lpb;
+        const int nlines = g.ny * g.nz;
+        timer.begin(FAM_X);
+        hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 0>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xa, g, fdx);
+        timer.end(FAM_X);
+        DNS_HIP_CHECK(hipGetLastError());
+        timer.begin(FAM_Z);
+        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
+                           twz, tbz, g, fdz);
+        timer.end(FAM_Z);
+        DNS_HIP_CHECK(hipGetLastError());
+        timer.begin(FAM_Y);
+        if (g.wall) {
+            double scale = 1.0 / ((double)g.nx * g.nz);
+            int lines = g.nxc * g.nz;
+            hipLaunchKernelGGL((k_thomas_y<T>), dim3(ceil_div(lines, NTHR)), dim3(NTHR), 0, st, A, Dth, iptab, ay,
+                               cy, lam1, lam3, scale, g);
+        } else {
+            double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,
+                               st, A, twy, lam1, lam2, lam3, scale, tby, g, fdy);
+        }
+        timer.end(FAM_Y);
+        DNS_HIP_CHECK(hipGetLastError());
+        timer.begin(FAM_Z);
+        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
+                           twz, tbz, g, fdz);
+        timer.end(FAM_Z);
+        DNS_HIP_CHECK(hipGetLastError());
+        XInvArgs<T> xi;
+        xi.A = A;
+        xi.out = psi;
+        xi.tw = twN;
+        xi.twx = twx;
+        xi.lpb = lpb;
+        timer.begin(FAM_X);
+        hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
+        timer.end(FAM_X);
+        DNS_HIP_CHECK(hipGetLastError());
+        (void)mode;
+        return DNS_OK;
+    }
+
+    int launch_pgrad(T **uu, T *psi, bool pupdate, double inv_adt)
+    {
+        PGradArgs<T> a;
+        a.ux = uu[0];
+        a.uy = uu[1];
+        a.uz = uu[2];
+        a.psi = psi;
+        a.p = p;
+        a.rdyu = rdyu_t;
+        a.rdx = (T)(1.0 / dx);
+        a.rdz = (T)(1.0 / dz);
+        a.s = T(1);
+        a.inv_adt = (T)inv_adt;
+        dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
+        timer.begin(FAM_PGRAD);
+        if (pupdate) hipLaunchKernelGGL((k_pgrad<T, true>), grd, blk, 0, st, a, g);
+        else hipLaunchKernelGGL((k_pgrad<T, false>), grd, blk, 0, st, a, g);
+        timer.end(FAM_PGRAD);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int launch_cn(double B, double alpha, double dt)
+    {
+        const double theta = alpha * dt * visc / 2;
+        timer.begin(FAM_CN);
+        hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(64), 0, st, ydc[0], ydc[1], ydc[2], ydc[3], ydc[4], ydc[5], theta,
+                           g.ny, cncoef);
+        for (int c = 0; c < 3; ++c) {
+            CNArgs<T> a;
+            a.u = u[c];
+            a.q = q[c];
+            a.p = p;
+            a.D = Dcn;
+            const int set = c == 1 ? 1 : 0;
+            a.a = ydc[3 * set + 0];
+            a.b = ydc[3 * set + 1];
+            a.c = ydc[3 * set + 2];
+            a.cp = cncoef + set * 2 * g.ny;
+            a.ip = a.cp + g.ny;
+            a.rdyu = rdyu_d;
+            a.B = B;
+            a.adt = alpha * dt;
+            a.theta = theta;
+            a.rdx = 1.0 / dx;
+            a.rdz = 1.0 / dz;
+            dim3 blk(64), grd(ceil_div(g.nx, 64), g.nz);
+            if (c == 0) hipLaunchKernelGGL((k_cn<T, 0>), grd, blk, 0, st, a, g);
+            else if (c == 1) hipLaunchKernelGGL((k_cn<T, 1>), grd, blk, 0, st, a, g);
+            else hipLaunchKernelGGL((k_cn<T, 2>), grd, blk, 0, st, a, g);
+        }
+        timer.end(FAM_CN);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int momentum(double Ac, double dt) override { return launch_momentum(u, nullptr, Ac, dt, 0.0, false); }
+
+    int project(int psi_field) override
+    {
+        T *psi = psi_field == DNS_FIELD_DELTA ? (delta ? delta : p) : p;
+        DNS_TRY(launch_poisson(u, psi, 0));
+        DNS_TRY(launch_pgrad(u, psi, false, 0.0));
+        return DNS_OK;
+    }
+
+    int step(double dt) override
+    {
+        for (int s = 0; s < 3; ++s) {
+            if (!imexy) {
+                DNS_TRY(launch_momentum(u, v, RK_A[s], dt, RK_B[s], true));
+                DNS_TRY(launch_poisson(v, p, 0));
+                DNS_TRY(launch_pgrad(v, p, false, 0.0));
+                for (int c = 0; c < 3; ++c) std::swap(u[c], v[c]);
+            } else {
+                DNS_TRY(launch_momentum(u, nullptr, RK_A[s], dt, 0.0, false));
+                DNS_TRY(launch_cn(RK_B[s], RK_ALPHA[s], dt));
+                DNS_TRY(launch_poisson(u, delta, 0));
+                DNS_TRY(launch_pgrad(u, delta, true, 1.0 / (RK_ALPHA[s] * dt)));
+            }
+        }
+        return DNS_OK;
+    }
+
+    int read_back(size_t count)
+    {
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, redout, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+
+    int cfl(double cflv, double *dt) override
+    {
+        const int nb = std::min(RED_BLOCKS, ceil_div(N, NTHR));
+        hipLaunchKernelGGL((k_maxabs3<T>), dim3(nb), dim3(NTHR), 0, st, u[0], u[1], u[2], N, red);
+        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, red, nb, 3, redout);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_TRY(read_back(3));
+        const double eps = 2.220446049250313e-16;
+        double conv = std::min({dx / (pinned[0] + eps), dymin / (pinned[1] + eps), dz / (pinned[2] + eps)});
+        double invdy2 = imexy ? 0.0 : 1 / (dymin * dymin);
+        double diff = 1 / (2 * visc * (1 / (dx * dx) + invdy2 + 1 / (dz * dz)) + eps);
+        *dt = std::min(cflv * conv, cflv * diff);
+        return DNS_OK;
+    }
+
+    int maxdiv(double *out) override
+    {
+        const int nb = std::min(RED_BLOCKS, ceil_div(N, NTHR));
+        hipLaunchKernelGGL((k_maxdiv<T>), dim3(nb), dim3(NTHR), 0, st, u[0], u[1], u[2], rdy_d, 1.0 / dx, 1.0 / dz, g,
+                           red);
+        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, red, nb, 1, redout);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_TRY(read_back(1));
+        *out = pinned[0];
+        return DNS_OK;
+    }
+
+    int kinetic(double *out) override
+    {
+        const int nb = std::min(RED_BLOCKS, ceil_div(N, NTHR));
+        hipLaunchKernelGGL((k_kinetic<T>), dim3(nb), dim3(NTHR), 0, st, u[0], u[1], u[2], dy_d, dyu_d, dx * dz, g,
+                           reinterpret_cast<dd *>(red));
+        hipLaunchKernelGGL(k_dd_final, dim3(1), dim3(NTHR), 0, st, reinterpret_cast<dd *>(red), nb, 1, redout);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_TRY(read_back(1));
+        *out = pinned[0];
+        return DNS_OK;
+    }
+
+    int chstats(double *acc) override
+    {
+        const int ny = g.ny, nf = g.wall ? ny - 1 : ny;
+        hipLaunchKernelGGL((k_chstats<T>), dim3(ny, CH_NCHUNK), dim3(NTHR), 0, st, u[0], u[1], u[2], g, nf, CH_NCHUNK,
+                           reinterpret_cast<dd *>(red));
+        hipLaunchKernelGGL(k_dd_final, dim3(std::min(8 * ny, 1024)), dim3(NTHR), 0, st, reinterpret_cast<dd *>(red),
+                           CH_NCHUNK, 8 * ny, redout);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_TRY(read_back((size_t)8 * ny));
+        for (int s = 0; s < 4; ++s)
+            for (int j = 0; j < ny; ++j) acc[s * ny + j] += pinned[s * ny + j];
+        for (int s = 0; s < 4; ++s)
+            for (int j = 0; j < nf; ++j) acc[4 * ny + s * nf + j] += pinned[(4 + s) * ny + j];
+        return DNS_OK;
+    }
+
+    int spectrum(double *E, int nbins) override
+    {
+        if (g.wall) {
+            set_error("spectra require a fully periodic box");
+            return DNS_EINVAL;
+        }
+        if (nbins <= 0 || nbins > 16384) {
+            set_error("nbins out of range");
+            return DNS_EINVAL;
+        }
+        const int NC = g.nxc * g.ny * g.nz;
+        const int nb = std::min(256, ceil_div(NC, NTHR));
+        size_t need = (size_t)nb * nbins;
+        if (need > redcap) {
+            hipStreamSynchronize(st);
+            hipFree(red);
+            redcap = need;
+            DNS_TRY(dalloc(&red, redcap));
+        }
+        std::vector<double> hostE(nbins);
+        double *dout = nullptr;
+        DNS_TRY(dalloc(&dout, nbins));
+        const double N2 = (double)N * (double)N;
+        for (int c = 0; c < 3; ++c) {
+            XPassArgs<T> xa;
+            xa.ux = u[c];
+            xa.uy = u[c];
+            xa.uz = u[c];
+            xa.A = A;
+            xa.rdy = rdy_t;
+            xa.rdx = T(0);
+            xa.rdz = T(0);
+            xa.tw = twN;
+            xa.twx = twx;
+            xa.lpb = lpb;
+            hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(g.ny * g.nz, lpb)), dim3(NTHR), ldsx, st, xa, g,
+                               fdx);
+            hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
+                               twz, tbz, g, fdz);
+            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,
+                               st, A, twy, lam1, lam2, lam3, 1.0, tby, g, fdy);
+            hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR), sizeof(double) * nbins, st, A, g, nbins,
+                               1.0 / N2, red);
+            hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nbins, NTHR)), dim3(NTHR), 0, st, red, nb, nbins, dout);
+            DNS_HIP_CHECK(hipGetLastError());
+            DNS_HIP_CHECK(hipMemcpyAsync(hostE.data(), dout, sizeof(double) * nbins, hipMemcpyDeviceToHost, st));
+            DNS_HIP_CHECK(hipStreamSynchronize(st));
+            for (int b = 0; b < nbins; ++b) E[b] += hostE[b];
+        }
+        hipFree(dout);
+        return DNS_OK;
+    }
+
+    int sync() override
+    {
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+};
+
+}  // namespace dns
+
+struct dns_fv {
+    std::unique_ptr<dns::FVBase> impl;
+};
+
+using namespace dns;
+
+extern "C" {
+
+const char *dns_last_error(void) { return dns::last_error(); }
+
+const char *dns_version(void) { return "libdnsrocm 0.1 (gfx950)"; }
+
+int dns_fv_create(const dns_fv_params *P, dns_fv **out)
+{
+    if (!out) {
+        set_error("null output handle");
+        return DNS_EINVAL;
+    }
+    *out = nullptr;
+    if (!P || !P->yfaces) {
+        set_error("null params or yfaces");
+        return DNS_EINVAL;
+    }
+    if (P->n[0] < 2 || P->n[1] < 2 || P->n[2] < 2) {
+        set_error("local block smaller than ghost width; use fewer ranks");
+        return DNS_EINVAL;
+    }
+    if ((long long)P->n[0] * P->n[1] * P->n[2] >= (1LL << 31) ||
+        (long long)(P->n[0] / 2 + 1) * P->n[1] * P->n[2] >= (1LL << 31)) {
+        set_error("grid too large for 32-bit indexing");
+        return DNS_EUNSUPPORTED;
+    }
+    if (P->ydiffusion && !P->ywall) {
+        set_error("implicit y-diffusion requires bc[2] == :wall");
+        return DNS_EINVAL;
+    }
+    if (P->precision != DNS_FP64 && P->precision != DNS_MIXED) {
+        set_error("precision must be DNS_FP64 or DNS_MIXED");
+        return DNS_EINVAL;
+    }
+    std::unique_ptr<dns_fv> h(new dns_fv);
+    int rc;
+    if (P->precision == DNS_FP64) {
+        auto *impl = new FVImpl<double>();
+        h->impl.reset(impl);
+        rc = impl->create(P);
+    } else {
+        auto *impl = new FVImpl<float>();
+        h->impl.reset(impl);
+        rc = impl->create(P);
+    }
+    if (rc != DNS_OK) return rc;
+    *out = h.release();
+    return DNS_OK;
+}
+
+int dns_fv_destroy(dns_fv *h)
+{
+    delete h;
+    return DNS_OK;
+}
+
+#define FV_CHECK(h)                         \
+    do {                                    \
+        if (!(h) || !(h)->impl) {           \
+            set_error("null dns_fv handle"); \
+            return DNS_EINVAL;              \
+        }                                   \
+    } while (0)
+
+int dns_fv_set_field(dns_fv *h, int32_t field, const double *host, int32_t layout)
+{
+    FV_CHECK(h);
+    return h->impl->set_field(field, host, layout);
+}
+int dns_fv_get_field(dns_fv *h, int32_t field, double *host, int32_t layout)
+{
+    FV_CHECK(h);
+    return h->impl->get_field(field, host, layout);
+}
+int dns_fv_momentum(dns_fv *h, double A, double dt)
+{
+    FV_CHECK(h);
+    return h->impl->momentum(A, dt);
+}
+int dns_fv_project(dns_fv *h, int32_t psi)
+{
+    FV_CHECK(h);
+    if (psi != DNS_FIELD_P && psi != DNS_FIELD_DELTA) {
+        set_error("psi must be DNS_FIELD_P or DNS_FIELD_DELTA");
+        return DNS_EINVAL;
+    }
+    return h->impl->project(psi);
+}
+int dns_fv_step(dns_fv *h, double dt)
+{
+    FV_CHECK(h);
+    return h->impl->step(dt);
+}
+int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt)
+{
+    FV_CHECK(h);
+    if (!dt) return DNS_EINVAL;
+    return h->impl->cfl(cfl, dt);
+}
+int dns_fv_maxdiv(dns_fv *h, double *out)
+{
+    FV_CHECK(h);
+    if (!out) return DNS_EINVAL;
+    return h->impl->maxdiv(out);
+}
+int dns_fv_kinetic_energy(dns_fv *h, double *out)
+{
+    FV_CHECK(h);
+    if (!out) return DNS_EINVAL;
+    return h->impl->kinetic(out);
+}
+int dns_fv_channelstats_sample(dns_fv *h, double *acc)
+{
+    FV_CHECK(h);
+    if (!acc) return DNS_EINVAL;
+    return h->impl->chstats(acc);
+}
+int dns_fv_spectrum_sample(dns_fv *h, double *E, int32_t nbins)
+{
+    FV_CHECK(h);
+    if (!E) return DNS_EINVAL;
+    return h->impl->spectrum(E, nbins);
+}
+int dns_fv_sync(dns_fv *h)
+{
+    FV_CHECK(h);
+    return h->impl->sync();
+}
+int dns_fv_timing(dns_fv *h, int32_t enable)
+{
+    FV_CHECK(h);
+    h->impl->timer.flush();
+    for (int f = 0; f < 8; ++f) {
+        h->impl->timer.ms[f] = 0;
+        h->impl->timer.cnt[f] = 0;
+    }
+    h->impl->timer.on = enable != 0;
+    return DNS_OK;
+}
+int dns_fv_timing_read(dns_fv *h, int32_t family, double *total_ms, int64_t *launches)
+{
+    FV_CHECK(h);
+    if (family < 0 || family >= 8 || !total_ms || !launches) return DNS_EINVAL;
+    h->impl->timer.flush();
+    *total_ms = h->impl->timer.ms[family];
+    *launches = h->impl->timer.cnt[family];
+    return DNS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,699 @@
+// Device kernels of the staggered finite-volume time step (gfx950).
+//
+// Storage: every field is an interior-only column-major array (x fastest)
+// of extent (nx, ny, nz); there are no ghost layers in HBM.  The ghost
+// values the reference keeps in its (nx+2, ny+2, nz+2) arrays
+// (src/halo.jl:19-45, 81-133) are produced inline by the load helpers below:
+// periodic directions wrap, and at walls cell-centred velocities mirror
+// antisymmetrically, the pressure mirrors symmetrically and the wall-normal
+// velocity is zero on the wall faces with an antisymmetric ghost face.
+// y metric arrays keep the reference's ghosted indexing: dy[jj], dyu[jj] for
+// jj = 0..ny+1 equal Δy[J], Δyu[J] at J = jj + 1 (src/setup.jl:148-193).
+// u.y row j stores face j+1 (the top face of cell j+1); with walls the last
+// row is the top wall face and always holds zero.
+#pragma once
+
+#include "common.hpp"
+#include "fft_lds.hpp"
+
+namespace dns {
+
+struct FVGeom {
+    int nx, ny, nz, nxc;
+    int wall;
+};
+
+__device__ inline int IX(int i, int j, int k, const FVGeom &g) { return i + g.nx * (j + g.ny * k); }
+__device__ inline int CX(int i, int j, int k, const FVGeom &g) { return i + g.nxc * (j + g.ny * k); }
+__device__ inline int wrapm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ inline int wrapp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+// Cell-centred field at ghosted row jj (0..ny+1); s = -1 for tangential
+// velocities (no-slip mirror), +1 for the pressure (Neumann mirror).
+template <typename T>
+__device__ inline T ldc(const T *__restrict__ f, int i, int jj, int k, const FVGeom &g, T s)
+{
+    if (jj < 1) {
+        if (g.wall) return s * f[IX(i, 0, k, g)];
+        jj = g.ny;
+    } else if (jj > g.ny) {
+        if (g.wall) return s * f[IX(i, g.ny - 1, k, g)];
+        jj = 1;
+    }
+    return f[IX(i, jj - 1, k, g)];
+}
+
+// Wall-normal face field at ghosted face index jj (0..ny+1).
+template <typename T> __device__ inline T ldf(const T *__restrict__ f, int i, int jj, int k, const FVGeom &g)
+{
+    if (jj < 1) {
+        if (g.wall) return T(0);
+        jj = g.ny;
+    } else if (jj > g.ny) {
+        if (g.wall) return -f[IX(i, g.ny - 2, k, g)];
+        jj = 1;
+    }
+    return f[IX(i, jj - 1, k, g)];
+}
+
+// ------------------------------------------------------------------------
+// Fused momentum right-hand side and Williamson register update
+// (src/operators.jl:44-178 and src/timestep.jl:54-58):
+//   q ← A q + Δt (−C(u) + ν D(u) + f)        all three components
+//   v ← u + B q                               (explicit mode, WRITE_V)
+// One thread per cell; block (64, 4), grid (nx/64, ny/4, nz).
+// ------------------------------------------------------------------------
+template <typename T> struct MomentumArgs {
+    const T *ux, *uy, *uz;
+    T *qx, *qy, *qz;
+    T *vx, *vy, *vz;
+    const T *dy, *rdy, *rdyu; // ghosted y metrics and reciprocals
+    T A, dt, nu, fx, fy, fz, epsy, B;
+    T rdx, rdx2, rdz, rdz2;
+};
+
+template <typename T, bool READ_Q, bool WRITE_V>
+__global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
+    const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
+    const int jj = j + 1;
+    const T m1 = T(-1);
+    const T *ux = a.ux, *uy = a.uy, *uz = a.uz;
+
+    const T ux000 = ux[IX(i, j, k, g)], uxp00 = ux[IX(ip, j, k, g)], uxm00 = ux[IX(im, j, k, g)];
+    const T ux0p0 = ldc(ux, i, jj + 1, k, g, m1), ux0m0 = ldc(ux, i, jj - 1, k, g, m1);
+    const T ux00p = ux[IX(i, j, kp, g)], ux00m = ux[IX(i, j, km, g)];
+    const T uxmp0 = ldc(ux, im, jj + 1, k, g, m1), uxm0p = ux[IX(im, j, kp, g)];
+
+    const T uy000 = ldf(uy, i, jj, k, g), uyp00 = ldf(uy, ip, jj, k, g), uym00 = ldf(uy, im, jj, k, g);
+    const T uy0p0 = ldf(uy, i, jj + 1, k, g), uy0m0 = ldf(uy, i, jj - 1, k, g);
+    const T uy00p = ldf(uy, i, jj, kp, g), uy00m = ldf(uy, i, jj, km, g);
+    const T uypm0 = ldf(uy, ip, jj - 1, k, g), uy0mp = ldf(uy, i, jj - 1, kp, g);
+
+    const T uz000 = uz[IX(i, j, k, g)], uzp00 = uz[IX(ip, j, k, g)], uzm00 = uz[IX(im, j, k, g)];
+    const T uz0p0 = ldc(uz, i, jj + 1, k, g, m1), uz0m0 = ldc(uz, i, jj - 1, k, g, m1);
+    const T uz00p = uz[IX(i, j, kp, g)], uz00m = uz[IX(i, j, km, g)];
+    const T uzp0m = uz[IX(ip, j, km, g)], uz0pm = ldc(uz, i, jj + 1, km, g, m1);
+
+    const T Dyj = a.dy[jj], Dyp = a.dy[jj + 1];
+    const T rDyj = a.rdy[jj], rDyp = a.rdy[jj + 1], rDyuj = a.rdyu[jj], rDyum = a.rdyu[jj - 1];
+    const T h = T(0.5);
+    const int c = IX(i, j, k, g);
+
+    // x momentum at the right x-face of the cell
+    {
+        T uce = (ux000 + uxp00) * h, ucw = (uxm00 + ux000) * h;
+        T vn = (uy000 + uyp00) * h, un = (ux000 + ux0p0) * h;
+        T vs = (uy0m0 + uypm0) * h, us = (ux0m0 + ux000) * h;
+        T wt = (uz000 + uzp00) * h, ut = (ux000 + ux00p) * h;
+        T wb = (uz00m + uzp0m) * h, ub = (ux00m + ux000) * h;
+        T conv = (uce * uce - ucw * ucw) * a.rdx + (vn * un - vs * us) * rDyj + (wt * ut - wb * ub) * a.rdz;
+        T diff = (uxp00 - T(2) * ux000 + uxm00) * a.rdx2 +
+                 a.epsy * ((ux0p0 - ux000) * rDyuj - (ux000 - ux0m0) * rDyum) * rDyj +
+                 (ux00p - T(2) * ux000 + ux00m) * a.rdz2;
+        T q = a.dt * (-conv + a.nu * diff + a.fx);
+        if (READ_Q) q += a.A * a.qx[c];
+        a.qx[c] = q;
+        if (WRITE_V) a.vx[c] = ux000 + a.B * q;
+    }
+    // y momentum at the top y-face; the staggered volume straddles cells
+    // jj and jj+1, hence the Δy-weighted transporting velocities
+    {
+        const bool active = !(g.wall && j == g.ny - 1);
+        if (active) {
+            T hw = h * rDyuj;
+            T ue = (ux000 * Dyj + ux0p0 * Dyp) * hw, ve = (uy000 + uyp00) * h;
+            T uw = (uxm00 * Dyj + uxmp0 * Dyp) * hw, vw = (uym00 + uy000) * h;
+            T vcn = (uy000 + uy0p0) * h, vcs = (uy0m0 + uy000) * h;
+            T wtv = (uz000 * Dyj + uz0p0 * Dyp) * hw, vt = (uy000 + uy00p) * h;
+            T wbv = (uz00m * Dyj + uz0pm * Dyp) * hw, vb = (uy00m + uy000) * h;
+            T conv = (ue * ve - uw * vw) * a.rdx + (vcn * vcn - vcs * vcs) * rDyuj + (wtv * vt - wbv * vb) * a.rdz;
+            T diff = (uyp00 - T(2) * uy000 + uym00) * a.rdx2 +
+                     a.epsy * ((uy0p0 - uy000) * rDyp - (uy000 - uy0m0) * rDyj) * rDyuj +
+                     (uy00p - T(2) * uy000 + uy00m) * a.rdz2;
+            T q = a.dt * (-conv + a.nu * diff + a.fy);
+            if (READ_Q) q += a.A * a.qy[c];
+            a.qy[c] = q;
+            if (WRITE_V) a.vy[c] = uy000 + a.B * q;
+        } else if (WRITE_V) {
+            a.vy[c] = uy000;
+        }
+    }
+    // z momentum at the back z-face
+    {
+        T ue = (ux000 + ux00p) * h, we = (uz000 + uzp00) * h;
+        T uw = (uxm00 + uxm0p) * h, ww = (uzm00 + uz000) * h;
+        T vnw = (uy000 + uy00p) * h, wn = (uz000 + uz0p0) * h;
+        T vsw = (uy0m0 + uy0mp) * h, ws = (uz0m0 + uz000) * h;
+        T wct = (uz000 + uz00p) * h, wcb = (uz00m + uz000) * h;
+        T conv = (ue * we - uw * ww) * a.rdx + (vnw * wn - vsw * ws) * rDyj + (wct * wct - wcb * wcb) * a.rdz;
+        T diff = (uzp00 - T(2) * uz000 + uzm00) * a.rdx2 +
+                 a.epsy * ((uz0p0 - uz000) * rDyuj - (uz000 - uz0m0) * rDyum) * rDyj +
+                 (uz00p - T(2) * uz000 + uz00m) * a.rdz2;
+        T q = a.dt * (-conv + a.nu * diff + a.fz);
+        if (READ_Q) q += a.A * a.qz[c];
+        a.qz[c] = q;
+        if (WRITE_V) a.vz[c] = uz000 + a.B * q;
+    }
+}
+
+// ------------------------------------------------------------------------
+// x pass, forward: builds the real x-lines (the backward-difference
+// divergence of src/operators.jl:10-26, MODE 0, or a plain field copy,
+// MODE 1), transforms them with a half-length complex FFT in LDS and
+// unpacks the nx/2+1 rfft coefficients into A(kx, j, k).
+// Lines L = j + ny k; `lpb` lines per block; d.n = nx/2.
+// twx[t] = exp(-2πi t / nx), t = 0..nx/2.
+// ------------------------------------------------------------------------
+template <typename T> struct XPassArgs {
+    const T *ux, *uy, *uz; // MODE 0: velocity; MODE 1: ux is the field
+    cplx<T> *A;
+    const T *rdy;
+    T rdx, rdz;
+    const cplx<T> *tw;  // length nx/2
+    const cplx<T> *twx; // length nx/2+1
+    int lpb;
+};
+
+template <typename T, int EPT, int MODE>
+__global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int N = d.n, lpb = a.lpb;
+    const int nlines = g.ny * g.nz;
+    const int L0 = blockIdx.x * lpb;
+    for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
+        int b = idx / N, t = idx - b * N;
+        int L = L0 + b;
+        cplx<T> z = mk<T>(T(0), T(0));
+        if (L < nlines) {
+            int j = L % g.ny, k = L / g.ny;
+            int i0 = 2 * t, i1 = 2 * t + 1;
+            if (MODE == 0) {
+                int km = wrapm(k, g.nz), jj = j + 1;
+                T rdyj = a.rdy[jj];
+                T x0 = a.ux[IX(i0, j, k, g)], x1 = a.ux[IX(i1, j, k, g)], xm = a.ux[IX(wrapm(i0, g.nx), j, k, g)];
+                T y0 = ldf(a.uy, i0, jj, k, g) - ldf(a.uy, i0, jj - 1, k, g);
+                T y1 = ldf(a.uy, i1, jj, k, g) - ldf(a.uy, i1, jj - 1, k, g);
+                T z0 = a.uz[IX(i0, j, k, g)] - a.uz[IX(i0, j, km, g)];
+                T z1 = a.uz[IX(i1, j, k, g)] - a.uz[IX(i1, j, km, g)];
+                z.x = (x0 - xm) * a.rdx + y0 * rdyj + z0 * a.rdz;
+                z.y = (x1 - x0) * a.rdx + y1 * rdyj + z1 * a.rdz;
+            } else {
+                z.x = a.ux[IX(i0, j, k, g)];
+                z.y = a.ux[IX(i1, j, k, g)];
+            }
+        }
+        buf[idx] = z;
+    }
+    __syncthreads();
+    fft_lds<T, EPT, false>(buf, lpb, N, d, a.tw);
+    const int nxc = N + 1;
+    for (int idx = threadIdx.x; idx < lpb * nxc; idx += blockDim.x) {
+        int b = idx / nxc, kx = idx - b * nxc;
+        int L = L0 + b;
+        if (L >= nlines) continue;
+        cplx<T> Zk = buf[b * N + (kx == N ? 0 : kx)];
+        cplx<T> Zc = cconj<T>(buf[b * N + (kx == 0 ? 0 : N - kx)]);
+        cplx<T> s = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
+        // X = (Zk + Zc)/2 − (i/2) w^kx (Zk − Zc)
+        cplx<T> wd = cmul<T>(a.twx[kx], df);
+        cplx<T> X = mk<T>(T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x));
+        a.A[kx + nxc * L] = X;
+    }
+}
+
+// ------------------------------------------------------------------------
+// x pass, backward (c2r): A(kx, j, k) → real lines out(i, j, k) as the
+// unnormalised backward real transform (FFTW c2r semantics: imaginary parts
+// of the kx = 0 and kx = nx/2 coefficients are ignored).
+// ------------------------------------------------------------------------
+template <typename T> struct XInvArgs {
+    const cplx<T> *A;
+    T *out;
+    const cplx<T> *tw;  // length nx/2
+    const cplx<T> *twx; // length nx/2+1
+    int lpb;
+};
+
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int N = d.n, lpb = a.lpb, nxc = N + 1;
+    const int nlines = g.ny * g.nz;
+    const int L0 = blockIdx.x * lpb;
+    for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
+        int b = idx / N, k = idx - b * N;
+        int L = L0 + b;
+        cplx<T> Z = mk<T>(T(0), T(0));
+        if (L < nlines) {
+            cplx<T> Xk = a.A[k + nxc * L];
+            cplx<T> Xn = a.A[(N - k) + nxc * L];
+            if (k == 0) {
+                Xk.y = T(0);
+                Xn.y = T(0);
+            }
+            cplx<T> Xc = cconj<T>(Xn);
+            cplx<T> ev = cadd<T>(Xk, Xc);              // 2 Ev
+            cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]); // 2 Od = w^-k (Xk − Xc)
+            // Z = Ev + i Od, and brfft = 2 IDFT(Z): fold the 2 in
+            Z = mk<T>(ev.x - od.y, ev.y + od.x);
+        }
+        buf[idx] = Z;
+    }
+    __syncthreads();
+    fft_lds<T, EPT, true>(buf, lpb, N, d, a.tw);
+    for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
+        int b = idx / N, t = idx - b * N;
+        int L = L0 + b;
+        if (L >= nlines) continue;
+        cplx<T> z = buf[idx];
+        reinterpret_cast<cplx<T> *>(a.out + (size_t)g.nx * L)[t] = z;
+    }
+}
+
+// ------------------------------------------------------------------------
+// z pass: in-place complex FFT along z of A(kx, j, kz) for a tile of `tb`
+// consecutive kx at fixed j (coalesced tb·16-byte row segments).
+// ------------------------------------------------------------------------
+template <typename T, int EPT, bool INV>
+__global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, FVGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = d.n, ls = n + 1;
+    const int kx0 = blockIdx.x * tb, j = blockIdx.y;
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb;
+        int kx = kx0 + b;
+        buf[b * ls + t] = kx < g.nxc ? A[CX(kx, j, t, g)] : mk<T>(T(0), T(0));
+    }
+    __syncthreads();
+    fft_lds<T, EPT, INV>(buf, tb, ls, d, tw);
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb;
+        int kx = kx0 + b;
+        if (kx < g.nxc) A[CX(kx, j, t, g)] = buf[b * ls + t];
+    }
+}
+
+// ------------------------------------------------------------------------
+// y solve, periodic y (src/poisson.jl:43-49, 136-138): fft y, divide by
+// λ1+λ2+λ3 (mean mode → 0), ifft y, all in LDS for a tile of `tb`
+// consecutive kx at fixed kz.  `scale` folds the 1/(nx ny nz)
+// normalisation of the inverse transforms.  With DIVIDE = false only the
+// forward y transform is applied (spectrum sampling).
+// ------------------------------------------------------------------------
+template <typename T, int EPT, bool DIVIDE>
+__global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw, const double *lam1,
+                                                   const double *lam2, const double *lam3, double scale, int tb,
+                                                   FVGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = d.n, ls = n + 1;
+    const int kx0 = blockIdx.x * tb, kz = blockIdx.y;
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb;
+        int kx = kx0 + b;
+        buf[b * ls + t] = kx < g.nxc ? A[CX(kx, t, kz, g)] : mk<T>(T(0), T(0));
+    }
+    __syncthreads();
+    fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
+    if (DIVIDE) {
+        for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+            int t = idx / tb, b = idx - t * tb;
+            int kx = kx0 + b;
+            if (kx >= g.nxc) continue;
+            double lam = lam1[kx] + lam2[t] + lam3[kz];
+            cplx<T> v = buf[b * ls + t];
+            double f = lam == 0.0 ? 0.0 : scale / lam;
+            buf[b * ls + t] = mk<T>(T((double)v.x * f), T((double)v.y * f));
+        }
+        __syncthreads();
+        fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
+    }
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb;
+        int kx = kx0 + b;
+        if (kx < g.nxc) A[CX(kx, t, kz, g)] = buf[b * ls + t];
+    }
+}
+
+// ------------------------------------------------------------------------
+// y solve with walls (src/poisson.jl:14-41): batched Thomas for
+// (D_y + λ1 + λ3) x = r̂ in FP64, one thread per (kx, kz) line, lines
+// flattened as L = kx + nxc kz so neighbouring lanes read neighbouring kx.
+// The inverse pivots ip[kx, j, |kz|] = 1/β_j of every line are computed
+// once at setup (the operator never changes); the mean mode (λ == 0) pins
+// row 1 exactly as the reference does.  The forward sweep stores d' in the
+// FP64 scratch D; `scale` folds 1/(nx nz).
+// ------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *A, double2 *D, const double *__restrict__ ip,
+                                                  const double *__restrict__ ay, const double *__restrict__ cy,
+                                                  const double *lam1, const double *lam3, double scale, FVGeom g)
+{
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= g.nxc * g.nz) return;
+    const int kx = L % g.nxc, kz = L / g.nxc;
+    const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
+    const bool mean = (lam1[kx] + lam3[kz]) == 0.0;
+    const int ny = g.ny;
+    const size_t ipbase = (size_t)kx + (size_t)g.nxc * (size_t)ny * kzs;
+    const size_t cstride = g.nxc, base = (size_t)kx + (size_t)g.nxc * (size_t)ny * kz;
+    // forward sweep
+    double2 dp;
+    {
+        cplx<T> x = A[base];
+        double w = ip[ipbase];
+        dp = mean ? make_double2(0.0, 0.0) : make_double2((double)x.x * scale * w, (double)x.y * scale * w);
+        D[base] = dp;
+    }
+#pragma unroll 4
+    for (int j = 1; j < ny; ++j) {
+        cplx<T> x = A[base + j * cstride];
+        double w = ip[ipbase + j * cstride];
+        double a = ay[j];
+        dp = make_double2(((double)x.x * scale - a * dp.x) * w, ((double)x.y * scale - a * dp.y) * w);
+        D[base + j * cstride] = dp;
+    }
+    // back substitution, c'_j = c_j / β_j
+    double2 xn = dp;
+    A[base + (ny - 1) * cstride] = mk<T>(T(xn.x), T(xn.y));
+#pragma unroll 4
+    for (int j = ny - 2; j >= 0; --j) {
+        double2 d2 = D[base + j * cstride];
+        double cpj = (j == 0 && mean) ? 0.0 : cy[j] * ip[ipbase + j * cstride];
+        xn = make_double2(d2.x - cpj * xn.x, d2.y - cpj * xn.y);
+        A[base + j * cstride] = mk<T>(T(xn.x), T(xn.y));
+    }
+}
+
+// ------------------------------------------------------------------------
+// Pressure-gradient update on active DOFs (src/operators.jl:180-220):
+//   u_c ← u_c − s ∂ψ/∂c,  plus (CN) p ← p + ψ/(αΔt)   (src/timestep.jl:51)
+// ------------------------------------------------------------------------
+template <typename T> struct PGradArgs {
+    T *ux, *uy, *uz;
+    const T *psi;
+    T *p;
+    const T *rdyu;
+    T rdx, rdz, s, inv_adt;
+};
+
+template <typename T, bool PUPDATE>
+__global__ void __launch_bounds__(256) k_pgrad(PGradArgs<T> a, FVGeom g)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int c = IX(i, j, k, g);
+    const T pc = a.psi[c];
+    const T pxp = a.psi[IX(wrapp(i, g.nx), j, k, g)];
+    const T pzp = a.psi[IX(i, j, wrapp(k, g.nz), g)];
+    a.ux[c] -= a.s * (pxp - pc) * a.rdx;
+    if (!(g.wall && j == g.ny - 1)) {
+        const T pyp = ldc(a.psi, i, j + 2, k, g, T(1));
+        a.uy[c] -= a.s * (pyp - pc) * a.rdyu[j + 1];
+    }
+    a.uz[c] -= a.s * (pzp - pc) * a.rdz;
+    if (PUPDATE) a.p[c] += pc * a.inv_adt;
+}
+
+// ------------------------------------------------------------------------
+// Semi-implicit y-diffusion (src/ydiffusion.jl:71-154), one velocity
+// component per launch, one thread per (i, k) column, all in FP64:
+//   v   = uⁿ + B q − αΔt ∂p              (active rows only)
+//   rhs = v + θ Dy uⁿ,    θ = ½ αΔtν
+//   (I − θ Dy) u* = rhs                   Thomas, c'/β from k_cn_coeffs
+// The matrix is the same for every column, so its pivots come from one
+// tiny kernel per stage; d' goes to the FP64 scratch D.
+// COMP: 0 = x, 1 = y, 2 = z.
+// ------------------------------------------------------------------------
+template <typename T> struct CNArgs {
+    T *u;
+    const T *q, *p;
+    double *D;
+    const double *a, *b, *c;   // Dy rows
+    const double *cp, *ip;     // c'_j and 1/β_j
+    const double *rdyu;        // 1/Δyu (ghosted index)
+    double B, adt, theta, rdx, rdz;
+};
+
+template <typename T, int COMP>
+__global__ void __launch_bounds__(256) k_cn(CNArgs<T> a, FVGeom g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= g.nx) return;
+    const int ny = g.ny;
+    const int ip1 = wrapp(i, g.nx), kp1 = wrapp(k, g.nz);
+    const int jact = (COMP == 1 && g.wall) ? ny - 1 : ny; // active rows
+    auto vrow = [&](int j, double un) -> double {
+        if (j >= jact) return un;
+        const int c = IX(i, j, k, g);
+        double dpdc;
+        if (COMP == 0) dpdc = ((double)a.p[IX(ip1, j, k, g)] - (double)a.p[c]) * a.rdx;
+        else if (COMP == 1) dpdc = ((double)ldc(a.p, i, j + 2, k, g, T(1)) - (double)a.p[c]) * a.rdyu[j + 1];
+        else dpdc = ((double)a.p[IX(i, j, kp1, g)] - (double)a.p[c]) * a.rdz;
+        return un + a.B * (double)a.q[c] - a.adt * dpdc;
+    };
+    double um = 0.0, u0 = (double)a.u[IX(i, 0, k, g)];
+    double up = ny > 1 ? (double)a.u[IX(i, 1, k, g)] : 0.0;
+    double dp = 0.0;
+    for (int j = 0; j < ny; ++j) {
+        double lap = a.b[j] * u0;
+        if (j > 0) lap += a.a[j] * um;
+        if (j < ny - 1) lap += a.c[j] * up;
+        double rhs = vrow(j, u0) + a.theta * lap;
+        dp = (j == 0) ? rhs * a.ip[0] : (rhs + a.theta * a.a[j] * dp) * a.ip[j];
+        a.D[IX(i, j, k, g)] = dp;
+        um = u0;
+        u0 = up;
+        up = (j + 2 < ny) ? (double)a.u[IX(i, j + 2, k, g)] : 0.0;
+    }
+    double x = dp;
+    a.u[IX(i, ny - 1, k, g)] = T(x);
+    for (int j = ny - 2; j >= 0; --j) {
+        x = a.D[IX(i, j, k, g)] - a.cp[j] * x;
+        a.u[IX(i, j, k, g)] = T(x);
+    }
+}
+
+// Pivots of I − θ tridiag(a, b, c) for the tangential (set 0) and normal
+// (set 1) rows: out[set][0][j] = c'_j, out[set][1][j] = 1/β_j.
+__global__ void k_cn_coeffs(const double *at, const double *bt, const double *ct, const double *an,
+                            const double *bn, const double *cn, double theta, int ny, double *out)
+{
+    const int set = threadIdx.x;
+    if (set > 1) return;
+    const double *a = set ? an : at, *b = set ? bn : bt, *c = set ? cn : ct;
+    double *cp = out + set * 2 * ny, *ip = cp + ny;
+    double beta = 1.0 - theta * b[0];
+    cp[0] = -theta * c[0] / beta;
+    ip[0] = 1.0 / beta;
+    for (int j = 1; j < ny; ++j) {
+        beta = (1.0 - theta * b[j]) + theta * a[j] * cp[j - 1];
+        cp[j] = -theta * c[j] / beta;
+        ip[j] = 1.0 / beta;
+    }
+}
+
+// ------------------------------------------------------------ reductions
+
+// Per-block maxima of |u_c| over the interior (CFL, src/timestep.jl:69-80).
+template <typename T>
+__global__ void __launch_bounds__(256) k_maxabs3(const T *ux, const T *uy, const T *uz, int N, double *part)
+{
+    __shared__ double s[3][256];
+    double m0 = 0, m1 = 0, m2 = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+        m0 = fmax(m0, fabs((double)ux[t]));
+        m1 = fmax(m1, fabs((double)uy[t]));
+        m2 = fmax(m2, fabs((double)uz[t]));
+    }
+    s[0][threadIdx.x] = m0;
+    s[1][threadIdx.x] = m1;
+    s[2][threadIdx.x] = m2;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c < 3; ++c) s[c][threadIdx.x] = fmax(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Per-block max |div u| computed in FP64 (src/fields.jl:38-42).
+template <typename T>
+__global__ void __launch_bounds__(256) k_maxdiv(const T *ux, const T *uy, const T *uz, const double *rdy, double rdx,
+                                                double rdz, FVGeom g, double *part)
+{
+    __shared__ double s[256];
+    const int N = g.nx * g.ny * g.nz;
+    double m = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+        int i = t % g.nx, r = t / g.nx, j = r % g.ny, k = r / g.ny;
+        int jj = j + 1;
+        double d = ((double)ux[t] - (double)ux[IX(wrapm(i, g.nx), j, k, g)]) * rdx +
+                   ((double)ldf(uy, i, jj, k, g) - (double)ldf(uy, i, jj - 1, k, g)) * rdy[jj] +
+                   ((double)uz[t] - (double)uz[IX(i, j, wrapm(k, g.nz), g)]) * rdz;
+        m = fmax(m, fabs(d));
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Block max of `cnt` partial maxima per slot; writes out[slot].
+__global__ void k_max_final(const double *part, int nblocks, int nslots, double *out)
+{
+    __shared__ double s[256];
+    for (int slot = 0; slot < nslots; ++slot) {
+        double m = 0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmax(m, part[b * nslots + slot]);
+        s[threadIdx.x] = m;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[slot] = s[0];
+        __syncthreads();
+    }
+}
+
+// Tree-reduces a per-thread double-double in LDS; thread 0 returns the total.
+__device__ inline dd block_dd_sum(dd v, dd *s)
+{
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = dd_add(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    dd r = s[0];
+    __syncthreads();
+    return r;
+}
+
+// Kinetic energy ½ Σ u² Δx Δ2[j] Δz over active DOFs (src/fields.jl:45-60),
+// compensated: per-thread double-double, block tree, per-block partials.
+template <typename T>
+__global__ void __launch_bounds__(256) k_kinetic(const T *ux, const T *uy, const T *uz, const double *dy,
+                                                 const double *dyu, double dxdz, FVGeom g, dd *part)
+{
+    __shared__ dd s[256];
+    const int N = g.nx * g.ny * g.nz;
+    dd acc{0.0, 0.0};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+        int j = (t / g.nx) % g.ny, jj = j + 1;
+        double a = (double)ux[t], b = (double)uy[t], c = (double)uz[t];
+        double ey = (g.wall && j == g.ny - 1) ? 0.0 : b * b * dxdz * dyu[jj] / 2;
+        acc = dd_add1(acc, a * a * dxdz * dy[jj] / 2);
+        acc = dd_add1(acc, ey);
+        acc = dd_add1(acc, c * c * dxdz * dy[jj] / 2);
+    }
+    dd r = block_dd_sum(acc, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Sums nparts double-doubles per slot (slot-major: part[slot * nparts + b]).
+__global__ void k_dd_final(const dd *part, int nparts, int nslots, double *out)
+{
+    __shared__ dd s[256];
+    for (int slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        dd acc{0.0, 0.0};
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc = dd_add(acc, part[(size_t)slot * nparts + b]);
+        dd r = block_dd_sum(acc, s);
+        if (threadIdx.x == 0) out[slot] = r.hi + r.lo;
+    }
+}
+
+// Channel statistics plane sums (src/stats.jl:39-58): block (j, chunk) sums
+// over a slice of the (x, z) plane; slots (cell row j): U, W, uu, ww;
+// (face row j, j < nf): V, vv, Uf, uv.  part[(slot*ny + j)*nchunk + chunk].
+template <typename T>
+__global__ void __launch_bounds__(256) k_chstats(const T *ux, const T *uy, const T *uz, FVGeom g, int nf,
+                                                 int nchunk, dd *part)
+{
+    __shared__ dd s[256];
+    const int j = blockIdx.x, chunk = blockIdx.y;
+    const int P = g.nx * g.nz;
+    dd acc[8];
+    for (int q = 0; q < 8; ++q) acc[q] = dd{0.0, 0.0};
+    const bool face = j < nf;
+    for (int t = chunk * blockDim.x + threadIdx.x; t < P; t += nchunk * blockDim.x) {
+        int i = t % g.nx, k = t / g.nx;
+        int c = IX(i, j, k, g);
+        double a = (double)ux[c], w = (double)uz[c];
+        acc[0] = dd_add1(acc[0], a);
+        acc[1] = dd_add1(acc[1], w);
+        acc[2] = dd_add1(acc[2], a * a);
+        acc[3] = dd_add1(acc[3], w * w);
+        if (face) {
+            double v = (double)uy[c];
+            double uf = (a + (double)ldc(ux, i, j + 2, k, g, T(-1))) / 2;
+            double vx = (v + (double)uy[IX(wrapp(i, g.nx), j, k, g)]) / 2;
+            acc[4] = dd_add1(acc[4], v);
+            acc[5] = dd_add1(acc[5], v * v);
+            acc[6] = dd_add1(acc[6], uf);
+            acc[7] = dd_add1(acc[7], uf * vx);
+        }
+    }
+    for (int q = 0; q < 8; ++q) {
+        dd r = block_dd_sum(acc[q], s);
+        if (threadIdx.x == 0) part[((size_t)q * g.ny + j) * nchunk + chunk] = r;
+    }
+}
+
+// Shell binning of one transformed component (src/stats.jl:143-157):
+// κ = round(|k|), weight 2 for 0 < kx < nx/2, adds w|Â|²/2/N² to per-block
+// LDS bins; part[block * nbins + κ].
+template <typename T>
+__global__ void __launch_bounds__(256) k_binshells(const cplx<T> *A, FVGeom g, int nbins, double invN2, double *part)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *bins = reinterpret_cast<double *>(smem_raw);
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) bins[b] = 0.0;
+    __syncthreads();
+    const int N = g.nxc * g.ny * g.nz;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
+        int kx = t % g.nxc, r = t / g.nxc, j = r % g.ny, k = r / g.ny;
+        int ky = j > g.ny / 2 ? j - g.ny : j;
+        int kz = k > g.nz / 2 ? k - g.nz : k;
+        int kap = (int)rint(sqrt((double)(kx * kx + ky * ky + kz * kz)));
+        double wgt = (kx > 0 && 2 * kx < g.nx) ? 2.0 : 1.0;
+        cplx<T> a = A[t];
+        double e = wgt * ((double)a.x * (double)a.x + (double)a.y * (double)a.y) / 2 * invN2;
+        if (kap < nbins) atomicAdd(&bins[kap], e);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) part[(size_t)blockIdx.x * nbins + b] = bins[b];
+}
+
+// Sums per-block bins in block order (deterministic final pass).
+__global__ void k_bins_final(const double *part, int nblocks, int nbins, double *out)
+{
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbins; b += gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int q = 0; q < nblocks; ++q) s += part[(size_t)q * nbins + b];
+        out[b] = s;
+    }
+}
+
+}  // namespace dns

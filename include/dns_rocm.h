@@ -1,0 +1,133 @@
+/*
+ * libdnsrocm -- C ABI of the MI355X (gfx950) single-device time-step hot
+ * path of DistributedNavierStokes.
+ *
+ * Every entry point takes plain C types (handles, host pointers, sizes),
+ * returns an int status (DNS_OK or a negative DNS_E* code) and never throws.
+ * dns_last_error() returns a thread-local message for the last failure.
+ * A handle owns its device memory and one HIP stream; entry points that
+ * return host-visible values synchronise that stream.  One handle is used
+ * by one host thread at a time.
+ *
+ * Field layouts exchanged with the host are column-major (x fastest):
+ *   DNS_LAYOUT_GHOSTED  (nx+2, ny+2, nz+2), ghost width 1, exactly the
+ *                       reference's `scalarfield`/`vectorfield` arrays
+ *                       (src/fields.jl:4-9); on output the ghosts are
+ *                       filled as exchange_halo! would (src/halo.jl:81-133);
+ *                       on input ghosts are ignored.
+ *   DNS_LAYOUT_INTERIOR (nx, ny, nz).
+ */
+#ifndef DNS_ROCM_H
+#define DNS_ROCM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNS_OK 0
+#define DNS_EINVAL (-1)
+#define DNS_EUNSUPPORTED (-2)
+#define DNS_EHIP (-3)
+#define DNS_ENOMEM (-4)
+
+#define DNS_FP64 0  /* FP64 storage and arithmetic */
+#define DNS_MIXED 1 /* FP32 storage, stencils and FFTs; FP64 Thomas solves and reductions */
+
+#define DNS_LAYOUT_INTERIOR 0
+#define DNS_LAYOUT_GHOSTED 1
+
+/* Field identifiers of an FV handle. */
+#define DNS_FIELD_UX 0
+#define DNS_FIELD_UY 1
+#define DNS_FIELD_UZ 2
+#define DNS_FIELD_P 3     /* pressure (lagged pressure in semi-implicit mode) */
+#define DNS_FIELD_QX 4    /* RK register q */
+#define DNS_FIELD_QY 5
+#define DNS_FIELD_QZ 6
+#define DNS_FIELD_DELTA 7 /* projection increment δ (semi-implicit mode) */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *dns_last_error(void);
+
+/* Library version string. */
+const char *dns_version(void);
+
+/* ------------------------------------------------------------------ FV
+ * Staggered symmetry-preserving finite-volume box/channel solver. */
+
+typedef struct dns_fv dns_fv;
+
+/* Replaces the keyword arguments of `setup` (src/setup.jl:34-113).  The
+ * `stretch` closure cannot cross the ABI: the caller evaluates the global
+ * y faces yfaces[j] = lims[2][1] + Ly*stretch(j/ny), j = 0..ny
+ * (src/setup.jl:155-156), and the library builds the ghost-extended metrics
+ * of `griddata` (src/setup.jl:148-193) from them. */
+typedef struct dns_fv_params {
+    int32_t n[3];          /* cells (nx, ny, nz) */
+    double lims[6];        /* (x0, x1, y0, y1, z0, z1) */
+    int32_t ywall;         /* 0: bc[2] == :periodic, 1: bc[2] == :wall */
+    const double *yfaces;  /* ny+1 host values */
+    double visc;           /* ν */
+    double bodyforce[3];   /* f */
+    int32_t ydiffusion;    /* 0: :explicit, 1: :implicit (per-stage CN in y) */
+    int32_t precision;     /* DNS_FP64 or DNS_MIXED */
+} dns_fv_params;
+
+/* setup(; ...) -- src/setup.jl:34-113 (with plan_poisson, src/poisson.jl:167-231,
+ * and plan_ydiffusion, src/ydiffusion.jl:187-201).  Allocates all device
+ * state (u, p, q, δ and the pipeline buffers) once; fields start at zero. */
+int dns_fv_create(const dns_fv_params *params, dns_fv **out);
+int dns_fv_destroy(dns_fv *h);
+
+/* Host <-> device field copies (one component per call); ghosted output
+ * fills the halos like exchange_halo! (src/halo.jl:81-133). */
+int dns_fv_set_field(dns_fv *h, int32_t field, const double *host, int32_t layout);
+int dns_fv_get_field(dns_fv *h, int32_t field, double *host, int32_t layout);
+
+/* momentum!(q, u, A, Δt, s) -- src/operators.jl:147-178: q ← A q + Δt F(u). */
+int dns_fv_momentum(dns_fv *h, double A, double dt);
+
+/* project!(u, ψ, s) -- src/timestep.jl:25-31, with ψ = DNS_FIELD_P or
+ * DNS_FIELD_DELTA. */
+int dns_fv_project(dns_fv *h, int32_t psi_field);
+
+/* step!(u, p, q, δ, s, Δt) -- src/timestep.jl:41-63: one Williamson RK3 step
+ * (explicit or semi-implicit per the handle).  Asynchronous. */
+int dns_fv_step(dns_fv *h, double dt);
+
+/* cfl_timestep(u, s; cfl) -- src/timestep.jl:69-80. */
+int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt);
+
+/* maxdiv(u, s) -- src/fields.jl:38-42 (divergence residual in FP64). */
+int dns_fv_maxdiv(dns_fv *h, double *out);
+
+/* kinetic_energy(u, s) -- src/fields.jl:45-60 (compensated FP64 sum). */
+int dns_fv_kinetic_energy(dns_fv *h, double *out);
+
+/* One sample of sample_channelstats! (src/stats.jl:39-58): adds the (x, z)
+ * plane sums to the host accumulator acc laid out as
+ * Uc, Wc, uuc, wwc (ny values each) then Vf, vvf, Uf, uvf (nf values each),
+ * nf = ny-1 with walls, ny otherwise. */
+int dns_fv_channelstats_sample(dns_fv *h, double *acc);
+
+/* One sample of sample_spectrum! (src/stats.jl:120-157): adds ½|û|²/N² per
+ * integer shell κ = round(|k|) to E[0..nbins-1] (fully periodic boxes). */
+int dns_fv_spectrum_sample(dns_fv *h, double *E, int32_t nbins);
+
+/* Waits for all work queued on the handle's stream. */
+int dns_fv_sync(dns_fv *h);
+
+/* Kernel timing hook used by the benchmark: when enabled, the library
+ * records HIP events around every launch of the named kernel family on the
+ * handle's stream and accumulates their durations.
+ * family: 0 momentum, 1 x-pass, 2 z-pass, 3 y-solve, 4 pgrad, 5 CN. */
+int dns_fv_timing(dns_fv *h, int32_t enable);
+int dns_fv_timing_read(dns_fv *h, int32_t family, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNS_ROCM_H */

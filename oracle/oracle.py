@@ -1,0 +1,158 @@
+"""ctypes wrapper of the FP64 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py import this module; it checks the HIP library and is never part of
+the product path.  Arrays are the reference's ghosted column-major fields
+(numpy ``order="F"``, shape ``(nx+2, ny+2, nz+2)``).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ct.POINTER(ct.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ct.CDLL(path)
+        L.ofv_create.restype = ct.c_int
+        L.ofv_create.argtypes = [ct.POINTER(ct.c_int), _dp, ct.c_int, _dp, ct.c_double, _dp, ct.c_int,
+                                 ct.POINTER(ct.c_void_p)]
+        L.ofv_destroy.argtypes = [ct.c_void_p]
+        for name in ("ofv_cfl_timestep", "ofv_maxdiv", "ofv_kinetic_energy"):
+            getattr(L, name).restype = ct.c_double
+        L.ofv_rbuf.restype = _dp
+        L.ospec_splitmix64.restype = ct.c_uint64
+        L.ospec_splitmix64.argtypes = [ct.c_uint64]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need float64 F-order"
+    return a.ctypes.data_as(_dp)
+
+
+class OracleFV:
+    """The reference's FV single-rank CPU path on ghosted numpy arrays."""
+
+    def __init__(self, prob):
+        self.prob = prob
+        L = lib()
+        n = (ct.c_int * 3)(*prob.n)
+        lims = np.array([v for ab in prob.lims for v in ab], dtype=np.float64)
+        force = np.array(prob.bodyforce, dtype=np.float64)
+        h = ct.c_void_p()
+        rc = L.ofv_create(n, _p(lims), int(prob.wall), _p(prob.yfaces), prob.visc, _p(force),
+                          int(prob.implicit), ct.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"oracle create failed ({rc})")
+        self.h = h
+        self.L = L
+
+    def __del__(self):
+        try:
+            self.L.ofv_destroy(self.h)
+        except Exception:
+            pass
+
+    # fields --------------------------------------------------------------
+    def scalarfield(self):
+        return np.zeros(self.prob.ghosted_shape, dtype=np.float64, order="F")
+
+    def vectorfield(self):
+        return [self.scalarfield() for _ in range(3)]
+
+    def griddata(self):
+        ny = self.prob.n[1]
+        dy, dyu = np.zeros(ny + 2), np.zeros(ny + 2)
+        yf, yc = np.zeros(ny + 1), np.zeros(ny)
+        dymin = ct.c_double()
+        self.L.ofv_griddata(self.h, _p(dy), _p(dyu), ct.byref(dymin), _p(yf), _p(yc))
+        return dict(dy=dy, dyu=dyu, dymin=dymin.value, yfg=yf, ycg=yc)
+
+    def exchange_halo(self, phi, stag):
+        self.L.ofv_exchange_halo(self.h, _p(phi), ct.c_int(stag))
+
+    def exchange_halo_u(self, u):
+        self.L.ofv_exchange_halo_u(self.h, *[_p(a) for a in u])
+
+    def divergence(self, u):
+        r = np.zeros(self.prob.n, dtype=np.float64, order="F")
+        self.L.ofv_divergence(self.h, _p(r), *[_p(a) for a in u])
+        return r
+
+    def momentum(self, q, u, A, dt):
+        self.L.ofv_momentum(self.h, *[_p(a) for a in q], *[_p(a) for a in u], ct.c_double(A), ct.c_double(dt))
+
+    def pgrad(self, phi, p, c, s=1.0):
+        self.L.ofv_pgrad(self.h, _p(phi), _p(p), ct.c_int(c), ct.c_double(s))
+
+    def poisson_from(self, r, p):
+        """poisson!(p) with the divergence buffer set to r (interior shape)."""
+        buf = self.L.ofv_rbuf(self.h)
+        N = int(np.prod(self.prob.n))
+        np.ctypeslib.as_array(buf, shape=(N,))[:] = np.asarray(r, dtype=np.float64).ravel(order="F")
+        self.L.ofv_poisson(self.h, _p(p))
+
+    def project(self, u, psi):
+        self.L.ofv_project(self.h, *[_p(a) for a in u], _p(psi))
+
+    def step(self, u, p, q, delta, dt):
+        self.L.ofv_step(self.h, *[_p(a) for a in u], _p(p), *[_p(a) for a in q], _p(delta), ct.c_double(dt))
+
+    def cfl_timestep(self, u, cfl):
+        return self.L.ofv_cfl_timestep(self.h, *[_p(a) for a in u], ct.c_double(cfl))
+
+    def maxdiv(self, u):
+        return self.L.ofv_maxdiv(self.h, *[_p(a) for a in u])
+
+    def kinetic_energy(self, u):
+        return self.L.ofv_kinetic_energy(self.h, *[_p(a) for a in u])
+
+    def channelstats_sample(self, u, acc):
+        self.L.ofv_channelstats_sample(self.h, *[_p(a) for a in u], _p(acc))
+
+    def spectrum_sample(self, u, E):
+        self.L.ofv_spectrum_sample(self.h, *[_p(a) for a in u], _p(E), ct.c_int(len(E)))
+
+    # host loop: solve! (src/timestep.jl:92-111) ----------------------------
+    def solve(self, u, tlims, dt=None, cfl=0.5, processors=(), p=None):
+        p = self.scalarfield() if p is None else p
+        q = self.vectorfield()
+        delta = self.scalarfield() if self.prob.implicit else p
+        self.exchange_halo_u(u)
+        self.project(u, delta)
+        t, tend = float(tlims[0]), float(tlims[1])
+        n = 0
+        dts = []
+        for proc in processors:
+            proc(u, p, t, n)
+        while t < tend - 1e-10 * (abs(tend) + 1):
+            dtn = min(dt if dt is not None else self.cfl_timestep(u, cfl), tend - t)
+            self.step(u, p, q, delta, dtn)
+            dts.append(dtn)
+            t += dtn
+            n += 1
+            for proc in processors:
+                proc(u, p, t, n)
+        return dict(u=u, p=p, t=t, n=n, dts=dts)
+
+
+def splitmix64(x: int) -> int:
+    return int(lib().ospec_splitmix64(ct.c_uint64(x & 0xFFFFFFFFFFFFFFFF)))

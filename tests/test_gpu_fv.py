@@ -1,0 +1,205 @@
+"""GPU parity of the finite-volume path against the FP64 CPU oracle on the
+same seeded inputs, through the C ABI (via the package's ctypes binding).
+FP64 tolerance: 1e-11 relative (trajectories), 1e-12 (single operators)."""
+import math
+
+import numpy as np
+import pytest
+
+import distributednavierstokes_amd as dns
+from distributednavierstokes_amd.fv import Setup, ROCmBackend
+from distributednavierstokes_amd.problem import FVProblem, tanhstretch
+from fvhelpers import (caseproblem, channelproblem, fill_velocity, randomfield, relmax, vzeros, weighted_dot, zeros)
+from oracle.oracle import OracleFV
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(prob):
+    return Setup(prob, ROCmBackend()), OracleFV(prob)
+
+
+def problems():
+    return {
+        "periodic": FVProblem(n=(16, 12, 8), visc=0.01, bodyforce=(0.3, 0.1, -0.2)),
+        "channel": channelproblem(visc=0.02, bodyforce=(1.0, 0.0, 0.0)),
+        "channel_cn": channelproblem(visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit"),
+        "case_wall": caseproblem("wall", visc=0.02),
+        "case_periodic": caseproblem("periodic", visc=0.02),
+    }
+
+
+@pytest.mark.parametrize("name", ["periodic", "channel", "case_wall", "case_periodic"])
+def test_ghost_fill_matches_oracle(name):
+    prob = problems()[name]
+    s, o = pair(prob)
+    u = randomfield(prob, seed=3)
+    ref = [a.copy() for a in u]
+    o.exchange_halo_u(ref)
+    dns.exchange_halo_(u, s)
+    for c in range(3):
+        assert np.array_equal(u[c], ref[c])
+
+
+@pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
+@pytest.mark.parametrize("A", [0.0, -5 / 9])
+def test_momentum_parity(name, A):
+    prob = problems()[name]
+    s, o = pair(prob)
+    u = randomfield(prob, seed=5, halo=o.exchange_halo_u)
+    q0 = randomfield(prob, seed=6)
+    qg = [a.copy() for a in q0]
+    qo = [a.copy() for a in q0]
+    o.momentum(qo, u, A, 0.01)
+    dns.momentum_(qg, u, A, 0.01, s)
+    for c in range(3):
+        assert relmax(qg[c], qo[c]) < 1e-12, c
+
+
+@pytest.mark.parametrize("name", ["periodic", "channel", "case_wall", "case_periodic"])
+def test_projection_parity_and_divergence(name):
+    prob = problems()[name]
+    s, o = pair(prob)
+    u = randomfield(prob, seed=7, halo=o.exchange_halo_u)
+    ug = [a.copy() for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    o.project(u, po)
+    dns.project_(ug, pg, s)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-12
+    assert relmax(pg, po) < 1e-11
+    assert dns.maxdiv(ug, s) < 1e-12
+    assert abs(dns.maxdiv(ug, s) - o.maxdiv(ug)) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
+def test_step_parity(name):
+    prob = problems()[name]
+    s, o = pair(prob)
+    u = randomfield(prob, seed=9, halo=o.exchange_halo_u)
+    o.project(u, zeros(prob))
+    ug = [a.copy() for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    qo, qg = vzeros(prob), vzeros(prob)
+    do, dg = (zeros(prob), zeros(prob)) if prob.implicit else (po, pg)
+    for _ in range(3):
+        o.step(u, po, qo, do, 0.004)
+        dns.step_(ug, pg, qg, dg, s, 0.004)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11, c
+    assert relmax(pg, po) < 1e-10
+    assert dns.maxdiv(ug, s) < 1e-12
+
+
+@pytest.mark.parametrize("bcy,yd", [("periodic", "explicit"), ("wall", "explicit"), ("wall", "implicit")])
+def test_solve_with_statistics_parity(bcy, yd):
+    """runcase of test/mpitests.jl:158-172: trajectories and statistics
+    agree with the oracle to 1e-11."""
+    prob = caseproblem(bcy, visc=0.02, bodyforce=(0.5, 0.0, 0.0), ydiffusion=yd)
+    s, o = pair(prob)
+    fx = lambda x, y, z: math.sin(x) * math.cos(y) * math.cos(z) + 0.2 * math.cos(2 * z)  # noqa: E731
+    fy = lambda x, y, z: (1 - y ** 2) * math.sin(2 * x)  # noqa: E731
+    fz = lambda x, y, z: math.cos(x) * math.sin(z) + 0.1 * math.sin(2 * y)  # noqa: E731
+    u = vzeros(prob)
+    fill_velocity(prob, u, fx, fy, fz)
+    o.exchange_halo_u(u)
+    ug = [a.copy() for a in u]
+    if bcy == "wall":
+        ny, nf = prob.n[1], prob.n[1] - 1
+        acc = np.zeros(4 * ny + 4 * nf)
+        procs = [lambda uu, p, t, n: (n % 2 == 0) and o.channelstats_sample(uu, acc)]
+        cs = dns.channelstats(s, nupdate=2)
+        gp = {"cs": cs.sample}
+    else:
+        nb = math.ceil(math.sqrt(sum((v // 2) ** 2 for v in prob.n))) + 1
+        E = np.zeros(nb)
+        procs = [lambda uu, p, t, n: (n % 2 == 0) and o.spectrum_sample(uu, E)]
+        sp = dns.spectrumstats(s, nupdate=2)
+        gp = {"sp": sp.sample}
+    o.solve(u, (0.0, 0.05), dt=0.01, processors=procs)
+    dns.solve_(u=ug, setup=s, tlims=(0.0, 0.05), dt=0.01, processors=gp)
+    for c in range(3):
+        assert np.abs(ug[c][1:-1, 1:-1, 1:-1] - u[c][1:-1, 1:-1, 1:-1]).max() < 1e-11
+    assert dns.maxdiv(ug, s) < 1e-12
+    if bcy == "wall":
+        assert np.abs(cs.acc - acc).max() < 1e-11 * max(1.0, np.abs(acc).max())
+    else:
+        assert np.abs(sp.E - E).max() < 1e-11 * max(1.0, np.abs(E).max())
+
+
+def test_cfl_and_kinetic_energy_parity():
+    prob = channelproblem(visc=0.05, bodyforce=(1.0, 0.0, 0.0))
+    s, o = pair(prob)
+    u = randomfield(prob, seed=11, halo=o.exchange_halo_u)
+    assert math.isclose(dns.cfl_timestep(u, s, 0.5), o.cfl_timestep(u, 0.5), rel_tol=1e-15)
+    assert math.isclose(dns.kinetic_energy(u, s), o.kinetic_energy(u), rel_tol=1e-13)
+
+
+def test_skew_symmetry_on_device():
+    """⟨u, C(u)⟩ = 0 for divergence-free u (test/serialtests.jl:52-58)."""
+    prob = channelproblem(n=(32, 24, 16), visc=0.0)
+    s, o = pair(prob)
+    u = randomfield(prob, seed=12, halo=o.exchange_halo_u)
+    dns.project_(u, zeros(prob), s)
+    q = vzeros(prob)
+    dns.momentum_(q, u, 0.0, 1.0, s)
+    d, n2 = weighted_dot(prob, o.griddata(), u, q)
+    assert abs(d) < 1e-12 * n2
+
+
+def test_taylor_green_64_parity():
+    """The 64³ periodic Taylor–Green box (FP64, fixed Δt = 1e-3, Fourier y):
+    20 steps against the oracle."""
+    prob = FVProblem(n=(64, 64, 64), visc=1 / 1600)
+    s, o = pair(prob)
+    u = vzeros(prob)
+    fill_velocity(prob, u, lambda x, y, z: math.sin(x) * math.cos(y) * math.cos(z),
+                  lambda x, y, z: -math.cos(x) * math.sin(y) * math.cos(z))
+    o.exchange_halo_u(u)
+    ug = [a.copy() for a in u]
+    o.solve(u, (0.0, 0.02), dt=1e-3)
+    dns.solve_(u=ug, setup=s, tlims=(0.0, 0.02), dt=1e-3)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11
+
+
+def test_channel_cfl_trajectory_parity():
+    """Channel with walls, tanh stretching and CFL-adaptive Δt: 10 steps."""
+    prob = channelproblem(n=(32, 24, 16), lims=((0, 4 * math.pi), (-1, 1), (0, 4 * math.pi / 3)),
+                          visc=1 / 180, bodyforce=(1.0, 0.0, 0.0))
+    s, o = pair(prob)
+    u = randomfield(prob, seed=13)
+    fill_velocity(prob, u, lambda x, y, z: 1 - y ** 2)
+    o.exchange_halo_u(u)
+    ug = [a.copy() for a in u]
+    st = o.solve(u, (0.0, 0.3), cfl=0.5)
+    gst = dns.solve_(u=ug, setup=s, tlims=(0.0, 0.3), cfl=0.5)
+    assert gst.n == st["n"]
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-10
+
+
+def test_mixed_precision_error_is_small():
+    prob = channelproblem(n=(32, 24, 16), visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit")
+    prob32 = channelproblem(n=(32, 24, 16), visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit",
+                            precision="mixed")
+    s64, o = pair(prob)
+    s32 = Setup(prob32, ROCmBackend())
+    u = randomfield(prob, seed=14)
+    fill_velocity(prob, u, lambda x, y, z: 1 - y ** 2)
+    o.exchange_halo_u(u)
+    u64 = [a.copy() for a in u]
+    u32 = [a.copy() for a in u]
+    dns.solve_(u=u64, setup=s64, tlims=(0.0, 0.05), dt=0.005)
+    dns.solve_(u=u32, setup=s32, tlims=(0.0, 0.05), dt=0.005)
+    num = math.sqrt(sum(float(((u32[c] - u64[c])[1:-1, 1:-1, 1:-1] ** 2).sum()) for c in range(3)))
+    den = math.sqrt(sum(float((u64[c][1:-1, 1:-1, 1:-1] ** 2).sum()) for c in range(3)))
+    assert num / den < 1e-5
+    assert dns.maxdiv(u32, s64) < 1e-5
+
+
+def test_errors_raise():
+    with pytest.raises(dns.DNSError):
+        dns.setup(n=(8, 8, 8), bc=("periodic", "periodic", "periodic"), ydiffusion="implicit")
+    with pytest.raises(dns.DNSError):
+        dns.setup(n=(14, 8, 8))  # nx/2 = 7: unsupported radix
