@@ -34,7 +34,7 @@ def test_ghost_fill_matches_oracle(name):
     prob = problems()[name]
     s, o = pair(prob)
     u = randomfield(prob, seed=3)
-    ref = [a.copy() for a in u]
+    ref = [a.copy(order="F") for a in u]
     o.exchange_halo_u(ref)
     dns.exchange_halo_(u, s)
     for c in range(3):
@@ -48,8 +48,8 @@ def test_momentum_parity(name, A):
     s, o = pair(prob)
     u = randomfield(prob, seed=5, halo=o.exchange_halo_u)
     q0 = randomfield(prob, seed=6)
-    qg = [a.copy() for a in q0]
-    qo = [a.copy() for a in q0]
+    qg = [a.copy(order="F") for a in q0]
+    qo = [a.copy(order="F") for a in q0]
     o.momentum(qo, u, A, 0.01)
     dns.momentum_(qg, u, A, 0.01, s)
     for c in range(3):
@@ -61,7 +61,7 @@ def test_projection_parity_and_divergence(name):
     prob = problems()[name]
     s, o = pair(prob)
     u = randomfield(prob, seed=7, halo=o.exchange_halo_u)
-    ug = [a.copy() for a in u]
+    ug = [a.copy(order="F") for a in u]
     po, pg = zeros(prob), zeros(prob)
     o.project(u, po)
     dns.project_(ug, pg, s)
@@ -78,7 +78,7 @@ def test_step_parity(name):
     s, o = pair(prob)
     u = randomfield(prob, seed=9, halo=o.exchange_halo_u)
     o.project(u, zeros(prob))
-    ug = [a.copy() for a in u]
+    ug = [a.copy(order="F") for a in u]
     po, pg = zeros(prob), zeros(prob)
     qo, qg = vzeros(prob), vzeros(prob)
     do, dg = (zeros(prob), zeros(prob)) if prob.implicit else (po, pg)
@@ -103,7 +103,7 @@ def test_solve_with_statistics_parity(bcy, yd):
     u = vzeros(prob)
     fill_velocity(prob, u, fx, fy, fz)
     o.exchange_halo_u(u)
-    ug = [a.copy() for a in u]
+    ug = [a.copy(order="F") for a in u]
     if bcy == "wall":
         ny, nf = prob.n[1], prob.n[1] - 1
         acc = np.zeros(4 * ny + 4 * nf)
@@ -156,7 +156,7 @@ def test_taylor_green_64_parity():
     fill_velocity(prob, u, lambda x, y, z: math.sin(x) * math.cos(y) * math.cos(z),
                   lambda x, y, z: -math.cos(x) * math.sin(y) * math.cos(z))
     o.exchange_halo_u(u)
-    ug = [a.copy() for a in u]
+    ug = [a.copy(order="F") for a in u]
     o.solve(u, (0.0, 0.02), dt=1e-3)
     dns.solve_(u=ug, setup=s, tlims=(0.0, 0.02), dt=1e-3)
     for c in range(3):
@@ -171,7 +171,7 @@ def test_channel_cfl_trajectory_parity():
     u = randomfield(prob, seed=13)
     fill_velocity(prob, u, lambda x, y, z: 1 - y ** 2)
     o.exchange_halo_u(u)
-    ug = [a.copy() for a in u]
+    ug = [a.copy(order="F") for a in u]
     st = o.solve(u, (0.0, 0.3), cfl=0.5)
     gst = dns.solve_(u=ug, setup=s, tlims=(0.0, 0.3), cfl=0.5)
     assert gst.n == st["n"]
@@ -188,8 +188,8 @@ def test_mixed_precision_error_is_small():
     u = randomfield(prob, seed=14)
     fill_velocity(prob, u, lambda x, y, z: 1 - y ** 2)
     o.exchange_halo_u(u)
-    u64 = [a.copy() for a in u]
-    u32 = [a.copy() for a in u]
+    u64 = [a.copy(order="F") for a in u]
+    u32 = [a.copy(order="F") for a in u]
     dns.solve_(u=u64, setup=s64, tlims=(0.0, 0.05), dt=0.005)
     dns.solve_(u=u32, setup=s32, tlims=(0.0, 0.05), dt=0.005)
     num = math.sqrt(sum(float(((u32[c] - u64[c])[1:-1, 1:-1, 1:-1] ** 2).sum()) for c in range(3)))
