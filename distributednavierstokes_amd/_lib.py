@@ -64,6 +64,27 @@ _SIGS = {
     "dns_fv_sync": (ct.c_int, [ct.c_void_p]),
     "dns_fv_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "dns_fv_timing_read": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.POINTER(ct.c_int64)]),
+    "dns_spec_create": (ct.c_int, [ct.POINTER(SpecParams), ct.POINTER(ct.c_void_p)]),
+    "dns_spec_destroy": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_set_state": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp]),
+    "dns_spec_get_state": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp]),
+    "dns_spec_set_physical": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_get_physical": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_spec_to_phys": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_phys_to_spec": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_project": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_rhs": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_step": (ct.c_int, [ct.c_void_p, ct.c_double, _dp]),
+    "dns_spec_vsummax": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_shell_energies": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32, _dp]),
+    "dns_spec_rescale_shells": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32, _dp]),
+    "dns_spec_noise": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
+    "dns_spec_energy": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_dissipation": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_maxdiv": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_sync": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "dns_spec_timing_read": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.POINTER(ct.c_int64)]),
 }
 
 _LIB = None

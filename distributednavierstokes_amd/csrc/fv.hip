@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "fv_kernels.hpp"
+#include "host_util.hpp"
 
 namespace dns {
 
@@ -32,76 +33,6 @@ double modlam(int k0, int n, double d)
     double c = kk == 0 ? 1.0 : (double)cosl(pi * (2.0L * (long double)kk / (long double)n));
     return 2.0 * (c - 1.0) / (d * d);
 }
-
-// tw[t] = exp(-2πi t/n), t = 0..len-1, rounded from long double.
-template <typename T> std::vector<cplx<T>> twiddles(int n, int len)
-{
-    const long double twopi = 6.283185307179586476925286766559005768L;
-    std::vector<cplx<T>> tw(len);
-    for (int t = 0; t < len; ++t) {
-        long double a = twopi * (long double)t / (long double)n;
-        tw[t] = mk<T>((T)cosl(a), (T)(-sinl(a)));
-    }
-    return tw;
-}
-
-template <typename V> int dupload(V **dptr, const std::vector<V> &h)
-{
-    *dptr = nullptr;
-    size_t bytes = std::max<size_t>(1, h.size()) * sizeof(V);
-    DNS_HIP_CHECK(hipMalloc((void **)dptr, bytes));
-    if (!h.empty()) DNS_HIP_CHECK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
-    return DNS_OK;
-}
-
-template <typename V> int dalloc(V **dptr, size_t count)
-{
-    *dptr = nullptr;
-    DNS_HIP_CHECK(hipMalloc((void **)dptr, std::max<size_t>(1, count) * sizeof(V)));
-    DNS_HIP_CHECK(hipMemset(*dptr, 0, std::max<size_t>(1, count) * sizeof(V)));
-    return DNS_OK;
-}
-
-// Per-family HIP-event timing of kernel launches on the handle's stream.
-struct LaunchTimer {
-    bool on = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[8];
-    double ms[8] = {0};
-    long long cnt[8] = {0};
-    hipStream_t st = nullptr;
-    hipEvent_t cur = nullptr;
-    void begin(int fam)
-    {
-        (void)fam;
-        if (!on) return;
-        hipEventCreate(&cur);
-        hipEventRecord(cur, st);
-    }
-    void end(int fam)
-    {
-        if (!on) return;
-        hipEvent_t e;
-        hipEventCreate(&e);
-        hipEventRecord(e, st);
-        ev[fam].push_back({cur, e});
-        if (ev[fam].size() > 4096) flush();
-    }
-    void flush()
-    {
-        hipStreamSynchronize(st);
-        for (int f = 0; f < 8; ++f) {
-            for (auto &p : ev[f]) {
-                float t = 0;
-                hipEventElapsedTime(&t, p.first, p.second);
-                ms[f] += t;
-                cnt[f] += 1;
-                hipEventDestroy(p.first);
-                hipEventDestroy(p.second);
-            }
-            ev[f].clear();
-        }
-    }
-};
 
 enum { FAM_MOM = 0, FAM_X = 1, FAM_Z = 2, FAM_Y = 3, FAM_PGRAD = 4, FAM_CN = 5 };
 

@@ -15,6 +15,7 @@
 
 #include "common.hpp"
 #include "fft_lds.hpp"
+#include "reduce.hpp"
 
 namespace dns {
 
@@ -559,38 +560,6 @@ __global__ void __launch_bounds__(256) k_maxdiv(const T *ux, const T *uy, const 
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-// Block max of `cnt` partial maxima per slot; writes out[slot].
-__global__ void k_max_final(const double *part, int nblocks, int nslots, double *out)
-{
-    __shared__ double s[256];
-    for (int slot = 0; slot < nslots; ++slot) {
-        double m = 0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmax(m, part[b * nslots + slot]);
-        s[threadIdx.x] = m;
-        __syncthreads();
-        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[slot] = s[0];
-        __syncthreads();
-    }
-}
-
-// Tree-reduces a per-thread double-double in LDS; thread 0 returns the total.
-__device__ inline dd block_dd_sum(dd v, dd *s)
-{
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] = dd_add(s[threadIdx.x], s[threadIdx.x + w]);
-        __syncthreads();
-    }
-    dd r = s[0];
-    __syncthreads();
-    return r;
-}
-
 // Kinetic energy ½ Σ u² Δx Δ2[j] Δz over active DOFs (src/fields.jl:45-60),
 // compensated: per-thread double-double, block tree, per-block partials.
 template <typename T>
@@ -610,18 +579,6 @@ __global__ void __launch_bounds__(256) k_kinetic(const T *ux, const T *uy, const
     }
     dd r = block_dd_sum(acc, s);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
-// Sums nparts double-doubles per slot (slot-major: part[slot * nparts + b]).
-__global__ void k_dd_final(const dd *part, int nparts, int nslots, double *out)
-{
-    __shared__ dd s[256];
-    for (int slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        dd acc{0.0, 0.0};
-        for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc = dd_add(acc, part[(size_t)slot * nparts + b]);
-        dd r = block_dd_sum(acc, s);
-        if (threadIdx.x == 0) out[slot] = r.hi + r.lo;
-    }
 }
 
 // Channel statistics plane sums (src/stats.jl:39-58): block (j, chunk) sums
@@ -684,16 +641,6 @@ __global__ void __launch_bounds__(256) k_binshells(const cplx<T> *A, FVGeom g, i
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) part[(size_t)blockIdx.x * nbins + b] = bins[b];
-}
-
-// Sums per-block bins in block order (deterministic final pass).
-__global__ void k_bins_final(const double *part, int nblocks, int nbins, double *out)
-{
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbins; b += gridDim.x * blockDim.x) {
-        double s = 0;
-        for (int q = 0; q < nblocks; ++q) s += part[(size_t)q * nbins + b];
-        out[b] = s;
-    }
 }
 
 }  // namespace dns

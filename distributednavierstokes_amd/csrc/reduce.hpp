@@ -1,0 +1,64 @@
+// Block-level reductions shared by the FV and spectral kernels: FP64 maxima,
+// compensated (double-double) sums and binned sums, each finished by a
+// deterministic second pass over per-block partials.
+#pragma once
+
+#include "common.hpp"
+
+namespace dns {
+
+// Block max of `cnt` partial maxima per slot; writes out[slot].
+static __global__ void k_max_final(const double *part, int nblocks, int nslots, double *out)
+{
+    __shared__ double s[256];
+    for (int slot = 0; slot < nslots; ++slot) {
+        double m = 0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) m = fmax(m, part[b * nslots + slot]);
+        s[threadIdx.x] = m;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[slot] = s[0];
+        __syncthreads();
+    }
+}
+
+// Tree-reduces a per-thread double-double in LDS; thread 0 returns the total.
+__device__ inline dd block_dd_sum(dd v, dd *s)
+{
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = dd_add(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    dd r = s[0];
+    __syncthreads();
+    return r;
+}
+
+// Sums nparts double-doubles per slot (slot-major: part[slot * nparts + b]).
+static __global__ void k_dd_final(const dd *part, int nparts, int nslots, double *out)
+{
+    __shared__ dd s[256];
+    for (int slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        dd acc{0.0, 0.0};
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) acc = dd_add(acc, part[(size_t)slot * nparts + b]);
+        dd r = block_dd_sum(acc, s);
+        if (threadIdx.x == 0) out[slot] = r.hi + r.lo;
+    }
+}
+
+// Sums per-block bins in block order (deterministic final pass).
+static __global__ void k_bins_final(const double *part, int nblocks, int nbins, double *out)
+{
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbins; b += gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int q = 0; q < nblocks; ++q) s += part[(size_t)q * nbins + b];
+        out[b] = s;
+    }
+}
+
+}  // namespace dns

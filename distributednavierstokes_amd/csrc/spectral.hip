@@ -1,0 +1,592 @@
+// Host side of the pseudo-spectral path: spectral_setup, the IF-RK3 step
+// orchestration, forcing/IC/diagnostic entry points and their C ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "host_util.hpp"
+#include "spectral_kernels.hpp"
+
+namespace dns {
+
+// Wray RK3 (src/spectral/step.jl:16-20)
+static const double W_A[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double W_B[3] = {1.0 / 4.0, 0.0, 0.0};
+static const double W_DC[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+
+constexpr int SP_EPT = 16, SP_XEPT = 8, SP_SEPT = 8, SP_MAXO = 6, SP_NTHR = 256, SP_REDB = 1024;
+
+enum { SFAM_Y = 0, SFAM_Z = 1, SFAM_X = 2, SFAM_STAGE = 3, SFAM_FORCE = 4 };
+
+struct SpecBase {
+    virtual ~SpecBase() {}
+    virtual int set_state(const double *h, int which) = 0;
+    virtual int get_state(double *h, int which) = 0;
+    virtual int set_phys(const double *h) = 0;
+    virtual int get_phys(double *h) = 0;
+    virtual int spec_to_phys() = 0;
+    virtual int phys_to_spec() = 0;
+    virtual int project() = 0;
+    virtual int rhs() = 0;
+    virtual int step(double dt, double *vmax) = 0;
+    virtual int vsummax(double *out) = 0;
+    virtual int shells(int s0, int ns, double *E) = 0;
+    virtual int rescale(int s0, int ns, const double *f) = 0;
+    virtual int noise(uint64_t seed) = 0;
+    virtual int energy(double *e, double *diss) = 0;
+    virtual int maxdiv(double *out) = 0;
+    virtual int sync() = 0;
+    LaunchTimer timer;
+};
+
+template <typename T> struct SpecImpl : SpecBase {
+    SpecGeom g{};
+    double visc = 0, l = 0;
+    hipStream_t st = nullptr;
+    size_t ns = 0; // K m m (one component)
+    cplx<T> *uh = nullptr, *ustart = nullptr, *du = nullptr, *P = nullptr, *Q = nullptr;
+    T *phys = nullptr;
+    cplx<T> *twN = nullptr, *twx = nullptr, *twn = nullptr;
+    FFTDesc fdn{}, fdh{};
+    int tb = 1, tbs = 1, lpb = 1;
+    size_t lds = 0, ldss = 0, ldsx = 0;
+    double *red = nullptr, *redout = nullptr, *factors = nullptr, *pinned = nullptr;
+    size_t redcap = 0;
+
+    ~SpecImpl() override
+    {
+        if (st) hipStreamSynchronize(st);
+        timer.flush();
+        void *ptrs[] = {uh, ustart, du, P, Q, phys, twN, twx, twn, red, redout, factors};
+        for (void *p_ : ptrs)
+            if (p_) hipFree(p_);
+        if (pinned) hipHostFree(pinned);
+        if (st) hipStreamDestroy(st);
+    }
+
+    int create(const dns_spec_params *S)
+    {
+        g.n = S->n;
+        g.kcut = S->kcut;
+        g.K = S->kcut + 1;
+        g.m = 2 * S->kcut + 1;
+        g.kfac = 6.283185307179586 / S->l; // T(2π / l), src/spectral/setup.jl:101
+        visc = S->visc;
+        l = S->l;
+        ns = (size_t)g.K * g.m * g.m;
+        if (!make_fft_desc(g.n, fdn) || !make_fft_desc(g.n / 2, fdh)) {
+            set_error("unsupported grid: n must factor into 2, 3 and 5");
+            return DNS_EUNSUPPORTED;
+        }
+        if (g.n > SP_EPT * SP_NTHR || g.n / 2 > SP_XEPT * SP_NTHR) {
+            set_error("unsupported grid: n too large");
+            return DNS_EUNSUPPORTED;
+        }
+        DNS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        timer.st = st;
+        DNS_TRY(dalloc(&uh, 3 * ns));
+        DNS_TRY(dalloc(&ustart, 3 * ns));
+        DNS_TRY(dalloc(&P, (size_t)g.K * g.n * g.n * 6));
+        DNS_TRY(dalloc(&Q, (size_t)g.K * g.n * g.m * 6));
+        DNS_TRY(dupload(&twn, twiddles<T>(g.n, g.n)));
+        DNS_TRY(dupload(&twN, twiddles<T>(std::max(1, g.n / 2), std::max(1, g.n / 2))));
+        DNS_TRY(dupload(&twx, twiddles<T>(g.n, g.n / 2 + 1)));
+
+        int dev = 0, maxlds = 65536;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (maxlds <= 0) maxlds = 65536;
+        const size_t sz = sizeof(cplx<T>);
+        tb = std::max(1, std::min(64, SP_EPT * SP_NTHR / g.n));
+        while (tb > 1 && (size_t)tb * (g.n + 1) * sz > (size_t)maxlds) --tb;
+        tb = std::min(tb, g.K);
+        lds = (size_t)tb * (g.n + 1) * sz;
+        tbs = std::max(1, std::min({64, SP_SEPT * SP_NTHR / g.n, SP_MAXO * SP_NTHR / g.m}));
+        while (tbs > 1 && (size_t)tbs * (g.n + 1) * sz > (size_t)maxlds) --tbs;
+        tbs = std::min(tbs, g.K);
+        ldss = (size_t)tbs * (g.n + 1) * sz;
+        const size_t perline = (size_t)4 * g.n * sizeof(T);
+        lpb = (int)std::max<size_t>(1, std::min<size_t>(16, 32768 / perline));
+        while (lpb > 1 && lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
+        ldsx = perline * lpb;
+        if (lds > (size_t)maxlds || ldss > (size_t)maxlds || ldsx + 2048 > (size_t)maxlds || tbs * g.m > SP_MAXO * SP_NTHR) {
+            set_error("spectral tiles do not fit the LDS limit");
+            return DNS_EUNSUPPORTED;
+        }
+        redcap = std::max<size_t>((size_t)SP_REDB * 4, 2 * ((size_t)g.n * g.n / lpb + 16));
+        DNS_TRY(dalloc(&red, redcap));
+        DNS_TRY(dalloc(&redout, 64));
+        DNS_TRY(dalloc(&factors, 4096));
+        DNS_HIP_CHECK(hipHostMalloc((void **)&pinned, sizeof(double) * 4096));
+        return DNS_OK;
+    }
+
+    // --------------------------------------------------------- transfers
+    cplx<T> *state_ptr(int which) { return which == 1 ? ustart : which == 2 ? du : uh; }
+
+    int set_state(const double *h, int which) override
+    {
+        cplx<T> *d = state_ptr(which);
+        if (!d) {
+            set_error("state buffer not allocated");
+            return DNS_EINVAL;
+        }
+        std::vector<cplx<T>> s(3 * ns);
+        for (size_t t = 0; t < 3 * ns; ++t) s[t] = mk<T>((T)h[2 * t], (T)h[2 * t + 1]);
+        DNS_HIP_CHECK(hipMemcpyAsync(d, s.data(), sizeof(cplx<T>) * 3 * ns, hipMemcpyHostToDevice, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+
+    int get_state(double *h, int which) override
+    {
+        cplx<T> *d = state_ptr(which);
+        if (!d) {
+            set_error("state buffer not allocated (call dns_spec_rhs first)");
+            return DNS_EINVAL;
+        }
+        std::vector<cplx<T>> s(3 * ns);
+        DNS_HIP_CHECK(hipMemcpyAsync(s.data(), d, sizeof(cplx<T>) * 3 * ns, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t t = 0; t < 3 * ns; ++t) {
+            h[2 * t] = (double)s[t].x;
+            h[2 * t + 1] = (double)s[t].y;
+        }
+        return DNS_OK;
+    }
+
+    int ensure_phys()
+    {
+        if (!phys) DNS_TRY(dalloc(&phys, (size_t)3 * g.n * g.n * g.n));
+        return DNS_OK;
+    }
+
+    int set_phys(const double *h) override
+    {
+        DNS_TRY(ensure_phys());
+        size_t N3 = (size_t)3 * g.n * g.n * g.n;
+        std::vector<T> s(N3);
+        for (size_t t = 0; t < N3; ++t) s[t] = (T)h[t];
+        DNS_HIP_CHECK(hipMemcpyAsync(phys, s.data(), sizeof(T) * N3, hipMemcpyHostToDevice, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+
+    int get_phys(double *h) override
+    {
+        DNS_TRY(ensure_phys());
+        size_t N3 = (size_t)3 * g.n * g.n * g.n;
+        std::vector<T> s(N3);
+        DNS_HIP_CHECK(hipMemcpyAsync(s.data(), phys, sizeof(T) * N3, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t t = 0; t < N3; ++t) h[t] = (double)s[t];
+        return DNS_OK;
+    }
+
+    // ----------------------------------------------------------- passes
+    int backward_yz()
+    {
+        timer.begin(SFAM_Y);
+        hipLaunchKernelGGL((k_sp_yback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, uh, Q,
+                           twn, tb, g, fdn);
+        timer.end(SFAM_Y);
+        timer.begin(SFAM_Z);
+        hipLaunchKernelGGL((k_sp_zback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, 3), dim3(SP_NTHR), lds, st, Q, P,
+                           twn, tb, g, fdn);
+        timer.end(SFAM_Z);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    // vpart: per-block velocity bounds (stage 1 and vsummax read them; later
+    // stages write to the scratch half of the reduction buffer)
+    template <int MODE> int xpass(double *vpart)
+    {
+        XMidArgs<T> a;
+        a.P = P;
+        a.phys = phys;
+        a.vpart = vpart;
+        a.tw = twN;
+        a.twx = twx;
+        a.lpb = lpb;
+        timer.begin(SFAM_X);
+        hipLaunchKernelGGL((k_sp_xmid<T, SP_XEPT, MODE>), dim3(ceil_div((long)g.n * g.n, lpb)), dim3(SP_NTHR), ldsx,
+                           st, a, g, fdh);
+        timer.end(SFAM_X);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int zfwd(int ncomp)
+    {
+        timer.begin(SFAM_Z);
+        hipLaunchKernelGGL((k_sp_zfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, ncomp), dim3(SP_NTHR), lds, st, P, Q,
+                           twn, tb, g, fdn);
+        timer.end(SFAM_Z);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    template <int MODE> int ystage(int stage, double dt)
+    {
+        StageArgs<T> s;
+        s.Q = Q;
+        s.uh = uh;
+        s.ustart = ustart;
+        s.du = du;
+        s.tw = twn;
+        s.tb = tbs;
+        s.first = stage == 0;
+        s.a = (T)W_A[stage];
+        s.b = (T)W_B[stage];
+        s.nudcdt = (T)(visc * W_DC[stage] * dt);
+        s.dt = (T)dt;
+        timer.begin(SFAM_STAGE);
+        hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, MODE>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR),
+                           ldss, st, s, g, fdn);
+        timer.end(SFAM_STAGE);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int spec_to_phys() override
+    {
+        DNS_TRY(ensure_phys());
+        DNS_TRY(backward_yz());
+        return xpass<XMID_C2R>(red);
+    }
+
+    int phys_to_spec() override
+    {
+        DNS_TRY(ensure_phys());
+        DNS_TRY(xpass<XMID_R2C>(red));
+        DNS_TRY(zfwd(3));
+        hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, uh, twn,
+                           tb, g, fdn);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int project() override
+    {
+        int nb = std::min<long>(SP_REDB, ceil_div((long)ns, SP_NTHR));
+        hipLaunchKernelGGL((k_sp_project<T>), dim3(nb), dim3(SP_NTHR), 0, st, uh, g);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int rhs() override
+    {
+        if (!du) DNS_TRY(dalloc(&du, 3 * ns));
+        DNS_TRY(backward_yz());
+        DNS_TRY(xpass<XMID_RHS>(red));
+        DNS_TRY(zfwd(6));
+        return ystage<1>(0, 0.0);
+    }
+
+    int vmax_final(int nparts, double *out)
+    {
+        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(SP_NTHR), 0, st, red, nparts, 1, redout);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, redout, sizeof(double), hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        *out = pinned[0];
+        return DNS_OK;
+    }
+
+    // spectral_step! (src/spectral/step.jl:68-92): three (rhs, IF stage,
+    // projection) stages; vmax of the first stage's physical velocity.
+    int step(double dt, double *vmax) override
+    {
+        for (int s = 0; s < 3; ++s) {
+            DNS_TRY(backward_yz());
+            DNS_TRY(xpass<XMID_RHS>(s == 0 ? red : red + redcap / 2));
+            DNS_TRY(zfwd(6));
+            DNS_TRY(ystage<0>(s, dt));
+        }
+        if (vmax) DNS_TRY(vmax_final(ceil_div((long)g.n * g.n, lpb), vmax));
+        return DNS_OK;
+    }
+
+    int vsummax(double *out) override
+    {
+        DNS_TRY(backward_yz());
+        DNS_TRY(xpass<XMID_VMAX>(red));
+        return vmax_final(ceil_div((long)g.n * g.n, lpb), out);
+    }
+
+    long cube_count(int r) const
+    {
+        if (r >= g.kcut) return (long)ns;
+        return (long)(r + 1) * (2 * r + 1) * (2 * r + 1);
+    }
+
+    int shells(int s0, int nsh, double *E) override
+    {
+        if (nsh <= 0 || nsh > 4096 || s0 < 0) {
+            set_error("invalid shell range");
+            return DNS_EINVAL;
+        }
+        int r = s0 + nsh - 1;
+        long count = cube_count(r);
+        int nb = (int)std::max<long>(1, std::min<long>(256, (count + SP_NTHR - 1) / SP_NTHR));
+        size_t need = (size_t)nb * nsh;
+        if (need > redcap) {
+            hipStreamSynchronize(st);
+            hipFree(red);
+            redcap = need;
+            DNS_TRY(dalloc(&red, redcap));
+        }
+        timer.begin(SFAM_FORCE);
+        hipLaunchKernelGGL((k_sp_shells<T>), dim3(nb), dim3(SP_NTHR), sizeof(double) * nsh, st, uh, s0, nsh, r, count,
+                           g, red);
+        hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nsh, SP_NTHR)), dim3(SP_NTHR), 0, st, red, nb, nsh, factors);
+        timer.end(SFAM_FORCE);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, factors, sizeof(double) * nsh, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        for (int q = 0; q < nsh; ++q) E[q] = pinned[q];
+        return DNS_OK;
+    }
+
+    int rescale(int s0, int nsh, const double *f) override
+    {
+        if (nsh <= 0 || nsh > 4096 || s0 < 0) {
+            set_error("invalid shell range");
+            return DNS_EINVAL;
+        }
+        for (int q = 0; q < nsh; ++q) pinned[q] = f[q];
+        DNS_HIP_CHECK(hipMemcpyAsync(factors, pinned, sizeof(double) * nsh, hipMemcpyHostToDevice, st));
+        int r = s0 + nsh - 1;
+        long count = cube_count(r);
+        int nb = (int)std::max<long>(1, std::min<long>(SP_REDB, (count + SP_NTHR - 1) / SP_NTHR));
+        timer.begin(SFAM_FORCE);
+        hipLaunchKernelGGL((k_sp_rescale<T>), dim3(nb), dim3(SP_NTHR), 0, st, uh, s0, nsh, r, count, factors, g);
+        timer.end(SFAM_FORCE);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipStreamSynchronize(st)); // the pinned staging buffer is reused
+        return DNS_OK;
+    }
+
+    int noise(uint64_t seed) override
+    {
+        DNS_TRY(ensure_phys());
+        size_t N3 = (size_t)3 * g.n * g.n * g.n;
+        int nb = (int)std::min<size_t>(4096, (N3 + SP_NTHR - 1) / SP_NTHR);
+        hipLaunchKernelGGL((k_sp_noise<T>), dim3(nb), dim3(SP_NTHR), 0, st, phys, seed, g.n);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int energy(double *e, double *diss) override
+    {
+        int nb = (int)std::min<long>(SP_REDB, ceil_div((long)ns, SP_NTHR));
+        hipLaunchKernelGGL((k_sp_energy<T>), dim3(nb), dim3(SP_NTHR), 0, st, uh, g, reinterpret_cast<dd *>(red));
+        hipLaunchKernelGGL(k_dd_final, dim3(2), dim3(SP_NTHR), 0, st, reinterpret_cast<dd *>(red), nb, 2, redout);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, redout, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        if (e) *e = pinned[0];
+        if (diss) *diss = visc * pinned[1];
+        return DNS_OK;
+    }
+
+    int maxdiv(double *out) override
+    {
+        int nb = (int)std::min<long>(SP_REDB, ceil_div((long)ns, SP_NTHR));
+        hipLaunchKernelGGL((k_sp_maxdiv<T>), dim3(nb), dim3(SP_NTHR), 0, st, uh, g, red);
+        return vmax_final(nb, out);
+    }
+
+    int sync() override
+    {
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+};
+
+}  // namespace dns
+
+struct dns_spec {
+    std::unique_ptr<dns::SpecBase> impl;
+};
+
+using namespace dns;
+
+#define SP_CHECK(h)                           \
+    do {                                      \
+        if (!(h) || !(h)->impl) {             \
+            set_error("null dns_spec handle"); \
+            return DNS_EINVAL;                \
+        }                                     \
+    } while (0)
+
+extern "C" {
+
+int dns_spec_create(const dns_spec_params *S, dns_spec **out)
+{
+    if (!out || !S) {
+        set_error("null params or output handle");
+        return DNS_EINVAL;
+    }
+    *out = nullptr;
+    if (S->n % 2 != 0) {
+        set_error("n must be even");
+        return DNS_EINVAL;
+    }
+    if (S->kcut < 1) {
+        set_error("kcut must be at least 1");
+        return DNS_EINVAL;
+    }
+    if (3 * S->kcut > S->n) {
+        set_error("aliasing: 3kcut ≤ n is required (2/3 rule)");
+        return DNS_EINVAL;
+    }
+    if (S->precision != DNS_FP64 && S->precision != DNS_MIXED) {
+        set_error("precision must be DNS_FP64 or DNS_MIXED");
+        return DNS_EINVAL;
+    }
+    if ((long long)S->n * S->n * S->n * 3 >= (1LL << 40)) {
+        set_error("grid too large");
+        return DNS_EUNSUPPORTED;
+    }
+    std::unique_ptr<dns_spec> h(new dns_spec);
+    int rc;
+    if (S->precision == DNS_FP64) {
+        auto *impl = new SpecImpl<double>();
+        h->impl.reset(impl);
+        rc = impl->create(S);
+    } else {
+        auto *impl = new SpecImpl<float>();
+        h->impl.reset(impl);
+        rc = impl->create(S);
+    }
+    if (rc != DNS_OK) return rc;
+    *out = h.release();
+    return DNS_OK;
+}
+
+int dns_spec_destroy(dns_spec *h)
+{
+    delete h;
+    return DNS_OK;
+}
+int dns_spec_set_state(dns_spec *h, int32_t which, const double *host)
+{
+    SP_CHECK(h);
+    if (!host || which < 0 || which > 1) return DNS_EINVAL;
+    return h->impl->set_state(host, which);
+}
+int dns_spec_get_state(dns_spec *h, int32_t which, double *host)
+{
+    SP_CHECK(h);
+    if (!host || which < 0 || which > 2) return DNS_EINVAL;
+    return h->impl->get_state(host, which);
+}
+int dns_spec_set_physical(dns_spec *h, const double *host)
+{
+    SP_CHECK(h);
+    if (!host) return DNS_EINVAL;
+    return h->impl->set_phys(host);
+}
+int dns_spec_get_physical(dns_spec *h, double *host)
+{
+    SP_CHECK(h);
+    if (!host) return DNS_EINVAL;
+    return h->impl->get_phys(host);
+}
+int dns_spec_spec_to_phys(dns_spec *h)
+{
+    SP_CHECK(h);
+    return h->impl->spec_to_phys();
+}
+int dns_spec_phys_to_spec(dns_spec *h)
+{
+    SP_CHECK(h);
+    return h->impl->phys_to_spec();
+}
+int dns_spec_project(dns_spec *h)
+{
+    SP_CHECK(h);
+    return h->impl->project();
+}
+int dns_spec_rhs(dns_spec *h)
+{
+    SP_CHECK(h);
+    return h->impl->rhs();
+}
+int dns_spec_step(dns_spec *h, double dt, double *vmax)
+{
+    SP_CHECK(h);
+    return h->impl->step(dt, vmax);
+}
+int dns_spec_vsummax(dns_spec *h, double *out)
+{
+    SP_CHECK(h);
+    if (!out) return DNS_EINVAL;
+    return h->impl->vsummax(out);
+}
+int dns_spec_shell_energies(dns_spec *h, int32_t s0, int32_t nshells, double *E)
+{
+    SP_CHECK(h);
+    if (!E) return DNS_EINVAL;
+    return h->impl->shells(s0, nshells, E);
+}
+int dns_spec_rescale_shells(dns_spec *h, int32_t s0, int32_t nshells, const double *factors)
+{
+    SP_CHECK(h);
+    if (!factors) return DNS_EINVAL;
+    return h->impl->rescale(s0, nshells, factors);
+}
+int dns_spec_noise(dns_spec *h, uint64_t seed)
+{
+    SP_CHECK(h);
+    return h->impl->noise(seed);
+}
+int dns_spec_energy(dns_spec *h, double *out)
+{
+    SP_CHECK(h);
+    if (!out) return DNS_EINVAL;
+    return h->impl->energy(out, nullptr);
+}
+int dns_spec_dissipation(dns_spec *h, double *out)
+{
+    SP_CHECK(h);
+    if (!out) return DNS_EINVAL;
+    return h->impl->energy(nullptr, out);
+}
+int dns_spec_maxdiv(dns_spec *h, double *out)
+{
+    SP_CHECK(h);
+    if (!out) return DNS_EINVAL;
+    return h->impl->maxdiv(out);
+}
+int dns_spec_sync(dns_spec *h)
+{
+    SP_CHECK(h);
+    return h->impl->sync();
+}
+int dns_spec_timing(dns_spec *h, int32_t enable)
+{
+    SP_CHECK(h);
+    h->impl->timer.flush();
+    for (int f = 0; f < 8; ++f) {
+        h->impl->timer.ms[f] = 0;
+        h->impl->timer.cnt[f] = 0;
+    }
+    h->impl->timer.on = enable != 0;
+    return DNS_OK;
+}
+int dns_spec_timing_read(dns_spec *h, int32_t family, double *total_ms, int64_t *launches)
+{
+    SP_CHECK(h);
+    if (family < 0 || family >= 8 || !total_ms || !launches) return DNS_EINVAL;
+    h->impl->timer.flush();
+    *total_ms = h->impl->timer.ms[family];
+    *launches = h->impl->timer.cnt[family];
+    return DNS_OK;
+}
+
+}  // extern "C"

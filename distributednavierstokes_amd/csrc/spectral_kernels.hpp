@@ -1,0 +1,524 @@
+// Device kernels of the pseudo-spectral HIT step (gfx950).
+//
+// Spectral state uh: (K, m, m, 3) complex, K = kcut+1, m = 2kcut+1, first
+// index fastest, compact frequency order j <= kcut ? j : j - m
+// (src/spectral/operators.jl:7); û = F[u]/n³ (src/spectral/fft.jl:176-179).
+// The 3D transforms run as three line passes without transposes:
+//   backward:  y pass (pad m → n, inverse FFT)      uh → B1 (K, n, m, 3)
+//              z pass (pad m → n, inverse FFT)      B1 → P[..,0:3] (K, n, n)
+//              x pass (c2r of the K retained kx)    P  → physical, LDS only
+//   forward:   x pass (r2c, keep kx < K)            physical → P (K, n, n)
+//              z pass (FFT, keep retained kz)       P → Q (K, n, m)
+//              y pass (FFT, keep retained ky, /n³)  Q → state
+// In the time step the x pass is fused: c2r of the 3 velocity components,
+// the 6 products v_i v_j (and the CFL velocity bound), r2c of each product;
+// physical fields never reach HBM.  The forward y pass of the products is
+// fused with the divergence-form nonlinear term −i k_j σ̂_ij, the Wray
+// integrating-factor stage and the projection.
+#pragma once
+
+#include "common.hpp"
+#include "fft_lds.hpp"
+#include "reduce.hpp"
+
+namespace dns {
+
+struct SpecGeom {
+    int n, kcut, K, m;
+    double kfac; // 2π / l
+};
+
+__device__ inline int sp_fine(int c, const SpecGeom &g) { return c <= g.kcut ? c : c + (g.n - g.m); }
+__device__ inline int sp_freq(int c, const SpecGeom &g) { return c <= g.kcut ? c : c - g.m; }
+// fine position → compact index, or -1 in the killed band
+__device__ inline int sp_compact(int f, const SpecGeom &g)
+{
+    if (f <= g.kcut) return f;
+    if (f >= g.n - g.kcut) return f - (g.n - g.m);
+    return -1;
+}
+
+// ------------------------------------------------------------ y / z passes
+
+// Backward y pass: B1[i, y, kz, c] = Σ_ky uh[i, ky, kz, c] e^{+i ky y}.
+// grid (ceil(K/tb), m, 3); LDS tb·(n+1).
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb,
+                                                  SpecGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int i0 = blockIdx.x * tb, kz = blockIdx.y, c = blockIdx.z;
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        int jc = sp_compact(t, g);
+        cplx<T> v = mk<T>(T(0), T(0));
+        if (i < K && jc >= 0) v = uh[i + (size_t)K * (jc + (size_t)m * (kz + (size_t)m * c))];
+        buf[b * ls + t] = v;
+    }
+    __syncthreads();
+    fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
+    }
+}
+
+// Backward z pass: P[i, y, z, c] = Σ_kz B1[i, y, kz, c] e^{+i kz z}.
+// grid (ceil(K/tb), n, 3).
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, SpecGeom g,
+                                                  FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int i0 = blockIdx.x * tb, y = blockIdx.y, c = blockIdx.z;
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        int kc = sp_compact(t, g);
+        cplx<T> v = mk<T>(T(0), T(0));
+        if (i < K && kc >= 0) v = B1[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * c))];
+        buf[b * ls + t] = v;
+    }
+    __syncthreads();
+    fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
+    }
+}
+
+// Forward z pass: Q[i, y, kc, p] = Σ_z P[i, y, z, p] e^{−i kz z} at the
+// retained kz.  grid (ceil(K/tb), n, ncomp).
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, SpecGeom g,
+                                                 FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int i0 = blockIdx.x * tb, y = blockIdx.y, p = blockIdx.z;
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        buf[b * ls + t] = i < K ? P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * p))] : mk<T>(T(0), T(0));
+    }
+    __syncthreads();
+    fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
+    for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
+        int kc = idx / tb, b = idx - kc * tb, i = i0 + b;
+        if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
+    }
+}
+
+// Forward y pass with truncation and 1/n³ (truncate_y_kernel!,
+// src/spectral/fft.jl:190-193): out[i, jc, kc, p] (K, m, m, ncomp).
+// grid (ceil(K/tb), m, ncomp).
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, SpecGeom g,
+                                                 FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int i0 = blockIdx.x * tb, kc = blockIdx.y, p = blockIdx.z;
+    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        buf[b * ls + t] = i < K ? Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
+    }
+    __syncthreads();
+    fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
+    const T inv = T(1.0 / ((double)n * n * n));
+    for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
+        int jc = idx / tb, b = idx - jc * tb, i = i0 + b;
+        if (i < K) out[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))] = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
+    }
+}
+
+// ------------------------------------------------------------- x passes
+
+enum { XMID_RHS = 0, XMID_C2R = 1, XMID_R2C = 2, XMID_VMAX = 3 };
+
+// x pass over (y, z) lines L = y + n z, `lpb` lines per block, d.n = n/2.
+//   XMID_RHS : c2r of P[..,0:3] → v (LDS), vmax partial, six products, r2c
+//              of each product into P[.., 0:6] (kx < K)
+//   XMID_C2R : c2r of P[..,0:3] → phys (n, n, n, 3)
+//   XMID_VMAX: c2r of P[..,0:3] → vmax partial only
+//   XMID_R2C : r2c of phys (n, n, n, 3) → P[..,0:3] (kx < K)
+// LDS: v[3][lpb][n] reals followed by work[lpb][n/2] complex.
+template <typename T> struct XMidArgs {
+    cplx<T> *P;
+    T *phys;
+    double *vpart;
+    const cplx<T> *tw;  // length n/2
+    const cplx<T> *twx; // length n/2 + 1
+    int lpb;
+};
+
+template <typename T, int EPT, int MODE>
+__global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int n = g.n, N = n / 2, K = g.K, lpb = a.lpb;
+    T *v = reinterpret_cast<T *>(smem_raw);
+    cplx<T> *work = reinterpret_cast<cplx<T> *>(smem_raw + sizeof(T) * (size_t)3 * lpb * n);
+    const int nlines = n * n;
+    const int L0 = blockIdx.x * lpb;
+    const size_t pstride = (size_t)K * n * n; // component stride of P
+    if (MODE != XMID_R2C) {
+        for (int c = 0; c < 3; ++c) {
+            for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
+                int b = idx / N, k = idx - b * N, L = L0 + b;
+                cplx<T> Z = mk<T>(T(0), T(0));
+                if (L < nlines) {
+                    const cplx<T> *row = a.P + (size_t)L * K + c * pstride;
+                    cplx<T> Xk = k < K ? row[k] : mk<T>(T(0), T(0));
+                    cplx<T> Xn = (N - k) < K ? row[N - k] : mk<T>(T(0), T(0));
+                    if (k == 0) {
+                        Xk.y = T(0);
+                        Xn.y = T(0);
+                    }
+                    cplx<T> Xc = cconj<T>(Xn);
+                    cplx<T> ev = cadd<T>(Xk, Xc);
+                    cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]);
+                    Z = mk<T>(ev.x - od.y, ev.y + od.x);
+                }
+                work[idx] = Z;
+            }
+            __syncthreads();
+            fft_lds<T, EPT, true>(work, lpb, N, d, a.tw);
+            for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
+                int b = idx / N, t = idx - b * N;
+                cplx<T> z = work[idx];
+                T *vl = v + ((size_t)c * lpb + b) * n;
+                vl[2 * t] = z.x;
+                vl[2 * t + 1] = z.y;
+            }
+            __syncthreads();
+        }
+        if (MODE == XMID_RHS || MODE == XMID_VMAX) {
+            __shared__ double red[256];
+            double mx = 0;
+            for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
+                int b = idx / n;
+                if (L0 + b >= nlines) continue;
+                double s = fabs((double)v[idx]) + fabs((double)v[(size_t)lpb * n + idx]) +
+                           fabs((double)v[(size_t)2 * lpb * n + idx]);
+                mx = fmax(mx, s);
+            }
+            red[threadIdx.x] = mx;
+            __syncthreads();
+            for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+                if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) a.vpart[blockIdx.x] = red[0];
+        }
+        if (MODE == XMID_C2R) {
+            for (int c = 0; c < 3; ++c)
+                for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
+                    int b = idx / n, x = idx - b * n, L = L0 + b;
+                    if (L < nlines) a.phys[x + (size_t)n * L + (size_t)c * n * nlines] = v[((size_t)c * lpb + b) * n + x];
+                }
+        }
+        if (MODE != XMID_RHS) return;
+    }
+    // forward r2c of products (RHS) or of physical components (R2C)
+    const int nout = MODE == XMID_RHS ? 6 : 3;
+    for (int p = 0; p < nout; ++p) {
+        const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
+        const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
+        for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
+            int b = idx / N, t = idx - b * N, L = L0 + b;
+            cplx<T> z = mk<T>(T(0), T(0));
+            if (MODE == XMID_RHS) {
+                const T *va = v + ((size_t)ca * lpb + b) * n, *vb = v + ((size_t)cb * lpb + b) * n;
+                z = mk<T>(va[2 * t] * vb[2 * t], va[2 * t + 1] * vb[2 * t + 1]);
+            } else if (L < nlines) {
+                const T *src = a.phys + (size_t)n * L + (size_t)p * n * nlines;
+                z = mk<T>(src[2 * t], src[2 * t + 1]);
+            }
+            work[idx] = z;
+        }
+        __syncthreads();
+        fft_lds<T, EPT, false>(work, lpb, N, d, a.tw);
+        for (int idx = threadIdx.x; idx < lpb * K; idx += blockDim.x) {
+            int b = idx / K, kx = idx - b * K, L = L0 + b;
+            if (L >= nlines) continue;
+            cplx<T> Zk = work[b * N + (kx == N ? 0 : kx)];
+            cplx<T> Zc = cconj<T>(work[b * N + (kx == 0 ? 0 : N - kx)]);
+            cplx<T> s = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
+            cplx<T> wd = cmul<T>(a.twx[kx], df);
+            a.P[kx + (size_t)K * L + p * pstride] = mk<T>(T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x));
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------- y forward + RHS + IF stage
+
+// Forward y pass of the six product spectra with the divergence-form
+// nonlinear term (src/spectral/operators.jl:35-79), the Wray
+// integrating-factor stage (src/spectral/step.jl:22-45) and the projection
+// (src/spectral/operators.jl:10-22).  Each thread owns up to MAXO outputs
+// (b, jc) of the tile; their du accumulate in registers across the six
+// product passes.  MODE 0: stage update of uh/ustart; MODE 1: write du.
+// grid (ceil(K/tb), m).
+template <typename T> struct StageArgs {
+    const cplx<T> *Q;
+    cplx<T> *uh, *ustart, *du;
+    const cplx<T> *tw;
+    int tb;
+    int first;   // stage 1: the stage start is uh itself
+    T a, b, nudcdt, dt;
+};
+
+template <typename T, int EPT, int MAXO, int MODE>
+__global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, FFTDesc d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+    const int n = g.n, ls = n + 1, K = g.K, m = g.m, tb = s.tb;
+    const int i0 = blockIdx.x * tb, kc = blockIdx.y;
+    const T inv = T(1.0 / ((double)n * n * n));
+    const T kz = T(g.kfac * sp_freq(kc, g));
+    cplx<T> acc[MAXO][3];
+#pragma unroll
+    for (int o = 0; o < MAXO; ++o)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[o][c] = mk<T>(T(0), T(0));
+    for (int p = 0; p < 6; ++p) {
+        for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
+            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            buf[b * ls + t] = i < K ? s.Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
+        }
+        __syncthreads();
+        fft_lds<T, EPT, false>(buf, tb, ls, d, s.tw);
+#pragma unroll
+        for (int o = 0; o < MAXO; ++o) {
+            int oi = threadIdx.x + o * blockDim.x;
+            if (oi < tb * m) {
+                int jc = oi / tb, b = oi - jc * tb, i = i0 + b;
+                if (i < K) {
+                    cplx<T> sg = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
+                    T kx = T(g.kfac * i), ky = T(g.kfac * sp_freq(jc, g));
+                    // −i k σ  =  (k σ.y, −k σ.x)
+                    auto mi = [](T k, cplx<T> z) { return mk<T>(k * z.y, -k * z.x); };
+                    switch (p) {
+                        case 0: acc[o][0] = cadd<T>(acc[o][0], mi(kx, sg)); break;              // σxx
+                        case 1: acc[o][1] = cadd<T>(acc[o][1], mi(ky, sg)); break;              // σyy
+                        case 2: acc[o][2] = cadd<T>(acc[o][2], mi(kz, sg)); break;              // σzz
+                        case 3:                                                                  // σxy
+                            acc[o][0] = cadd<T>(acc[o][0], mi(ky, sg));
+                            acc[o][1] = cadd<T>(acc[o][1], mi(kx, sg));
+                            break;
+                        case 4:                                                                  // σyz
+                            acc[o][1] = cadd<T>(acc[o][1], mi(kz, sg));
+                            acc[o][2] = cadd<T>(acc[o][2], mi(ky, sg));
+                            break;
+                        default:                                                                 // σzx
+                            acc[o][0] = cadd<T>(acc[o][0], mi(kz, sg));
+                            acc[o][2] = cadd<T>(acc[o][2], mi(kx, sg));
+                            break;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = 0; o < MAXO; ++o) {
+        int oi = threadIdx.x + o * blockDim.x;
+        if (oi >= tb * m) continue;
+        int jc = oi / tb, b = oi - jc * tb, i = i0 + b;
+        if (i >= K) continue;
+        const size_t base = i + (size_t)K * (jc + (size_t)m * kc), cs = (size_t)K * m * m;
+        if (MODE == 1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.du[base + c * cs] = acc[o][c];
+            continue;
+        }
+        T kx = T(g.kfac * i), ky = T(g.kfac * sp_freq(jc, g));
+        T kk = kx * kx + ky * ky + kz * kz;
+        T phi = exp(-s.nudcdt * kk);
+        cplx<T> u[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cplx<T> st = s.first ? s.uh[base + c * cs] : s.ustart[base + c * cs];
+            cplx<T> dd_ = acc[o][c];
+            T at = s.a * s.dt, bt = s.b * s.dt;
+            u[c] = mk<T>(phi * (st.x + at * dd_.x), phi * (st.y + at * dd_.y));
+            s.ustart[base + c * cs] = mk<T>(phi * (st.x + bt * dd_.x), phi * (st.y + bt * dd_.y));
+        }
+        // projection û ← û − k (k·û)/|k|², k = 0 untouched
+        cplx<T> pr = mk<T>(T(0), T(0));
+        if (kk != T(0)) {
+            pr = mk<T>((kx * u[0].x + ky * u[1].x + kz * u[2].x) / kk, (kx * u[0].y + ky * u[1].y + kz * u[2].y) / kk);
+        }
+        s.uh[base] = mk<T>(u[0].x - kx * pr.x, u[0].y - kx * pr.y);
+        s.uh[base + cs] = mk<T>(u[1].x - ky * pr.x, u[1].y - ky * pr.y);
+        s.uh[base + 2 * cs] = mk<T>(u[2].x - kz * pr.x, u[2].y - kz * pr.y);
+    }
+}
+
+// ------------------------------------------------------- pointwise kernels
+
+template <typename T> __global__ void __launch_bounds__(256) k_sp_project(cplx<T> *uh, SpecGeom g)
+{
+    const size_t cs = (size_t)g.K * g.m * g.m;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < cs; t += (size_t)gridDim.x * blockDim.x) {
+        int i = t % g.K, r = t / g.K, j = r % g.m, k = r / g.m;
+        T k1 = T(g.kfac * i), k2 = T(g.kfac * sp_freq(j, g)), k3 = T(g.kfac * sp_freq(k, g));
+        T kk = k1 * k1 + k2 * k2 + k3 * k3;
+        cplx<T> a = uh[t], b = uh[t + cs], c = uh[t + 2 * cs];
+        cplx<T> p = mk<T>(T(0), T(0));
+        if (kk != T(0)) p = mk<T>((k1 * a.x + k2 * b.x + k3 * c.x) / kk, (k1 * a.y + k2 * b.y + k3 * c.y) / kk);
+        uh[t] = mk<T>(a.x - k1 * p.x, a.y - k1 * p.y);
+        uh[t + cs] = mk<T>(b.x - k2 * p.x, b.y - k2 * p.y);
+        uh[t + 2 * cs] = mk<T>(c.x - k3 * p.x, c.y - k3 * p.y);
+    }
+}
+
+__device__ inline int sp_shell(int i, int j, int k, const SpecGeom &g)
+{
+    long a = i, b = sp_freq(j, g), c = sp_freq(k, g);
+    return (int)floor(sqrt((double)(a * a + b * b + c * c)));
+}
+
+// Enumerates the modes of the cube |k_i| <= r (all modes when r >= kcut):
+// flat index → (i, j, k) state indices.
+__device__ inline bool sp_cube(long t, int r, const SpecGeom &g, int &i, int &j, int &k)
+{
+    if (r >= g.kcut) {
+        i = t % g.K;
+        long q = t / g.K;
+        j = q % g.m;
+        k = q / g.m;
+        return true;
+    }
+    int w = 2 * r + 1;
+    i = t % (r + 1);
+    long q = t / (r + 1);
+    int jj = q % w, kk = q / w;
+    j = jj <= r ? jj : g.m - (w - jj); // freq jj - w → compact m + (jj - w)
+    k = kk <= r ? kk : g.m - (w - kk);
+    return true;
+}
+
+// Per-block shell energies (src/spectral/forcing.jl:12-24) of shells
+// s0..s0+ns−1 over the cube |k_i| <= smax; part[block * ns + s].
+template <typename T>
+__global__ void __launch_bounds__(256) k_sp_shells(const cplx<T> *uh, int s0, int ns, int r, long count, SpecGeom g,
+                                                   double *part)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *bins = reinterpret_cast<double *>(smem_raw);
+    for (int b = threadIdx.x; b < ns; b += blockDim.x) bins[b] = 0.0;
+    __syncthreads();
+    const size_t cs = (size_t)g.K * g.m * g.m;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sp_cube(t, r, g, i, j, k);
+        int sh = sp_shell(i, j, k, g) - s0;
+        if (sh < 0 || sh >= ns) continue;
+        size_t base = i + (size_t)g.K * (j + (size_t)g.m * k);
+        double e = 0;
+        for (int c = 0; c < 3; ++c) {
+            cplx<T> a = uh[base + c * cs];
+            e += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+        }
+        atomicAdd(&bins[sh], (i == 0 ? 1.0 : 2.0) * e / 2);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < ns; b += blockDim.x) part[(size_t)blockIdx.x * ns + b] = bins[b];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sp_rescale(cplx<T> *uh, int s0, int ns, int r, long count,
+                                                    const double *factors, SpecGeom g)
+{
+    const size_t cs = (size_t)g.K * g.m * g.m;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        sp_cube(t, r, g, i, j, k);
+        int sh = sp_shell(i, j, k, g) - s0;
+        if (sh < 0 || sh >= ns) continue;
+        T f = T(factors[sh]);
+        size_t base = i + (size_t)g.K * (j + (size_t)g.m * k);
+        for (int c = 0; c < 3; ++c) uh[base + c * cs] = cscale<T>(uh[base + c * cs], f);
+    }
+}
+
+// Counter-based Gaussian noise (src/spectral/ic.jl:31-48), physical grid.
+__device__ inline uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+template <typename T> __global__ void __launch_bounds__(256) k_sp_noise(T *v, uint64_t seed, int n)
+{
+    const size_t n3 = (size_t)n * n * n;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < 3 * n3; t += (size_t)gridDim.x * blockDim.x) {
+        size_t c = t / n3, r = t - c * n3;
+        size_t i = r % n, q = r / n, j = q % n, k = q / n;
+        uint64_t key = (((uint64_t)c * n + k) * n + j) * n + i;
+        uint64_t r1 = splitmix64(seed + 2 * key), r2 = splitmix64(seed + 2 * key + 1);
+        double u1 = ((double)(r1 >> 11) + 0.5) * 0x1p-53;
+        double u2 = ((double)(r2 >> 11) + 0.5) * 0x1p-53;
+        v[t] = T(sqrt(-2.0 * log(u1)) * cospi(2.0 * u2));
+    }
+}
+
+// Energy (slot 0) and Σ w k²|û|² (slot 1), compensated per block.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sp_energy(const cplx<T> *uh, SpecGeom g, dd *part)
+{
+    __shared__ dd s[256];
+    const size_t cs = (size_t)g.K * g.m * g.m;
+    dd e{0.0, 0.0}, w2{0.0, 0.0};
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < cs; t += (size_t)gridDim.x * blockDim.x) {
+        int i = t % g.K, r = t / g.K, j = r % g.m, k = r / g.m;
+        double a2 = 0;
+        for (int c = 0; c < 3; ++c) {
+            cplx<T> a = uh[t + c * cs];
+            a2 += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+        }
+        double k1 = g.kfac * i, k2 = g.kfac * sp_freq(j, g), k3 = g.kfac * sp_freq(k, g);
+        double w = i == 0 ? 1.0 : 2.0;
+        e = dd_add1(e, w * a2 / 2);
+        w2 = dd_add1(w2, w * a2 * (k1 * k1 + k2 * k2 + k3 * k3));
+    }
+    dd r0 = block_dd_sum(e, s);
+    dd r1 = block_dd_sum(w2, s);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r0;
+        part[gridDim.x + blockIdx.x] = r1;
+    }
+}
+
+template <typename T> __global__ void __launch_bounds__(256) k_sp_maxdiv(const cplx<T> *uh, SpecGeom g, double *part)
+{
+    __shared__ double s[256];
+    const size_t cs = (size_t)g.K * g.m * g.m;
+    double mx = 0;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < cs; t += (size_t)gridDim.x * blockDim.x) {
+        int i = t % g.K, r = t / g.K, j = r % g.m, k = r / g.m;
+        double k1 = g.kfac * i, k2 = g.kfac * sp_freq(j, g), k3 = g.kfac * sp_freq(k, g);
+        cplx<T> a = uh[t], b = uh[t + cs], c = uh[t + 2 * cs];
+        double re = k1 * a.x + k2 * b.x + k3 * c.x, im = k1 * a.y + k2 * b.y + k3 * c.y;
+        mx = fmax(mx, sqrt(re * re + im * im));
+    }
+    s[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+}  // namespace dns

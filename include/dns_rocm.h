@@ -126,6 +126,70 @@ int dns_fv_sync(dns_fv *h);
 int dns_fv_timing(dns_fv *h, int32_t enable);
 int dns_fv_timing_read(dns_fv *h, int32_t family, double *total_ms, int64_t *launches);
 
+/* ------------------------------------------------------------ spectral
+ * Pseudo-spectral HIT solver on a cubic periodic box, 2/3-truncated state.
+ * State layout (host and device): (K, m, m, 3) complex, K = kcut+1,
+ * m = 2kcut+1, first index fastest, compact frequency order
+ * (src/spectral/operators.jl:7); host buffers interleave (re, im) doubles.
+ * Physical layout: (n, n, n, 3) real on the cell-centred transform grid. */
+
+typedef struct dns_spec dns_spec;
+
+/* Replaces the keyword arguments of `spectral_setup`
+ * (src/spectral/setup.jl:33-48); kcut must be given (n ÷ 3 by default in the
+ * reference). */
+typedef struct dns_spec_params {
+    int32_t n;
+    int32_t kcut;
+    double l;
+    double visc;
+    int32_t precision; /* DNS_FP64, or DNS_MIXED = FP32 FFTs/state with FP64 reductions */
+} dns_spec_params;
+
+/* spectral_setup(; ...) -- src/spectral/setup.jl:33-158. */
+int dns_spec_create(const dns_spec_params *params, dns_spec **out);
+int dns_spec_destroy(dns_spec *h);
+
+/* which: 0 = uh (the state), 1 = ustart (the second Wray register),
+ * 2 = du (the last dns_spec_rhs result; read only). */
+int dns_spec_set_state(dns_spec *h, int32_t which, const double *host);
+int dns_spec_get_state(dns_spec *h, int32_t which, double *host);
+
+/* The physical buffer (s.fft.v / s.fft.rbuf of the reference). */
+int dns_spec_set_physical(dns_spec *h, const double *host);
+int dns_spec_get_physical(dns_spec *h, double *host);
+
+/* spec_to_phys!(v, uh, s) -- src/spectral/fft.jl:200-214 (state → physical). */
+int dns_spec_spec_to_phys(dns_spec *h);
+/* phys_to_spec!(uh, w, s) -- src/spectral/fft.jl:221-232 (physical → state). */
+int dns_spec_phys_to_spec(dns_spec *h);
+/* spectral_project!(uh, s) -- src/spectral/operators.jl:25-29. */
+int dns_spec_project(dns_spec *h);
+/* spectral_rhs!(du, uh, s) -- src/spectral/operators.jl:60-79 (into du). */
+int dns_spec_rhs(dns_spec *h);
+/* spectral_step!(uh, s, Δt, cache) -- src/spectral/step.jl:68-92; writes
+ * vsummax of the first stage to *vmax when vmax is not NULL (synchronises);
+ * asynchronous otherwise. */
+int dns_spec_step(dns_spec *h, double dt, double *vmax);
+/* vsummax(spec_to_phys(uh)) -- src/spectral/step.jl:54-60, 140-145. */
+int dns_spec_vsummax(dns_spec *h, double *out);
+/* shell_energies(uh, s, s0:s0+nshells-1) -- src/spectral/forcing.jl:12-24. */
+int dns_spec_shell_energies(dns_spec *h, int32_t s0, int32_t nshells, double *E);
+/* rescale_shell! for shells s0..s0+nshells-1 by factors[] -- src/spectral/forcing.jl:27-35. */
+int dns_spec_rescale_shells(dns_spec *h, int32_t s0, int32_t nshells, const double *factors);
+/* noise_kernel! into the physical buffer -- src/spectral/ic.jl:38-48. */
+int dns_spec_noise(dns_spec *h, uint64_t seed);
+/* spectral_energy / spectral_dissipation / spectral_maxdiv --
+ * src/spectral/operators.jl:87-110 (compensated FP64 sums). */
+int dns_spec_energy(dns_spec *h, double *out);
+int dns_spec_dissipation(dns_spec *h, double *out);
+int dns_spec_maxdiv(dns_spec *h, double *out);
+int dns_spec_sync(dns_spec *h);
+/* Kernel timing hook, as dns_fv_timing; family: 0 y pass, 1 z pass,
+ * 2 fused x pass, 3 y pass + IF stage, 4 forcing. */
+int dns_spec_timing(dns_spec *h, int32_t enable);
+int dns_spec_timing_read(dns_spec *h, int32_t family, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

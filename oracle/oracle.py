@@ -156,3 +156,159 @@ class OracleFV:
 
 def splitmix64(x: int) -> int:
     return int(lib().ospec_splitmix64(ct.c_uint64(x & 0xFFFFFFFFFFFFFFFF)))
+
+
+# ----------------------------------------------------------------- spectral
+
+_cp = ct.POINTER(ct.c_double)
+
+
+def _pc(a):
+    assert a.dtype == np.complex128 and a.flags.f_contiguous, "need complex128 F-order"
+    return a.ctypes.data_as(_cp)
+
+
+def _spec_sigs(L):
+    L.ospec_create.restype = ct.c_void_p
+    L.ospec_create.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double]
+    L.ospec_destroy.argtypes = [ct.c_void_p]
+    for name in ("ospec_rhs", "ospec_step", "ospec_vsummax", "ospec_energy", "ospec_dissipation", "ospec_maxdiv"):
+        getattr(L, name).restype = ct.c_double
+    L.ospec_step.argtypes = [ct.c_void_p, _cp, ct.c_double, _cp, _cp]
+
+
+class OracleSpectral:
+    """The reference's single-rank pseudo-spectral CPU path (FP64).
+
+    State arrays: complex (K, m, m, 3) F-order; physical: real (n, n, n, 3)."""
+
+    def __init__(self, prob):
+        self.prob = prob
+        self.L = lib()
+        _spec_sigs(self.L)
+        self.h = ct.c_void_p(self.L.ospec_create(prob.n, prob.kcut, prob.l, prob.visc))
+        self.n, self.kcut, self.K, self.m = prob.n, prob.kcut, prob.K, prob.m
+
+    def __del__(self):
+        try:
+            self.L.ospec_destroy(self.h)
+        except Exception:
+            pass
+
+    def specvelocity(self):
+        return np.zeros((self.K, self.m, self.m, 3), dtype=np.complex128, order="F")
+
+    def physfield(self):
+        return np.zeros((self.n, self.n, self.n, 3), dtype=np.float64, order="F")
+
+    def spec_to_phys(self, uh, v=None):
+        v = self.physfield() if v is None else v
+        self.L.ospec_spec_to_phys(self.h, _pc(uh), _p(v))
+        return v
+
+    def phys_to_spec(self, w, uh=None):
+        uh = self.specvelocity() if uh is None else uh
+        self.L.ospec_phys_to_spec(self.h, _p(np.asfortranarray(w)), _pc(uh))
+        return uh
+
+    def project(self, uh):
+        self.L.ospec_project(self.h, _pc(uh))
+        return uh
+
+    def rhs(self, uh):
+        du = self.specvelocity()
+        vmax = self.L.ospec_rhs(self.h, _pc(du), _pc(uh))
+        return du, vmax
+
+    def step(self, uh, dt, cache=None):
+        cache = cache or (self.specvelocity(), self.specvelocity())
+        return self.L.ospec_step(self.h, _pc(uh), ct.c_double(dt), _pc(cache[0]), _pc(cache[1]))
+
+    def shell_energies(self, uh, shells):
+        shells = list(shells)
+        s0, ns = min(shells), max(shells) - min(shells) + 1
+        E = np.zeros(ns)
+        self.L.ospec_shell_energies(self.h, _pc(uh), ct.c_int(s0), ct.c_int(ns), _p(E))
+        return np.array([E[s - s0] for s in shells])
+
+    def rescale_shell(self, uh, sh, factor):
+        self.L.ospec_rescale_shell(self.h, _pc(uh), ct.c_int(sh), ct.c_double(factor))
+
+    def energy(self, uh):
+        return self.L.ospec_energy(self.h, _pc(uh))
+
+    def dissipation(self, uh):
+        return self.L.ospec_dissipation(self.h, _pc(uh))
+
+    def maxdiv(self, uh):
+        return self.L.ospec_maxdiv(self.h, _pc(uh))
+
+    def noise(self, seed):
+        v = self.physfield()
+        self.L.ospec_noise(self.h, ct.c_uint64(seed), _p(v))
+        return v
+
+    # host logic of the reference ---------------------------------------
+    def velocityfield(self, fx, fy, fz, project=True):
+        """spectral_velocityfield! (src/spectral/ic.jl:13-25)."""
+        n, l = self.n, self.prob.l
+        x = l / n * (np.arange(1, n + 1) - 0.5)
+        X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+        w = self.physfield()
+        for c, f in enumerate((fx, fy, fz)):
+            w[..., c] = np.vectorize(f)(X, Y, Z)
+        uh = self.phys_to_spec(w)
+        if project:
+            self.project(uh)
+        return uh
+
+    def randomfield(self, totalenergy=1.0, kpeak=4.0, seed=0, profile=None):
+        """spectral_randomfield! (src/spectral/ic.jl:59-84)."""
+        profile = profile or (lambda k: k ** 4 * np.exp(-2 * (k / kpeak) ** 2))
+        uh = self.phys_to_spec(self.noise(seed))
+        self.project(uh)
+        kdiag = int(np.floor(np.sqrt(3) * self.kcut))
+        E = self.shell_energies(uh, range(0, kdiag + 1))
+        tot = sum(profile(k) for k in range(0, kdiag + 1))
+        for sh, e in zip(range(0, kdiag + 1), E):
+            e0 = totalenergy * profile(sh) / tot
+            self.rescale_shell(uh, sh, np.sqrt(e0 / e) if e > 0 else 0.0)
+        return uh
+
+    def shellforcing(self, uh, shells=(1, 2), eref=None):
+        """shellforcing (src/spectral/forcing.jl:47-56)."""
+        shells = list(shells)
+        eref = list(self.shell_energies(uh, shells) if eref is None else eref)
+
+        def forcing(uh2):
+            e = self.shell_energies(uh2, shells)
+            for sh, e0, e1 in zip(shells, eref, e):
+                self.rescale_shell(uh2, sh, np.sqrt(e0 / e1) if e1 > 0 else 1.0)
+
+        return forcing
+
+    def solve(self, uh, tlims, dt=None, cfl=0.85, forcing=None):
+        """spectral_solve! host loop (src/spectral/step.jl:118-170), no tstops."""
+        self.project(uh)
+        kmax = 2 * np.pi * self.kcut / self.prob.l
+        eps = 2.220446049250313e-16
+        vmax = 0.0
+        if dt is None:
+            v = self.spec_to_phys(uh)
+            vmax = self.L.ospec_vsummax(self.h, _p(v))
+        t, tend = float(tlims[0]), float(tlims[1])
+        tol = 1e-10 * (abs(tend) + 1)
+        n = 0
+        cache = (self.specvelocity(), self.specvelocity())
+        dts = []
+        while t < tend - tol:
+            dtn = min(dt if dt is not None else cfl * np.sqrt(3) / (kmax * (vmax + eps)), tend - t)
+            vloc = self.step(uh, dtn, cache)
+            if forcing is not None:
+                forcing(uh)
+            if dt is None:
+                vmax = vloc
+            t += dtn
+            n += 1
+            dts.append(dtn)
+        return dict(uh=uh, t=t, n=n, dts=dts)
