@@ -42,12 +42,13 @@ def channel_ic(prob, seed=0, amp=0.1):
     u = [np.zeros(prob.ghosted_shape, order="F") for _ in range(3)]
     i = np.arange(nx, dtype=np.uint64)[:, None, None]
     j = np.arange(ny, dtype=np.uint64)[None, :, None]
-    k = np.arange(nz, dtype=np.uint64)[None, None, :]
     with np.errstate(over="ignore"):
         for c in range(3):
-            key = ((np.uint64(c) * np.uint64(nz) + k) * np.uint64(ny) + j) * np.uint64(nx) + i
-            r = ((splitmix64(np.uint64(seed) + key) >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
-            u[c][1:-1, 1:-1, 1:-1] = amp * (2 * r - 1)
+            for k0 in range(0, nz, 16):  # k-slabs bound the temporaries
+                k = np.arange(k0, min(nz, k0 + 16), dtype=np.uint64)[None, None, :]
+                key = ((np.uint64(c) * np.uint64(nz) + k) * np.uint64(ny) + j) * np.uint64(nx) + i
+                r = ((splitmix64(np.uint64(seed) + key) >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+                u[c][1:-1, 1:-1, 1 + k0:1 + k0 + k.shape[2]] = amp * (2 * r - 1)
     u[0][1:-1, 1:-1, 1:-1] += (1 - co["yc"] ** 2)[None, :, None]
     if prob.wall:
         u[1][:, -2, :] = 0.0  # top wall face
@@ -60,6 +61,99 @@ def channel_problem(n, ydiffusion="explicit", precision="fp64"):
     return FVProblem(n=n, lims=((0.0, 4 * math.pi), (-1.0, 1.0), (0.0, 4 * math.pi / 3)),
                      bc=("periodic", "wall", "periodic"), stretch=tanhstretch(1.5), visc=1 / 180,
                      bodyforce=(1.0, 0.0, 0.0), ydiffusion=ydiffusion, precision=precision)
+
+
+def tgv_problem(n=64):
+    from distributednavierstokes_amd.problem import FVProblem
+
+    return FVProblem(n=(n, n, n), visc=1 / 1600)
+
+
+def tgv_ic(prob):
+    co = prob.coordinates()
+    u = [np.zeros(prob.ghosted_shape, order="F") for _ in range(3)]
+    u[0][1:-1, 1:-1, 1:-1] = (np.sin(co["xf"])[:, None, None] * np.cos(co["yc"])[None, :, None]
+                              * np.cos(co["zc"])[None, None, :])
+    u[1][1:-1, 1:-1, 1:-1] = (-np.cos(co["xc"])[:, None, None] * np.sin(co["yf"])[None, :, None]
+                              * np.cos(co["zc"])[None, None, :])
+    return u
+
+
+def spectral_mmin_bytes(n, kcut, esize):
+    """Minimal-pass model of one IF-RK3 step (complex elements × 2·esize):
+    per stage y back 3Km² + 3Knm, z back 3Knm + 3Kn², fused x 3Kn² + 6Kn²,
+    z forward 6Kn² + 6Knm, y forward + RHS + IF + projection 6Knm + 9Km²
+    (+3Km² in stage 1 for the register copy)."""
+    K, m = kcut + 1, 2 * kcut + 1
+    Km2, Knm, Kn2 = K * m * m, K * n * m, K * n * n
+    elems = 39 * Km2 + 54 * Knm + 54 * Kn2
+    return elems * 2 * esize
+
+
+def run_spectral(n, kcut, precision, steps, warmup, cfl=0.5):
+    import ctypes as ct
+
+    from distributednavierstokes_amd import _lib
+    import distributednavierstokes_amd as dns
+
+    s = dns.spectral_setup(n=n, kcut=kcut, visc=2e-4, T=np.float32 if precision == "mixed" else np.float64)
+    L = _lib.lib()
+    dns.spectral_randomfield_(dns.spectral.DeviceState(s), s, totalenergy=1.0, kpeak=4, seed=0)
+    forcing = dns.shellforcing(dns.spectral.DeviceState(s), s, shells=(1, 2))
+    out = np.zeros(1)
+    _lib.check(L.dns_spec_vsummax(s.handle, _lib.dptr(out)))
+    vmax = float(out[0])
+    kmax = 2 * math.pi * kcut / s.l
+
+    def one_step():
+        nonlocal vmax
+        dt = cfl * math.sqrt(3) / (kmax * (vmax + 1e-16))
+        _lib.check(L.dns_spec_step(s.handle, dt, _lib.dptr(out)))
+        forcing(dns.spectral.DeviceState(s), s)
+        vmax = float(out[0])
+
+    for _ in range(warmup):
+        one_step()
+    _lib.check(L.dns_spec_sync(s.handle))
+    _lib.check(L.dns_spec_timing(s.handle, 1))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    _lib.check(L.dns_spec_sync(s.handle))
+    t1 = time.perf_counter()
+    fam = {}
+    names = {0: "y_pass", 1: "z_pass", 2: "x_pass_products", 3: "y_pass_if_stage", 4: "forcing"}
+    for f, nm in names.items():
+        ms, cnt = ct.c_double(), ct.c_int64()
+        _lib.check(L.dns_spec_timing_read(s.handle, f, ct.byref(ms), ct.byref(cnt)))
+        if cnt.value:
+            fam[nm] = {"ms_per_step": ms.value / steps, "launches_per_step": cnt.value / steps}
+    _lib.check(L.dns_spec_timing(s.handle, 0))
+    e = s.scalar(L.dns_spec_energy)
+    md = s.scalar(L.dns_spec_maxdiv)
+    esize = 4 if precision == "mixed" else 8
+    sps = steps / (t1 - t0)
+    b = spectral_mmin_bytes(n, kcut, esize)
+    return {"workload": f"HIT {n}^3 kcut={kcut} {precision}, shell forcing k<=2, nu=2e-4, CFL {cfl}",
+            "steps_per_s": sps, "ms_per_step": 1e3 / sps, "mcell_updates_per_s": n ** 3 * sps / 1e6,
+            "bytes_per_step": b, "achieved_gbs": b * sps / 1e9, "frac": b * sps / 1e9 / HBM_PEAK_GBS,
+            "kernels": fam, "checks": {"energy": e, "maxdiv": md}}
+
+
+def fv_form(prob, steps, warmup, fixed_dt=None, ic=None):
+    r = run_gpu(prob, steps, warmup, fixed_dt=fixed_dt, ic=ic)
+    N = int(np.prod(prob.n))
+    esize = 4 if prob.precision == "mixed" else 8
+    sps = steps / r["seconds"]
+    units = mmin_units_per_step(prob, cfl=fixed_dt is None)
+    b = units * N * esize
+    return {"workload": f"FV {prob.n[0]}x{prob.n[1]}x{prob.n[2]} {prob.bc[1]} y, {prob.ydiffusion}, {prob.precision}, "
+                        + ("fixed dt" if fixed_dt else "CFL 0.5"),
+            "steps_per_s": sps, "ms_per_step": 1e3 / sps, "mcell_updates_per_s": N * sps / 1e6,
+            "units_per_step": units, "bytes_per_step": b, "achieved_gbs": b * sps / 1e9,
+            "frac": b * sps / 1e9 / HBM_PEAK_GBS,
+            "kernels": {FAMILIES[f]: v[0] / steps for f, v in r["fam"].items()},
+            "checks": {"maxdiv": r["maxdiv"], "kinetic_energy": r["ke"]}}
 
 
 def mmin_units_per_step(prob, cfl=True):
@@ -93,7 +187,7 @@ def family_bytes_per_launch(prob, fam, esize):
     return 0.0
 
 
-def run_gpu(prob, steps, warmup, cfl=0.5):
+def run_gpu(prob, steps, warmup, cfl=0.5, fixed_dt=None, ic=None):
     import ctypes as ct
 
     from distributednavierstokes_amd import _lib
@@ -101,7 +195,7 @@ def run_gpu(prob, steps, warmup, cfl=0.5):
 
     L = _lib.lib()
     s = Setup(prob, ROCmBackend())
-    u = channel_ic(prob)
+    u = (ic or channel_ic)(prob)
     s.upload_u(u)
     zero = np.zeros(prob.ghosted_shape, order="F")
     for name in ("p", "qx", "qy", "qz"):
@@ -114,9 +208,13 @@ def run_gpu(prob, steps, warmup, cfl=0.5):
 
     def one_step():
         nonlocal t
-        _lib.check(L.dns_fv_cfl_timestep(s.handle, cfl, _lib.dptr(out)))
-        _lib.check(L.dns_fv_step(s.handle, float(out[0])))
-        t += float(out[0])
+        if fixed_dt is None:
+            _lib.check(L.dns_fv_cfl_timestep(s.handle, cfl, _lib.dptr(out)))
+            dt = float(out[0])
+        else:
+            dt = fixed_dt
+        _lib.check(L.dns_fv_step(s.handle, dt))
+        t += dt
 
     for _ in range(warmup):
         one_step()
@@ -170,6 +268,7 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the other configurations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scales the FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -213,6 +312,14 @@ def main():
                "sample": f"{args.cpu_steps} CFL steps of the same {prob.n[0]}x{prob.n[1]}x{prob.n[2]} "
                          f"workload after 1 warm-up step (FP64 C oracle, reference-literal kernel sequence, "
                          f"own FFT instead of FFTW, OpenMP)"}
+    forms = None
+    if not args.no_forms:
+        forms = {}
+        forms["tgv64_fp64"] = fv_form(tgv_problem(64), 50, 5, fixed_dt=1e-3, ic=tgv_ic)
+        for prec in ("fp64", "mixed"):
+            forms[f"channel_cn_512x384x512_{prec}"] = fv_form(channel_problem((512, 384, 512), "implicit", prec), 10, 2)
+        for prec in ("fp64", "mixed"):
+            forms[f"hit512_{prec}"] = run_spectral(512, 170, prec, 5, 1)
     line = {
         "metric": "RK3 steps/s & Mcells/s, FP64 vs mixed at stated err; %HBM roofline; LES err",
         "value": sps,
@@ -237,6 +344,7 @@ def main():
         "kernels": fam_summary,
         "checks": {"maxdiv": r["maxdiv"], "kinetic_energy": r["ke"]},
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
 

@@ -56,7 +56,7 @@ static const double RK_A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 static const double RK_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 static const double RK_ALPHA[3] = {1.0 / 3.0, 5.0 / 12.0, 1.0 / 4.0};
 
-constexpr int X_EPT = 8, ZY_EPT = 16, NTHR = 256;
+constexpr int X_EPT = 8, ZY_EPT = 8, NTHR = 256;
 constexpr int CH_NCHUNK = 16;
 constexpr int RED_BLOCKS = 1024;
 
@@ -78,10 +78,12 @@ template <typename T> struct FVImpl : FVBase {
     cplx<T> *twN = nullptr, *twx = nullptr, *twz = nullptr, *twy = nullptr;
     int lpb = 1, tbz = 1, tby = 1;
     size_t ldsx = 0, ldsz = 0, ldsy = 0;
-    double *red = nullptr, *redout = nullptr;
+    double *red = nullptr, *redout = nullptr, *upart = nullptr;
     size_t redcap = 0;
     double *pinned = nullptr;
     int N = 0;
+    int nb_pgrad = 0;
+    bool umax_valid = false; // upart holds max |u_c| partials of the current u
 
     ~FVImpl() override
     {
@@ -89,7 +91,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.flush();
         void *ptrs[] = {u[0], u[1], u[2], v[0], v[1], v[2], q[0], q[1], q[2], p, delta, A, Dth, Dcn, dy_t, rdy_t,
                         rdyu_t, dy_d, dyu_d, rdy_d, rdyu_d, lam1, lam2, lam3, ay, cy, iptab, ydc[0], ydc[1], ydc[2],
-                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout};
+                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout, upart};
         for (void *q_ : ptrs)
             if (q_) hipFree(q_);
         if (pinned) hipHostFree(pinned);
@@ -103,6 +105,10 @@ template <typename T> struct FVImpl : FVBase {
         g.ny = ny;
         g.nz = nz;
         g.nxc = nx / 2 + 1;
+        {
+            const int per = 128 / (int)sizeof(cplx<T>); // complex values per 128-byte line
+            g.ldc = (g.nxc + per - 1) / per * per;
+        }
         g.wall = P->ywall ? 1 : 0;
         imexy = P->ydiffusion ? 1 : 0;
         visc = P->visc;
@@ -150,7 +156,7 @@ template <typename T> struct FVImpl : FVBase {
         }
         DNS_TRY(dalloc(&p, N));
         if (imexy) DNS_TRY(dalloc(&delta, N));
-        DNS_TRY(dalloc(&A, (size_t)g.nxc * ny * nz));
+        DNS_TRY(dalloc(&A, (size_t)g.ldc * ny * nz));
 
         std::vector<T> dyt(ny + 2), rdyt(ny + 2), rdyut(ny + 2);
         std::vector<double> rdyh(ny + 2), rdyuh(ny + 2);
@@ -211,7 +217,7 @@ template <typename T> struct FVImpl : FVBase {
                     }
                 }
             DNS_TRY(dupload(&iptab, ip));
-            DNS_TRY(dalloc(&Dth, (size_t)g.nxc * ny * nz));
+            DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
         } else {
             std::vector<double> l2(ny);
             for (int k = 0; k < ny; ++k) l2[k] = modlam(k, ny, dyh[1]);
@@ -278,6 +284,8 @@ template <typename T> struct FVImpl : FVBase {
 
         redcap = std::max<size_t>({(size_t)RED_BLOCKS * 4, (size_t)16 * ny * CH_NCHUNK + 16});
         DNS_TRY(dalloc(&red, redcap));
+        nb_pgrad = ceil_div(nx, 64) * ceil_div(ny, 4) * nz;
+        DNS_TRY(dalloc(&upart, (size_t)3 * nb_pgrad));
         DNS_TRY(dalloc(&redout, (size_t)8 * ny + 64));
         DNS_HIP_CHECK(hipHostMalloc((void **)&pinned, sizeof(double) * ((size_t)8 * ny + 64)));
         return DNS_OK;
@@ -322,6 +330,7 @@ template <typename T> struct FVImpl : FVBase {
                 for (int i = 0; i < nx; ++i) s[i + (size_t)nx * ((ny - 1) + (size_t)ny * k)] = T(0);
         DNS_HIP_CHECK(hipMemcpyAsync(d, s.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
         DNS_HIP_CHECK(hipStreamSynchronize(st));
+        if (fid <= DNS_FIELD_UZ) umax_valid = false;
         return DNS_OK;
     }
 
@@ -449,7 +458,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.begin(FAM_Y);
         if (g.wall) {
             double scale = 1.0 / ((double)g.nx * g.nz);
-            int lines = g.nxc * g.nz;
+            int lines = g.ldc * g.nz;
             hipLaunchKernelGGL((k_thomas_y<T>), dim3(ceil_div(lines, NTHR)), dim3(NTHR), 0, st, A, Dth, iptab, ay,
                                cy, lam1, lam3, scale, g);
         } else {
@@ -493,10 +502,11 @@ template <typename T> struct FVImpl : FVBase {
         a.inv_adt = (T)inv_adt;
         dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
         timer.begin(FAM_PGRAD);
-        if (pupdate) hipLaunchKernelGGL((k_pgrad<T, true>), grd, blk, 0, st, a, g);
-        else hipLaunchKernelGGL((k_pgrad<T, false>), grd, blk, 0, st, a, g);
+        if (pupdate) hipLaunchKernelGGL((k_pgrad<T, true, true>), grd, blk, 0, st, a, g, upart);
+        else hipLaunchKernelGGL((k_pgrad<T, false, true>), grd, blk, 0, st, a, g, upart);
         timer.end(FAM_PGRAD);
         DNS_HIP_CHECK(hipGetLastError());
+        umax_valid = true;
         return DNS_OK;
     }
 
@@ -571,9 +581,14 @@ template <typename T> struct FVImpl : FVBase {
 
     int cfl(double cflv, double *dt) override
     {
-        const int nb = std::min(RED_BLOCKS, ceil_div(N, NTHR));
-        hipLaunchKernelGGL((k_maxabs3<T>), dim3(nb), dim3(NTHR), 0, st, u[0], u[1], u[2], N, red);
-        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, red, nb, 3, redout);
+        if (umax_valid) {
+            // partial maxima left by the pressure-gradient pass that produced u
+            hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, upart, nb_pgrad, 3, redout);
+        } else {
+            const int nb = std::min(RED_BLOCKS, ceil_div(N, NTHR));
+            hipLaunchKernelGGL((k_maxabs3<T>), dim3(nb), dim3(NTHR), 0, st, u[0], u[1], u[2], N, red);
+            hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, red, nb, 3, redout);
+        }
         DNS_HIP_CHECK(hipGetLastError());
         DNS_TRY(read_back(3));
         const double eps = 2.220446049250313e-16;

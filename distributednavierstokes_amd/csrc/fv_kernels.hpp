@@ -19,13 +19,39 @@
 
 namespace dns {
 
+// nxc = nx/2 + 1 rfft coefficients per x-line; the complex pencil A is
+// stored with leading dimension ldc >= nxc rounded up to whole 128-byte
+// lines, so every tile of the z and y passes starts on a cache-line boundary.
 struct FVGeom {
-    int nx, ny, nz, nxc;
+    int nx, ny, nz, nxc, ldc;
     int wall;
 };
 
 __device__ inline int IX(int i, int j, int k, const FVGeom &g) { return i + g.nx * (j + g.ny * k); }
-__device__ inline int CX(int i, int j, int k, const FVGeom &g) { return i + g.nxc * (j + g.ny * k); }
+__device__ inline int CX(int i, int j, int k, const FVGeom &g) { return i + g.ldc * (j + g.ny * k); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
+// with its own L2.  This bijection of linear block ids gives every XCD one
+// contiguous range of the domain, so the y/z stencil neighbours a block
+// reads were mostly fetched into the same L2 by the blocks before it.
+__device__ inline int xcd_block(int bid, int nb)
+{
+    const int X = 8;
+    const int q = nb / X, r = nb - q * X;
+    const int xcd = bid % X, slot = bid / X;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// 3D block coordinates after the XCD remap of a (gx, gy, gz) grid.
+__device__ inline void xcd_block3(int &bx, int &by, int &bz)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nb = gx * gy * gridDim.z;
+    const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+    bx = lb % gx;
+    by = (lb / gx) % gy;
+    bz = lb / (gx * gy);
+}
 __device__ inline int wrapm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ inline int wrapp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
 
@@ -76,9 +102,11 @@ template <typename T> struct MomentumArgs {
 template <typename T, bool READ_Q, bool WRITE_V>
 __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = bx * 64 + threadIdx.x;
+    const int j = by * 4 + threadIdx.y;
+    const int k = bz;
     if (i >= g.nx || j >= g.ny) return;
     const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
     const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
@@ -188,7 +216,7 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
-    const int L0 = blockIdx.x * lpb;
+    const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
     for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
         int b = idx / N, t = idx - b * N;
         int L = L0 + b;
@@ -226,7 +254,7 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
         // X = (Zk + Zc)/2 − (i/2) w^kx (Zk − Zc)
         cplx<T> wd = cmul<T>(a.twx[kx], df);
         cplx<T> X = mk<T>(T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x));
-        a.A[kx + nxc * L] = X;
+        a.A[kx + (size_t)g.ldc * L] = X;
     }
 }
 
@@ -248,16 +276,16 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int N = d.n, lpb = a.lpb, nxc = N + 1;
+    const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
-    const int L0 = blockIdx.x * lpb;
+    const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
     for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
         int b = idx / N, k = idx - b * N;
         int L = L0 + b;
         cplx<T> Z = mk<T>(T(0), T(0));
         if (L < nlines) {
-            cplx<T> Xk = a.A[k + nxc * L];
-            cplx<T> Xn = a.A[(N - k) + nxc * L];
+            cplx<T> Xk = a.A[k + (size_t)g.ldc * L];
+            cplx<T> Xn = a.A[(N - k) + (size_t)g.ldc * L];
             if (k == 0) {
                 Xk.y = T(0);
                 Xn.y = T(0);
@@ -359,43 +387,71 @@ __global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw
 // FP64 scratch D; `scale` folds 1/(nx nz).
 // ------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *A, double2 *D, const double *__restrict__ ip,
-                                                  const double *__restrict__ ay, const double *__restrict__ cy,
-                                                  const double *lam1, const double *lam3, double scale, FVGeom g)
+__global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *__restrict__ A, double2 *__restrict__ D,
+                                                  const double *__restrict__ ip, const double *__restrict__ ay,
+                                                  const double *__restrict__ cy, const double *__restrict__ lam1,
+                                                  const double *__restrict__ lam3, double scale, FVGeom g)
 {
+    constexpr int CH = 8; // rows loaded ahead of the dependent recurrence
     const int L = blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= g.nxc * g.nz) return;
-    const int kx = L % g.nxc, kz = L / g.nxc;
+    const int kx = L % g.ldc, kz = L / g.ldc;
+    if (kx >= g.nxc || kz >= g.nz) return;
     const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
     const bool mean = (lam1[kx] + lam3[kz]) == 0.0;
     const int ny = g.ny;
-    const size_t ipbase = (size_t)kx + (size_t)g.nxc * (size_t)ny * kzs;
-    const size_t cstride = g.nxc, base = (size_t)kx + (size_t)g.nxc * (size_t)ny * kz;
-    // forward sweep
-    double2 dp;
-    {
-        cplx<T> x = A[base];
-        double w = ip[ipbase];
-        dp = mean ? make_double2(0.0, 0.0) : make_double2((double)x.x * scale * w, (double)x.y * scale * w);
-        D[base] = dp;
+    const size_t ipbase = (size_t)kx + (size_t)g.nxc * (size_t)ny * kzs, ips = g.nxc;
+    const size_t base = (size_t)kx + (size_t)g.ldc * (size_t)ny * kz, cs = g.ldc;
+    // forward sweep: d'_j = (r_j − a_j d'_{j−1}) / β_j
+    double2 dp = make_double2(0.0, 0.0);
+    for (int j0 = 0; j0 < ny; j0 += CH) {
+        cplx<T> xr[CH];
+        double wr[CH], ar[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int j = j0 + q;
+            if (j < ny) {
+                xr[q] = A[base + j * cs];
+                wr[q] = ip[ipbase + j * ips];
+                ar[q] = ay[j];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int j = j0 + q;
+            if (j < ny) {
+                if (j == 0) {
+                    dp = mean ? make_double2(0.0, 0.0)
+                              : make_double2((double)xr[q].x * scale * wr[q], (double)xr[q].y * scale * wr[q]);
+                } else {
+                    dp = make_double2(((double)xr[q].x * scale - ar[q] * dp.x) * wr[q],
+                                      ((double)xr[q].y * scale - ar[q] * dp.y) * wr[q]);
+                }
+                D[base + j * cs] = dp;
+            }
+        }
     }
-#pragma unroll 4
-    for (int j = 1; j < ny; ++j) {
-        cplx<T> x = A[base + j * cstride];
-        double w = ip[ipbase + j * cstride];
-        double a = ay[j];
-        dp = make_double2(((double)x.x * scale - a * dp.x) * w, ((double)x.y * scale - a * dp.y) * w);
-        D[base + j * cstride] = dp;
-    }
-    // back substitution, c'_j = c_j / β_j
+    // back substitution: x_j = d'_j − c'_j x_{j+1}, c'_j = c_j / β_j
     double2 xn = dp;
-    A[base + (ny - 1) * cstride] = mk<T>(T(xn.x), T(xn.y));
-#pragma unroll 4
-    for (int j = ny - 2; j >= 0; --j) {
-        double2 d2 = D[base + j * cstride];
-        double cpj = (j == 0 && mean) ? 0.0 : cy[j] * ip[ipbase + j * cstride];
-        xn = make_double2(d2.x - cpj * xn.x, d2.y - cpj * xn.y);
-        A[base + j * cstride] = mk<T>(T(xn.x), T(xn.y));
+    A[base + (size_t)(ny - 1) * cs] = mk<T>(T(xn.x), T(xn.y));
+    for (int j1 = ny - 2; j1 >= 0; j1 -= CH) {
+        double2 dr[CH];
+        double cr[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int j = j1 - q;
+            if (j >= 0) {
+                dr[q] = D[base + j * cs];
+                cr[q] = (j == 0 && mean) ? 0.0 : cy[j] * ip[ipbase + j * ips];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int j = j1 - q;
+            if (j >= 0) {
+                xn = make_double2(dr[q].x - cr[q] * xn.x, dr[q].y - cr[q] * xn.y);
+                A[base + j * cs] = mk<T>(T(xn.x), T(xn.y));
+            }
+        }
     }
 }
 
@@ -411,24 +467,54 @@ template <typename T> struct PGradArgs {
     T rdx, rdz, s, inv_adt;
 };
 
-template <typename T, bool PUPDATE>
-__global__ void __launch_bounds__(256) k_pgrad(PGradArgs<T> a, FVGeom g)
+template <typename T, bool PUPDATE, bool MAXU>
+__global__ void __launch_bounds__(256) k_pgrad(PGradArgs<T> a, FVGeom g, double *umax_part)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-    const int c = IX(i, j, k, g);
-    const T pc = a.psi[c];
-    const T pxp = a.psi[IX(wrapp(i, g.nx), j, k, g)];
-    const T pzp = a.psi[IX(i, j, wrapp(k, g.nz), g)];
-    a.ux[c] -= a.s * (pxp - pc) * a.rdx;
-    if (!(g.wall && j == g.ny - 1)) {
-        const T pyp = ldc(a.psi, i, j + 2, k, g, T(1));
-        a.uy[c] -= a.s * (pyp - pc) * a.rdyu[j + 1];
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = bx * 64 + threadIdx.x;
+    const int j = by * 4 + threadIdx.y;
+    const int k = bz;
+    double m0 = 0, m1 = 0, m2 = 0;
+    if (i < g.nx && j < g.ny) {
+        const int c = IX(i, j, k, g);
+        const T pc = a.psi[c];
+        const T pxp = a.psi[IX(wrapp(i, g.nx), j, k, g)];
+        const T pzp = a.psi[IX(i, j, wrapp(k, g.nz), g)];
+        const T ux = a.ux[c] - a.s * (pxp - pc) * a.rdx;
+        a.ux[c] = ux;
+        T uy = a.uy[c];
+        if (!(g.wall && j == g.ny - 1)) {
+            const T pyp = ldc(a.psi, i, j + 2, k, g, T(1));
+            uy -= a.s * (pyp - pc) * a.rdyu[j + 1];
+            a.uy[c] = uy;
+        }
+        const T uz = a.uz[c] - a.s * (pzp - pc) * a.rdz;
+        a.uz[c] = uz;
+        if (PUPDATE) a.p[c] += pc * a.inv_adt;
+        m0 = fabs((double)ux);
+        m1 = fabs((double)uy);
+        m2 = fabs((double)uz);
     }
-    a.uz[c] -= a.s * (pzp - pc) * a.rdz;
-    if (PUPDATE) a.p[c] += pc * a.inv_adt;
+    if (MAXU) {
+        // per-block max |u_c| of the updated field: the CFL bound of the
+        // next step (src/timestep.jl:72-76) without another pass over u
+        __shared__ double s[3][256];
+        const int t = threadIdx.x + 64 * threadIdx.y;
+        s[0][t] = m0;
+        s[1][t] = m1;
+        s[2][t] = m2;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w)
+                for (int q = 0; q < 3; ++q) s[q][t] = fmax(s[q][t], s[q][t + w]);
+            __syncthreads();
+        }
+        if (t < 3) {
+            const int lb = bx + gridDim.x * (by + gridDim.y * bz);
+            umax_part[(size_t)lb * 3 + t] = s[t][0];
+        }
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -632,10 +718,11 @@ __global__ void __launch_bounds__(256) k_binshells(const cplx<T> *A, FVGeom g, i
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
         int kx = t % g.nxc, r = t / g.nxc, j = r % g.ny, k = r / g.ny;
         int ky = j > g.ny / 2 ? j - g.ny : j;
+        const size_t at = kx + (size_t)g.ldc * r;
         int kz = k > g.nz / 2 ? k - g.nz : k;
         int kap = (int)rint(sqrt((double)(kx * kx + ky * ky + kz * kz)));
         double wgt = (kx > 0 && 2 * kx < g.nx) ? 2.0 : 1.0;
-        cplx<T> a = A[t];
+        cplx<T> a = A[at];
         double e = wgt * ((double)a.x * (double)a.x + (double)a.y * (double)a.y) / 2 * invN2;
         if (kap < nbins) atomicAdd(&bins[kap], e);
     }
