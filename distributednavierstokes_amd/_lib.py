@@ -20,6 +20,7 @@ LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
 FIELD = dict(ux=0, uy=1, uz=2, p=3, qx=4, qy=5, qz=6, delta=7)
 
 _dp = ct.POINTER(ct.c_double)
+_fp = ct.POINTER(ct.c_float)
 
 
 class FVParams(ct.Structure):
@@ -83,6 +84,21 @@ _SIGS = {
     "dns_spec_dissipation": (ct.c_int, [ct.c_void_p, _dp]),
     "dns_spec_maxdiv": (ct.c_int, [ct.c_void_p, _dp]),
     "dns_spec_sync": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_products": (ct.c_int, [ct.c_void_p]),
+    "dns_spec_get_products": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_sfs_sample": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_int32]),
+    "dns_sfs_get": (ct.c_int, [ct.c_void_p, _dp, _dp]),
+    "dns_sfs_physical": (ct.c_int, [ct.c_void_p, _dp, _dp]),
+    "dns_cnn_create": (ct.c_int, [ct.c_int32, ct.c_int32, ct.c_uint64, ct.POINTER(ct.c_void_p)]),
+    "dns_cnn_destroy": (ct.c_int, [ct.c_void_p]),
+    "dns_cnn_num_params": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64)]),
+    "dns_cnn_set_params": (ct.c_int, [ct.c_void_p, _fp]),
+    "dns_cnn_get_params": (ct.c_int, [ct.c_void_p, _fp]),
+    "dns_cnn_get_grads": (ct.c_int, [ct.c_void_p, _fp]),
+    "dns_cnn_forward": (ct.c_int, [ct.c_void_p, _fp, _fp]),
+    "dns_cnn_loss_grad": (ct.c_int, [ct.c_void_p, _fp, _fp, _dp]),
+    "dns_cnn_adam_step": (ct.c_int, [ct.c_void_p, ct.c_float, ct.c_float, ct.c_float, ct.c_float]),
+    "dns_cnn_sync": (ct.c_int, [ct.c_void_p]),
     "dns_spec_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "dns_spec_timing_read": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.POINTER(ct.c_int64)]),
 }
@@ -120,3 +136,8 @@ def check(rc):
 
 def dptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a):
+    assert a.dtype == __import__("numpy").float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)

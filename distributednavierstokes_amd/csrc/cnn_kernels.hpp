@@ -1,0 +1,323 @@
+// BF16 MFMA kernels of the sub-filter-stress CNN closure (gfx950): 3×3×3
+// periodic convolutions as implicit GEMMs, erf-GELU, MSE loss and Adam.
+//
+// Activations are channels-last BF16 arrays act[v][c], v = x + M(y + M(z + M b)),
+// with the channel count padded to a multiple of 8.  Weights are BF16
+// W[o][n][k] (o = 0..26 the offset (dx, dy, dz) = (o%3−1, (o/3)%3−1, o/9−1),
+// n = output channel padded to a multiple of 32, k = input channel, contiguous)
+// for the forward pass, and the transposed copy WT[o][k][n] for the data
+// gradient.  FP32 master weights and Adam moments live beside them.
+//
+// MFMA: v_mfma_f32_32x32x8_bf16 (wave64): C[32×32] += A[32×8]·B[8×32]; lane l
+// supplies A[l%32][4(l/32)..4(l/32)+3] and B[4(l/32)..+3][l%32]; it receives
+// C[8(r/4) + 4(l/32) + r%4][l%32] in accumulator r = 0..15.
+#pragma once
+
+#include "common.hpp"
+
+namespace dns {
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ inline unsigned short f2bf(float f)
+{
+    // round to nearest even
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16) | ((u & 0xffff) ? 0x40 : 0);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+__device__ inline float gelu_f(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ inline float gelu_d(float x)
+{
+    const float phi = 0.39894228040143268f * expf(-0.5f * x * x);
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * phi;
+}
+
+struct ConvGeom {
+    int M;     // grid points per direction
+    int B;     // batch
+    long V;    // B M³ voxels
+};
+
+// Neighbour voxel of v at offset o (sign = +1: v + d, −1: v − d), periodic.
+__device__ inline long conv_nb(long v, int o, int sign, const ConvGeom &g)
+{
+    const int M = g.M;
+    int x = (int)(v % M);
+    long r = v / M;
+    int y = (int)(r % M);
+    r /= M;
+    int z = (int)(r % M);
+    long b = r / M;
+    const int dx = (o % 3 - 1) * sign, dy = ((o / 3) % 3 - 1) * sign, dz = (o / 9 - 1) * sign;
+    x += dx;
+    y += dy;
+    z += dz;
+    x = x < 0 ? x + M : (x >= M ? x - M : x);
+    y = y < 0 ? y + M : (y >= M ? y - M : y);
+    z = z < 0 ? z + M : (z >= M ? z - M : z);
+    return x + (long)M * (y + (long)M * (z + (long)M * b));
+}
+
+// out[v][n] = Σ_o Σ_k W[o][n][k] · in[v + sign·d_o][k]   (+ epilogue)
+//   EPI 0: forward hidden layer: z = acc + bias → zbuf (BF16), a = gelu(z) → out (BF16)
+//   EPI 1: forward output layer: y = acc + bias → yout (FP32, nout channels)
+//   EPI 2: data gradient: d = acc · gelu'(zprev[v][n]) → out (BF16)
+// Block: 4 waves; tile (32·WM voxels) × (32·WN channels), WM·WN = 4.
+// KIN = channels of `in` (multiple of 8), NP = padded output channels
+// (multiple of 32) = row count of W, NOUT = channels stored.
+struct ConvArgs {
+    const unsigned short *in;
+    const unsigned short *W;
+    const float *bias;
+    const unsigned short *zprev;
+    unsigned short *out;
+    unsigned short *zbuf;
+    float *yout;
+    int kin, np, nout, ldo; // ldo: channel stride of out/zbuf/yout/zprev
+    int sign;
+};
+
+template <int WM, int WN, int EPI>
+__global__ void __launch_bounds__(256) k_conv(ConvArgs a, ConvGeom g)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const long v0 = (long)blockIdx.x * (32 * WM) + 32 * wm;
+    const int n0 = blockIdx.y * (32 * WN) + 32 * wn;
+    const int row = lane & 31, kh = 4 * (lane >> 5);
+    const long v = v0 + row;
+    const bool vok = v < g.V;
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    if (n0 < a.np) {
+        for (int o = 0; o < 27; ++o) {
+            const long nb = vok ? conv_nb(v, o, a.sign, g) : 0;
+            const unsigned short *ap = a.in + nb * a.kin + kh;
+            const unsigned short *bp = a.W + ((size_t)o * a.np + n0 + row) * a.kin + kh;
+            for (int k0 = 0; k0 < a.kin; k0 += 8) {
+                bf16x4 av, bv;
+                if (vok) {
+                    const uint2 raw = *reinterpret_cast<const uint2 *>(ap + k0);
+                    av = __builtin_bit_cast(bf16x4, raw);
+                } else {
+                    av = bf16x4{0, 0, 0, 0};
+                }
+                const uint2 rawb = *reinterpret_cast<const uint2 *>(bp + k0);
+                bv = __builtin_bit_cast(bf16x4, rawb);
+                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: accumulator r holds row 8(r/4) + 4(lane/32) + r%4, column lane%32
+    const int col = n0 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int rr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long vv = v0 + rr;
+        if (vv >= g.V || col >= a.nout) continue;
+        const float c = acc[r];
+        if (EPI == 0) {
+            const float z = c + a.bias[col];
+            a.zbuf[vv * a.ldo + col] = f2bf(z);
+            a.out[vv * a.ldo + col] = f2bf(gelu_f(z));
+        } else if (EPI == 1) {
+            a.yout[vv * a.ldo + col] = c + a.bias[col];
+        } else {
+            const float zp = bf2f(a.zprev[vv * a.ldo + col]);
+            a.out[vv * a.ldo + col] = f2bf(c * gelu_d(zp));
+        }
+    }
+}
+
+// Weight gradient of one offset per block-row: for o = blockIdx.y,
+// dW[o][n][k] += Σ_{v in chunk} dout[v][n] · in[v + d_o][k], tiles of 64
+// voxels staged (transposed) through LDS.  Each block covers a range of
+// voxels (blockIdx.x) and the full (NP × KP) output (NP, KP <= 64, multiples
+// of 32 after padding), writing FP32 partials part[blockIdx.x][o][n][k].
+struct WGradArgs {
+    const unsigned short *dout; // [V][ldd]
+    const unsigned short *in;   // [V][kin]
+    float *part;
+    int ldd, nout, kin, np, kp; // np, kp: padded extents of the partial tile
+    long chunk;                 // voxels per block
+};
+
+__global__ void __launch_bounds__(256) k_conv_wgrad(WGradArgs a, ConvGeom g)
+{
+    __shared__ unsigned short sd[64][64 + 4]; // [channel n][voxel]
+    __shared__ unsigned short si[64][64 + 4]; // [channel k][voxel]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wn = wave & 1, wk = wave >> 1; // 2 × 2 waves of 32 × 32
+    const int o = blockIdx.y;
+    const long vbeg = (long)blockIdx.x * a.chunk;
+    const long vend = min(vbeg + a.chunk, g.V);
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const int row = lane & 31, kh = 4 * (lane >> 5);
+    for (long vt = vbeg; vt < vend; vt += 64) {
+        // stage dout[vt..vt+63][0..np) and in[nb(v)][0..kp) transposed
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int vl = e >> 6, c = e & 63;
+            const long v = vt + vl;
+            unsigned short dval = 0, ival = 0;
+            if (v < vend) {
+                if (c < a.nout) dval = a.dout[v * a.ldd + c];
+                if (c < a.kin) ival = a.in[conv_nb(v, o, 1, g) * a.kin + c];
+            }
+            sd[c][vl] = dval;
+            si[c][vl] = ival;
+        }
+        __syncthreads();
+        if (32 * wn < a.np && 32 * wk < a.kp) {
+#pragma unroll
+            for (int k0 = 0; k0 < 64; k0 += 8) {
+                bf16x4 av, bv;
+                const unsigned short *pa = &sd[32 * wn + row][k0 + kh];
+                const unsigned short *pb = &si[32 * wk + row][k0 + kh];
+                av = bf16x4{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
+                bv = bf16x4{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
+                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (32 * wn < a.np && 32 * wk < a.kp) {
+        const int col = 32 * wk + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int n = 32 * wn + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            a.part[(((size_t)blockIdx.x * 27 + o) * a.np + n) * a.kp + col] = acc[r];
+        }
+    }
+}
+
+// Sums the partials over the voxel splits into the FP32 gradient
+// grad[o][n][k] (n < nout, k < kin) of the unpadded layout.
+__global__ void k_wgrad_reduce(const float *part, int nsplit, int np, int kp, int nout, int kin, float *grad)
+{
+    const int total = 27 * nout * kin;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int k = t % kin, n = (t / kin) % nout, o = t / (kin * nout);
+        double s = 0;
+        for (int q = 0; q < nsplit; ++q) s += part[(((size_t)q * 27 + o) * np + n) * kp + k];
+        grad[t] = (float)s;
+    }
+}
+
+// Bias gradient: per-channel sum of dout over all voxels (per-block partials).
+__global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *dout, int ldd, int nout, long V, float *part)
+{
+    __shared__ float s[256];
+    const int c = blockIdx.y;
+    float acc = 0;
+    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
+        acc += bf2f(dout[v * ldd + c]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)c * gridDim.x + blockIdx.x] = s[0];
+}
+
+__global__ void k_bias_reduce(const float *part, int nblocks, int nout, float *grad)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nout) return;
+    double s = 0;
+    for (int q = 0; q < nblocks; ++q) s += part[(size_t)c * nblocks + q];
+    grad[c] = (float)s;
+}
+
+// MSE loss over `nout` channels: dy = 2(y − t)/(V nout) (BF16, padded to
+// ldd channels with zeros), per-block partial sums of (y − t)².
+__global__ void __launch_bounds__(256) k_mse(const float *y, const float *t, int nout, long V, unsigned short *dy,
+                                             int ldd, double *part)
+{
+    __shared__ double s[256];
+    const double scale = 2.0 / ((double)V * nout);
+    double acc = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < V * ldd; e += (long)gridDim.x * blockDim.x) {
+        const long v = e / ldd;
+        const int c = (int)(e - v * ldd);
+        float d = 0.f;
+        if (c < nout) {
+            const float r = y[v * nout + c] - t[v * nout + c];
+            acc += (double)r * r;
+            d = (float)(scale * r);
+        }
+        dy[e] = f2bf(d);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Adam on FP32 masters; refreshes the BF16 forward (W[o][n][k], n padded to
+// np) and transposed (WT[o][k][n], k padded to kpt) copies.  bias tail of
+// each layer is stored after its weights in the flat parameter vector.
+struct AdamArgs {
+    float *w, *m, *v;
+    const float *g;
+    long count;
+    float lr, b1, b2, eps, c1, c2; // c1 = 1/(1−b1^t), c2 = 1/(1−b2^t)
+};
+
+__global__ void __launch_bounds__(256) k_adam(AdamArgs a)
+{
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < a.count; t += (long)gridDim.x * blockDim.x) {
+        const float g = a.g[t];
+        const float m = a.b1 * a.m[t] + (1.f - a.b1) * g;
+        const float v = a.b2 * a.v[t] + (1.f - a.b2) * g * g;
+        a.m[t] = m;
+        a.v[t] = v;
+        const float mh = m * a.c1, vh = v * a.c2;
+        a.w[t] -= a.lr * mh / (sqrtf(vh) + a.eps);
+    }
+}
+
+// Writes the BF16 copies of one layer's weights from its FP32 master w[o][n][k]
+// (n < nout, k < kin): W[o][n][k] with n padded to np and k to kinp, and
+// WT[o][k][n] with k padded to kpt (rows) and n to noutp (contiguous).
+__global__ void k_pack_weights(const float *w, int nout, int kin, unsigned short *W, int np, int kinp,
+                               unsigned short *WT, int kpt, int noutp)
+{
+    const int total = 27 * np * kinp;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int k = t % kinp, n = (t / kinp) % np, o = t / (kinp * np);
+        const float val = (n < nout && k < kin) ? w[(o * nout + n) * kin + k] : 0.f;
+        W[t] = f2bf(val);
+    }
+    if (!WT) return;
+    const int totalT = 27 * kpt * noutp;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < totalT; t += gridDim.x * blockDim.x) {
+        const int n = t % noutp, k = (t / noutp) % kpt, o = t / (noutp * kpt);
+        const float val = (n < nout && k < kin) ? w[(o * nout + n) * kin + k] : 0.f;
+        WT[t] = f2bf(val);
+    }
+}
+
+// FP32 planar (V, cin) → BF16 channels-last padded to ldc channels, with a
+// scale; and the inverse for FP32 outputs.
+__global__ void k_to_bf16(const float *x, int cin, float scale, long V, unsigned short *out, int ldc)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < V * ldc; e += (long)gridDim.x * blockDim.x) {
+        const long v = e / ldc;
+        const int c = (int)(e - v * ldc);
+        out[e] = c < cin ? f2bf(scale * x[v * cin + c]) : (unsigned short)0;
+    }
+}
+
+}  // namespace dns

@@ -3,6 +3,7 @@
 // the C ABI declared in include/dns_rocm.h.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -84,6 +85,7 @@ template <typename T> struct FVImpl : FVBase {
     int N = 0;
     int nb_pgrad = 0;
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
+    bool tiled_momentum = false; // DNS_MOMENTUM_TILED=1 selects the LDS-tiled z-marching stencil kernel
 
     ~FVImpl() override
     {
@@ -111,6 +113,10 @@ template <typename T> struct FVImpl : FVBase {
         }
         g.wall = P->ywall ? 1 : 0;
         imexy = P->ydiffusion ? 1 : 0;
+        {
+            const char *e = getenv("DNS_MOMENTUM_TILED");
+            tiled_momentum = e && e[0] == '1';
+        }
         visc = P->visc;
         for (int c = 0; c < 3; ++c) f[c] = P->bodyforce[c];
         N = nx * ny * nz;
@@ -414,9 +420,23 @@ template <typename T> struct FVImpl : FVBase {
         a.rdx2 = (T)(1.0 / (dx * dx));
         a.rdz = (T)(1.0 / dz);
         a.rdz2 = (T)(1.0 / (dz * dz));
-        dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
         timer.begin(FAM_MOM);
         const bool rq = Ac != 0.0;
+        if (tiled_momentum) {
+            const int zc = std::min(g.nz, 32);
+            dim3 tgrd(ceil_div(g.nx, 64), ceil_div(g.ny, 8), ceil_div(g.nz, zc));
+            if (write_v) {
+                if (rq) hipLaunchKernelGGL((k_momentum_tiled<T, true, true>), tgrd, dim3(256), 0, st, a, g, zc);
+                else hipLaunchKernelGGL((k_momentum_tiled<T, false, true>), tgrd, dim3(256), 0, st, a, g, zc);
+            } else {
+                if (rq) hipLaunchKernelGGL((k_momentum_tiled<T, true, false>), tgrd, dim3(256), 0, st, a, g, zc);
+                else hipLaunchKernelGGL((k_momentum_tiled<T, false, false>), tgrd, dim3(256), 0, st, a, g, zc);
+            }
+            timer.end(FAM_MOM);
+            DNS_HIP_CHECK(hipGetLastError());
+            return DNS_OK;
+        }
+        dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
         if (write_v) {
             if (rq) hipLaunchKernelGGL((k_momentum<T, true, true>), grd, blk, 0, st, a, g);
             else hipLaunchKernelGGL((k_momentum<T, false, true>), grd, blk, 0, st, a, g);

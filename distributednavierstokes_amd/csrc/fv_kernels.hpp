@@ -99,40 +99,17 @@ template <typename T> struct MomentumArgs {
     T rdx, rdx2, rdz, rdz2;
 };
 
+// The three momentum components at one cell from its 27 stencil values
+// (uXabc: component X at offsets a, b, c in x, y, z; p = +1, m = -1).
 template <typename T, bool READ_Q, bool WRITE_V>
-__global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
+__device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, int c, int j, int jj, T ux000,
+                                     T uxp00, T uxm00, T ux0p0, T ux0m0, T ux00p, T ux00m, T uxmp0, T uxm0p, T uy000,
+                                     T uyp00, T uym00, T uy0p0, T uy0m0, T uy00p, T uy00m, T uypm0, T uy0mp, T uz000,
+                                     T uzp00, T uzm00, T uz0p0, T uz0m0, T uz00p, T uz00m, T uzp0m, T uz0pm)
 {
-    int bx, by, bz;
-    xcd_block3(bx, by, bz);
-    const int i = bx * 64 + threadIdx.x;
-    const int j = by * 4 + threadIdx.y;
-    const int k = bz;
-    if (i >= g.nx || j >= g.ny) return;
-    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
-    const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
-    const int jj = j + 1;
-    const T m1 = T(-1);
-    const T *ux = a.ux, *uy = a.uy, *uz = a.uz;
-
-    const T ux000 = ux[IX(i, j, k, g)], uxp00 = ux[IX(ip, j, k, g)], uxm00 = ux[IX(im, j, k, g)];
-    const T ux0p0 = ldc(ux, i, jj + 1, k, g, m1), ux0m0 = ldc(ux, i, jj - 1, k, g, m1);
-    const T ux00p = ux[IX(i, j, kp, g)], ux00m = ux[IX(i, j, km, g)];
-    const T uxmp0 = ldc(ux, im, jj + 1, k, g, m1), uxm0p = ux[IX(im, j, kp, g)];
-
-    const T uy000 = ldf(uy, i, jj, k, g), uyp00 = ldf(uy, ip, jj, k, g), uym00 = ldf(uy, im, jj, k, g);
-    const T uy0p0 = ldf(uy, i, jj + 1, k, g), uy0m0 = ldf(uy, i, jj - 1, k, g);
-    const T uy00p = ldf(uy, i, jj, kp, g), uy00m = ldf(uy, i, jj, km, g);
-    const T uypm0 = ldf(uy, ip, jj - 1, k, g), uy0mp = ldf(uy, i, jj - 1, kp, g);
-
-    const T uz000 = uz[IX(i, j, k, g)], uzp00 = uz[IX(ip, j, k, g)], uzm00 = uz[IX(im, j, k, g)];
-    const T uz0p0 = ldc(uz, i, jj + 1, k, g, m1), uz0m0 = ldc(uz, i, jj - 1, k, g, m1);
-    const T uz00p = uz[IX(i, j, kp, g)], uz00m = uz[IX(i, j, km, g)];
-    const T uzp0m = uz[IX(ip, j, km, g)], uz0pm = ldc(uz, i, jj + 1, km, g, m1);
-
     const T Dyj = a.dy[jj], Dyp = a.dy[jj + 1];
     const T rDyj = a.rdy[jj], rDyp = a.rdy[jj + 1], rDyuj = a.rdyu[jj], rDyum = a.rdyu[jj - 1];
     const T h = T(0.5);
-    const int c = IX(i, j, k, g);
 
     // x momentum at the right x-face of the cell
     {
@@ -188,6 +165,107 @@ __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
         if (READ_Q) q += a.A * a.qz[c];
         a.qz[c] = q;
         if (WRITE_V) a.vz[c] = uz000 + a.B * q;
+    }
+}
+
+template <typename T, bool READ_Q, bool WRITE_V>
+__global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
+{
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = bx * 64 + threadIdx.x;
+    const int j = by * 4 + threadIdx.y;
+    const int k = bz;
+    if (i >= g.nx || j >= g.ny) return;
+    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
+    const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
+    const int jj = j + 1;
+    const T m1 = T(-1);
+    const T *ux = a.ux, *uy = a.uy, *uz = a.uz;
+
+    const T ux000 = ux[IX(i, j, k, g)], uxp00 = ux[IX(ip, j, k, g)], uxm00 = ux[IX(im, j, k, g)];
+    const T ux0p0 = ldc(ux, i, jj + 1, k, g, m1), ux0m0 = ldc(ux, i, jj - 1, k, g, m1);
+    const T ux00p = ux[IX(i, j, kp, g)], ux00m = ux[IX(i, j, km, g)];
+    const T uxmp0 = ldc(ux, im, jj + 1, k, g, m1), uxm0p = ux[IX(im, j, kp, g)];
+
+    const T uy000 = ldf(uy, i, jj, k, g), uyp00 = ldf(uy, ip, jj, k, g), uym00 = ldf(uy, im, jj, k, g);
+    const T uy0p0 = ldf(uy, i, jj + 1, k, g), uy0m0 = ldf(uy, i, jj - 1, k, g);
+    const T uy00p = ldf(uy, i, jj, kp, g), uy00m = ldf(uy, i, jj, km, g);
+    const T uypm0 = ldf(uy, ip, jj - 1, k, g), uy0mp = ldf(uy, i, jj - 1, kp, g);
+
+    const T uz000 = uz[IX(i, j, k, g)], uzp00 = uz[IX(ip, j, k, g)], uzm00 = uz[IX(im, j, k, g)];
+    const T uz0p0 = ldc(uz, i, jj + 1, k, g, m1), uz0m0 = ldc(uz, i, jj - 1, k, g, m1);
+    const T uz00p = uz[IX(i, j, kp, g)], uz00m = uz[IX(i, j, km, g)];
+    const T uzp0m = uz[IX(ip, j, km, g)], uz0pm = ldc(uz, i, jj + 1, km, g, m1);
+
+    momentum_cell<T, READ_Q, WRITE_V>(a, g, IX(i, j, k, g), j, jj, ux000, uxp00, uxm00, ux0p0, ux0m0, ux00p, ux00m,
+                                      uxmp0, uxm0p, uy000, uyp00, uym00, uy0p0, uy0m0, uy00p, uy00m, uypm0, uy0mp,
+                                      uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m, uz0pm);
+}
+
+// LDS-tiled variant: a block owns a 64 × 8 (x, y) tile and marches over
+// `zc` planes in z, keeping planes k−1, k, k+1 of all three components with
+// a one-cell halo in LDS (each plane is read from HBM once per block and the
+// 26 neighbours come from LDS).  256 threads, each computes rows ty and
+// ty + 4 of the tile.  Halo values follow the same ghost rules as ldc/ldf.
+template <typename T, bool READ_Q, bool WRITE_V>
+__global__ void __launch_bounds__(256) k_momentum_tiled(MomentumArgs<T> a, FVGeom g, int zc)
+{
+    constexpr int TX = 64, TY = 8, SX = TX + 2, SY = TY + 2, PL = SX * SY;
+    __shared__ T sm[3][3][PL]; // [slot][component][ly * SX + lx]
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int x0 = bx * TX, y0 = by * TY, k0 = bz * zc;
+    const int kend = min(k0 + zc, g.nz);
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    const T m1 = T(-1);
+
+    auto load_plane = [&](int slot, int k) {
+        const int kw = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+        for (int e = tid; e < 3 * PL; e += 256) {
+            const int comp = e / PL, r = e - comp * PL;
+            const int ly = r / SX, lx = r - ly * SX;
+            int i = (x0 + lx - 1) % g.nx;
+            if (i < 0) i += g.nx;
+            const int jj = y0 + ly; // ghosted row of tile row ly − 1
+            T v;
+            if (comp == 0) v = ldc(a.ux, i, jj, kw, g, m1);
+            else if (comp == 1) v = ldf(a.uy, i, jj, kw, g);
+            else v = ldc(a.uz, i, jj, kw, g, m1);
+            sm[slot][comp][r] = v;
+        }
+    };
+
+    load_plane(0, k0 - 1);
+    load_plane(1, k0);
+    load_plane(2, k0 + 1);
+    __syncthreads();
+    int sM = 0, s0 = 1, sP = 2;
+    for (int k = k0; k < kend; ++k) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = x0 + tx, j = y0 + ty + 4 * h;
+            if (i < g.nx && j < g.ny) {
+                const int c = (ty + 4 * h + 1) * SX + tx + 1;
+                const T *X0 = sm[s0][0], *Y0 = sm[s0][1], *Z0 = sm[s0][2];
+                const T *XP = sm[sP][0], *YP = sm[sP][1], *ZP = sm[sP][2];
+                const T *XM = sm[sM][0], *YM = sm[sM][1], *ZM = sm[sM][2];
+                momentum_cell<T, READ_Q, WRITE_V>(
+                    a, g, IX(i, j, k, g), j, j + 1,
+                    X0[c], X0[c + 1], X0[c - 1], X0[c + SX], X0[c - SX], XP[c], XM[c], X0[c - 1 + SX], XP[c - 1],
+                    Y0[c], Y0[c + 1], Y0[c - 1], Y0[c + SX], Y0[c - SX], YP[c], YM[c], Y0[c + 1 - SX], YP[c - SX],
+                    Z0[c], Z0[c + 1], Z0[c - 1], Z0[c + SX], Z0[c - SX], ZP[c], ZM[c], ZM[c + 1], ZM[c + SX]);
+            }
+        }
+        __syncthreads();
+        if (k + 1 < kend) {
+            load_plane(sM, k + 2);
+            const int t = sM;
+            sM = s0;
+            s0 = sP;
+            sP = t;
+        }
+        __syncthreads();
     }
 }
 

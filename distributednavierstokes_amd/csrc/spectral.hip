@@ -38,6 +38,8 @@ struct SpecBase {
     virtual int energy(double *e, double *diss) = 0;
     virtual int maxdiv(double *out) = 0;
     virtual int sync() = 0;
+    virtual int products() = 0;
+    virtual int get_products(double *h) = 0;
     LaunchTimer timer;
 };
 
@@ -59,7 +61,7 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         if (st) hipStreamSynchronize(st);
         timer.flush();
-        void *ptrs[] = {uh, ustart, du, P, Q, phys, twN, twx, twn, red, redout, factors};
+        void *ptrs[] = {uh, ustart, du, P, Q, phys, twN, twx, twn, red, redout, factors, aux};
         for (void *p_ : ptrs)
             if (p_) hipFree(p_);
         if (pinned) hipHostFree(pinned);
@@ -186,11 +188,11 @@ template <typename T> struct SpecImpl : SpecBase {
     }
 
     // ----------------------------------------------------------- passes
-    int backward_yz()
+    int backward_yz(const cplx<T> *src = nullptr)
     {
         timer.begin(SFAM_Y);
-        hipLaunchKernelGGL((k_sp_yback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, uh, Q,
-                           twn, tb, g, fdn);
+        hipLaunchKernelGGL((k_sp_yback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st,
+                           src ? src : uh, Q, twn, tb, g, fdn);
         timer.end(SFAM_Y);
         timer.begin(SFAM_Z);
         hipLaunchKernelGGL((k_sp_zback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, 3), dim3(SP_NTHR), lds, st, Q, P,
@@ -256,6 +258,56 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(ensure_phys());
         DNS_TRY(backward_yz());
         return xpass<XMID_C2R>(red);
+    }
+
+    // physical field of three spectral components at src into phys
+    int spec_to_phys_from(const cplx<T> *src)
+    {
+        DNS_TRY(ensure_phys());
+        DNS_TRY(backward_yz(src));
+        return xpass<XMID_C2R>(red);
+    }
+
+    // σ̂_p = F[v_i v_j]/n³ (xx, yy, zz, xy, yz, zx) of the state's velocity,
+    // written compactly as (K, m, m, 6) into P (free after the z pass).
+    int products() override
+    {
+        DNS_TRY(backward_yz());
+        DNS_TRY(xpass<XMID_RHS>(red + redcap / 2));
+        DNS_TRY(zfwd(6));
+        StageArgs<T> sa;
+        sa.Q = Q;
+        sa.uh = uh;
+        sa.ustart = ustart;
+        sa.du = P;
+        sa.tw = twn;
+        sa.tb = tbs;
+        sa.first = 1;
+        sa.a = sa.b = sa.nudcdt = sa.dt = T(0);
+        hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, 2>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR), ldss,
+                           st, sa, g, fdn);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    int get_products(double *h) override
+    {
+        const size_t n6 = 6 * ns;
+        std::vector<cplx<T>> b(n6);
+        DNS_HIP_CHECK(hipMemcpyAsync(b.data(), P, sizeof(cplx<T>) * n6, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        for (size_t t = 0; t < n6; ++t) {
+            h[2 * t] = (double)b[t].x;
+            h[2 * t + 1] = (double)b[t].y;
+        }
+        return DNS_OK;
+    }
+
+    cplx<T> *aux = nullptr; // τ of the SFS recipe (K, m, m, 6), LES handles only
+    int ensure_aux()
+    {
+        if (!aux) DNS_TRY(dalloc(&aux, 6 * ns));
+        return DNS_OK;
     }
 
     int phys_to_spec() override
@@ -406,6 +458,77 @@ template <typename T> struct SpecImpl : SpecBase {
         return DNS_OK;
     }
 };
+
+// sfs_sample! for one Gaussian bank cell (src/spectral/les.jl:313-382):
+// LES state ← ū, LES aux ← τ̂ (trace-free), both on the LES handle's compact
+// grid (n = M, kcut = M ÷ 3).  Clobbers the DNS pipeline buffers.
+template <typename T>
+int sfs_sample(SpecImpl<T> *D, SpecImpl<T> *L, double delta_fac, int protectshells)
+{
+    DNS_TRY(D->products());
+    DNS_HIP_CHECK(hipStreamSynchronize(D->st));
+    DNS_TRY(L->ensure_aux());
+    const double M = L->g.n;
+    const double delta = delta_fac * L->l / M;
+    const double kb = L->g.kfac * (protectshells + 1);
+    const int nb = (int)std::min<size_t>(SP_REDB, (L->ns + SP_NTHR - 1) / SP_NTHR);
+    hipLaunchKernelGGL((k_les_gather<T>), dim3(nb), dim3(SP_NTHR), 0, L->st, D->uh, D->P, D->g, L->uh, L->aux, L->g,
+                       kb * kb, delta);
+    DNS_HIP_CHECK(hipGetLastError());
+    DNS_TRY(L->products());
+    hipLaunchKernelGGL((k_les_tau<T>), dim3(nb), dim3(SP_NTHR), 0, L->st, L->aux, L->P, L->g);
+    DNS_HIP_CHECK(hipGetLastError());
+    DNS_HIP_CHECK(hipStreamSynchronize(L->st));
+    return DNS_OK;
+}
+
+template <typename T> int sfs_get(SpecImpl<T> *L, double *ubar, double *tau)
+{
+    if (ubar) DNS_TRY(L->get_state(ubar, 0));
+    if (tau) {
+        if (!L->aux) {
+            set_error("no SFS sample on this handle");
+            return DNS_EINVAL;
+        }
+        const size_t n6 = 6 * L->ns;
+        std::vector<cplx<T>> b(n6);
+        DNS_HIP_CHECK(hipMemcpyAsync(b.data(), L->aux, sizeof(cplx<T>) * n6, hipMemcpyDeviceToHost, L->st));
+        DNS_HIP_CHECK(hipStreamSynchronize(L->st));
+        for (size_t t = 0; t < n6; ++t) {
+            tau[2 * t] = (double)b[t].x;
+            tau[2 * t + 1] = (double)b[t].y;
+        }
+    }
+    return DNS_OK;
+}
+
+// Physical ū (M³×3) and τ (M³×6) of the last sample, host copies.
+template <typename T> int sfs_physical(SpecImpl<T> *L, double *ubar, double *tau)
+{
+    if (!L->aux) {
+        set_error("no SFS sample on this handle");
+        return DNS_EINVAL;
+    }
+    const size_t n3 = (size_t)L->g.n * L->g.n * L->g.n;
+    std::vector<T> b(3 * n3);
+    auto fetch = [&](double *dst) -> int {
+        DNS_HIP_CHECK(hipMemcpyAsync(b.data(), L->phys, sizeof(T) * 3 * n3, hipMemcpyDeviceToHost, L->st));
+        DNS_HIP_CHECK(hipStreamSynchronize(L->st));
+        for (size_t t = 0; t < 3 * n3; ++t) dst[t] = (double)b[t];
+        return DNS_OK;
+    };
+    if (ubar) {
+        DNS_TRY(L->spec_to_phys_from(L->uh));
+        DNS_TRY(fetch(ubar));
+    }
+    if (tau) {
+        DNS_TRY(L->spec_to_phys_from(L->aux));
+        DNS_TRY(fetch(tau));
+        DNS_TRY(L->spec_to_phys_from(L->aux + 3 * L->ns));
+        DNS_TRY(fetch(tau + 3 * n3));
+    }
+    return DNS_OK;
+}
 
 }  // namespace dns
 
@@ -563,6 +686,75 @@ int dns_spec_maxdiv(dns_spec *h, double *out)
     if (!out) return DNS_EINVAL;
     return h->impl->maxdiv(out);
 }
+int dns_spec_products(dns_spec *h)
+{
+    SP_CHECK(h);
+    return h->impl->products();
+}
+int dns_spec_get_products(dns_spec *h, double *host)
+{
+    SP_CHECK(h);
+    if (!host) return DNS_EINVAL;
+    return h->impl->get_products(host);
+}
+
+#define SFS_DISPATCH(call64, call32)                                                  \
+    do {                                                                              \
+        if (auto *d64 = dynamic_cast<SpecImpl<double> *>(dns->impl.get())) {          \
+            auto *l64 = dynamic_cast<SpecImpl<double> *>(les->impl.get());            \
+            if (!l64) break;                                                          \
+            return call64;                                                            \
+        }                                                                             \
+        if (auto *d32 = dynamic_cast<SpecImpl<float> *>(dns->impl.get())) {           \
+            auto *l32 = dynamic_cast<SpecImpl<float> *>(les->impl.get());             \
+            if (!l32) break;                                                          \
+            return call32;                                                            \
+        }                                                                             \
+    } while (0)
+
+int dns_sfs_sample(dns_spec *dns, dns_spec *les, double delta_fac, int32_t protectshells)
+{
+    SP_CHECK(dns);
+    SP_CHECK(les);
+    SpecBase *D = dns->impl.get(), *L = les->impl.get();
+    (void)D;
+    (void)L;
+    SFS_DISPATCH(([&]() -> int {
+                     if (l64->g.n % 2 || l64->g.kcut != l64->g.n / 3 || l64->g.n / 2 > d64->g.kcut ||
+                         l64->l != d64->l) {
+                         set_error("LES handle must have n = M (even), kcut = M ÷ 3, M/2 <= DNS kcut, same l");
+                         return DNS_EINVAL;
+                     }
+                     return sfs_sample(d64, l64, delta_fac, protectshells);
+                 }()),
+                 ([&]() -> int {
+                     if (l32->g.n % 2 || l32->g.kcut != l32->g.n / 3 || l32->g.n / 2 > d32->g.kcut ||
+                         l32->l != d32->l) {
+                         set_error("LES handle must have n = M (even), kcut = M ÷ 3, M/2 <= DNS kcut, same l");
+                         return DNS_EINVAL;
+                     }
+                     return sfs_sample(d32, l32, delta_fac, protectshells);
+                 }()));
+    set_error("DNS and LES handles must have the same precision");
+    return DNS_EINVAL;
+}
+
+int dns_sfs_get(dns_spec *les, double *ubar_hat, double *tau_hat)
+{
+    SP_CHECK(les);
+    if (auto *l64 = dynamic_cast<SpecImpl<double> *>(les->impl.get())) return sfs_get(l64, ubar_hat, tau_hat);
+    if (auto *l32 = dynamic_cast<SpecImpl<float> *>(les->impl.get())) return sfs_get(l32, ubar_hat, tau_hat);
+    return DNS_EINVAL;
+}
+
+int dns_sfs_physical(dns_spec *les, double *ubar, double *tau)
+{
+    SP_CHECK(les);
+    if (auto *l64 = dynamic_cast<SpecImpl<double> *>(les->impl.get())) return sfs_physical(l64, ubar, tau);
+    if (auto *l32 = dynamic_cast<SpecImpl<float> *>(les->impl.get())) return sfs_physical(l32, ubar, tau);
+    return DNS_EINVAL;
+}
+
 int dns_spec_sync(dns_spec *h)
 {
     SP_CHECK(h);

@@ -263,7 +263,8 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
 // integrating-factor stage (src/spectral/step.jl:22-45) and the projection
 // (src/spectral/operators.jl:10-22).  Each thread owns up to MAXO outputs
 // (b, jc) of the tile; their du accumulate in registers across the six
-// product passes.  MODE 0: stage update of uh/ustart; MODE 1: write du.
+// product passes.  MODE 0: stage update of uh/ustart; MODE 1: write du;
+// MODE 2: write the six product spectra σ̂_p (K, m, m, 6) to s.du.
 // grid (ceil(K/tb), m).
 template <typename T> struct StageArgs {
     const cplx<T> *Q;
@@ -300,7 +301,10 @@ __global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, F
             int oi = threadIdx.x + o * blockDim.x;
             if (oi < tb * m) {
                 int jc = oi / tb, b = oi - jc * tb, i = i0 + b;
-                if (i < K) {
+                if (i < K && MODE == 2) {
+                    s.du[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))] =
+                        cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
+                } else if (i < K) {
                     cplx<T> sg = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
                     T kx = T(g.kfac * i), ky = T(g.kfac * sp_freq(jc, g));
                     // −i k σ  =  (k σ.y, −k σ.x)
@@ -327,6 +331,7 @@ __global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, F
         }
         __syncthreads();
     }
+    if (MODE == 2) return;
 #pragma unroll
     for (int o = 0; o < MAXO; ++o) {
         int oi = threadIdx.x + o * blockDim.x;
@@ -519,6 +524,54 @@ template <typename T> __global__ void __launch_bounds__(256) k_sp_maxdiv(const c
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// ---------------------------------------------- SFS data recipe (les.jl)
+
+// Gaussian test filter with protected low shells (les_filterkernel!,
+// src/spectral/les.jl:69-86): 1 for kf²|k|² < (kf (protect+1))², else
+// exp(−Δ² k²/24).
+__device__ inline double les_gauss(double kk, double kbound2, double delta)
+{
+    return kk < kbound2 ? 1.0 : exp(-delta * delta * kk / 24.0);
+}
+
+// ū = T⅔(G(cut(û))) and G(cut(σ̂)) on the compact layout of the LES grid
+// (n = M, kcut = M ÷ 3: exactly the modes T⅔ keeps), gathered from the DNS
+// state and its product spectra (src/spectral/les.jl:316-359).  The mean is
+// zeroed as les_twothirds! does.
+template <typename T>
+__global__ void __launch_bounds__(256) k_les_gather(const cplx<T> *uhD, const cplx<T> *sigD, SpecGeom gD,
+                                                    cplx<T> *uhL, cplx<T> *tauL, SpecGeom gL, double kbound2,
+                                                    double delta)
+{
+    const size_t csL = (size_t)gL.K * gL.m * gL.m, csD = (size_t)gD.K * gD.m * gD.m;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < csL; t += (size_t)gridDim.x * blockDim.x) {
+        int i = t % gL.K, r = t / gL.K, j = r % gL.m, k = r / gL.m;
+        int fj = sp_freq(j, gL), fk = sp_freq(k, gL);
+        int jD = fj >= 0 ? fj : gD.m + fj, kD = fk >= 0 ? fk : gD.m + fk;
+        size_t d = i + (size_t)gD.K * (jD + (size_t)gD.m * kD);
+        double k1 = gL.kfac * i, k2 = gL.kfac * fj, k3 = gL.kfac * fk;
+        double w = les_gauss(k1 * k1 + k2 * k2 + k3 * k3, kbound2, delta);
+        if (i == 0 && fj == 0 && fk == 0) w = 0.0;
+        for (int c = 0; c < 3; ++c) uhL[t + c * csL] = cscale<T>(uhD[d + c * csD], T(w));
+        for (int p = 0; p < 6; ++p) tauL[t + p * csL] = cscale<T>(sigD[d + p * csD], T(w));
+    }
+}
+
+// τ = T⅔(G σ̂ − F[ū_i ū_j]/M³), trace-free (src/spectral/les.jl:360-370).
+template <typename T>
+__global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *sig2, SpecGeom gL)
+{
+    const size_t cs = (size_t)gL.K * gL.m * gL.m;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < cs; t += (size_t)gridDim.x * blockDim.x) {
+        cplx<T> v[6];
+        const bool mean = t == 0;
+        for (int p = 0; p < 6; ++p) v[p] = mean ? mk<T>(T(0), T(0)) : csub<T>(tauL[t + p * cs], sig2[t + p * cs]);
+        cplx<T> tr = mk<T>((v[0].x + v[1].x + v[2].x) / T(3), (v[0].y + v[1].y + v[2].y) / T(3));
+        for (int p = 0; p < 3; ++p) v[p] = csub<T>(v[p], tr);
+        for (int p = 0; p < 6; ++p) tauL[t + p * cs] = v[p];
+    }
 }
 
 }  // namespace dns

@@ -184,11 +184,56 @@ int dns_spec_noise(dns_spec *h, uint64_t seed);
 int dns_spec_energy(dns_spec *h, double *out);
 int dns_spec_dissipation(dns_spec *h, double *out);
 int dns_spec_maxdiv(dns_spec *h, double *out);
+/* σ̂ = F[v_i v_j]/n³ of the state's velocity at the retained modes, six
+ * products in the order xx, yy, zz, xy, yz, zx (the product batches of
+ * spectral_rhs!, src/spectral/operators.jl:66-75, and of sfs_sample!,
+ * src/spectral/les.jl:323-337), kept in the handle's product buffer until
+ * the next pipeline call; dns_spec_get_products copies (K, m, m, 6). */
+int dns_spec_products(dns_spec *h);
+int dns_spec_get_products(dns_spec *h, double *host);
 int dns_spec_sync(dns_spec *h);
+
+/* -------------------------------------------- SFS data (learned closure)
+ * sfs_sample! for one Gaussian bank cell (src/spectral/les.jl:313-382):
+ * ū = T⅔(G(cut(û))) and τ = T⅔(G(cut(σ̂)) − F[ū_i ū_j]/M³), trace-free, with
+ * G = exp(−Δ²k²/24), Δ = delta_fac·l/M, shells |k| <= protectshells left
+ * unfiltered.  `les` is a spectral handle with n = M, kcut = M ÷ 3 (the modes
+ * T⅔ keeps), the same l and precision, and M/2 <= the DNS kcut.  The results
+ * stay on `les`: its state holds ū, its SFS buffer holds τ̂ (6 components). */
+int dns_sfs_sample(dns_spec *dns, dns_spec *les, double delta_fac, int32_t protectshells);
+/* Copies ū (K, m, m, 3) and τ̂ (K, m, m, 6) of the LES handle (either may be NULL). */
+int dns_sfs_get(dns_spec *les, double *ubar_hat, double *tau_hat);
+/* Physical ū (M, M, M, 3) and τ (M, M, M, 6) on the LES grid (host). */
+int dns_sfs_physical(dns_spec *les, double *ubar, double *tau);
+
 /* Kernel timing hook, as dns_fv_timing; family: 0 y pass, 1 z pass,
  * 2 fused x pass, 3 y pass + IF stage, 4 forcing. */
 int dns_spec_timing(dns_spec *h, int32_t enable);
 int dns_spec_timing_read(dns_spec *h, int32_t family, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------ learned SFS closure CNN
+ * No counterpart in the reference (it ends at the SFS training data,
+ * src/spectral/les.jl:313-382): a 4-layer 3×3×3 periodic CNN, channels
+ * 3 → 64 → 64 → 64 → 6 with erf-GELU, on M³ grids in batches of B, BF16
+ * weights and activations with FP32 accumulation (gfx950 MFMA), Adam on FP32
+ * master weights.  Parameter vector (FP32): per layer W[o][n][k] (27·nout·kin,
+ * o ↔ offset (o%3−1, (o/3)%3−1, o/9−1) in (x, y, z)) then bias[nout].
+ * Host batches are interleaved [b][z][y][x][c] FP32. */
+typedef struct dns_cnn dns_cnn;
+int dns_cnn_create(int32_t M, int32_t batch, uint64_t seed, dns_cnn **out);
+int dns_cnn_destroy(dns_cnn *h);
+int dns_cnn_num_params(dns_cnn *h, int64_t *n);
+int dns_cnn_set_params(dns_cnn *h, const float *params);
+int dns_cnn_get_params(dns_cnn *h, float *params);
+/* Gradient of the last dns_cnn_loss_grad, same layout as the parameters. */
+int dns_cnn_get_grads(dns_cnn *h, float *grads);
+/* y (B·M³ × 6) of the batch x (B·M³ × 3). */
+int dns_cnn_forward(dns_cnn *h, const float *x, float *y);
+/* MSE loss of the network on (x, t) and its gradient (kept on the device). */
+int dns_cnn_loss_grad(dns_cnn *h, const float *x, const float *t, double *loss);
+/* One Adam update of the parameters with the current gradient. */
+int dns_cnn_adam_step(dns_cnn *h, float lr, float beta1, float beta2, float eps);
+int dns_cnn_sync(dns_cnn *h);
 
 #ifdef __cplusplus
 }

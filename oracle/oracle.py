@@ -312,3 +312,93 @@ class OracleSpectral:
             n += 1
             dts.append(dtn)
         return dict(uh=uh, t=t, n=n, dts=dts)
+
+
+def _spec_products(self, uh):
+    """σ̂ (K, m, m, 6) of the state's velocity: xx, yy, zz, xy, yz, zx."""
+    sig = np.zeros((self.K, self.m, self.m, 6), dtype=np.complex128, order="F")
+    self.L.ospec_products(self.h, _pc(uh), _pc(sig))
+    return sig
+
+
+OracleSpectral.products = _spec_products
+
+
+# ------------------------------------------------------- SFS data recipe
+# Restatement of the rank-0 recipe of sfs_sample! (src/spectral/les.jl:313-382)
+# on coarse rfft arrays (M/2+1, M, M) with wrapped negative frequencies, in
+# numpy: lesfreq (les.jl:41), les_twothirds! (46-54), les_filterkernel!
+# (69-86, Gaussian kernel), les_nonlinearity! (89-103), les_tracefree!
+# (134-143), les_spectrum (119-131), les_filter_reynolds (106-116).
+
+def lesfreq(M):
+    q = np.arange(M)
+    return np.where(q < (M + 1) // 2, q, q - M)
+
+
+def les_cube(state, kcut, M):
+    """The LES cube |k_i| <= M/2 of a compact DNS state (K, m, m, nc) as coarse
+    rfft arrays (M/2+1, M, M, nc) (les_gather! + lescoarse, les.jl:149-160)."""
+    m = 2 * kcut + 1
+    f = lesfreq(M)
+    jd = np.where(f >= 0, f, m + f)
+    return state[: M // 2 + 1][:, jd][:, :, jd]
+
+
+def les_twothirds(a, M):
+    kc = M // 3
+    k1 = np.arange(a.shape[0])[:, None, None]
+    f = lesfreq(M)
+    k2, k3 = f[None, :, None], f[None, None, :]
+    keep = (k1 <= kc) & (np.abs(k2) <= kc) & (np.abs(k3) <= kc) & ~((k1 == 0) & (k2 == 0) & (k3 == 0))
+    return a * keep[..., None] if a.ndim == 4 else a * keep
+
+
+def les_gaussian(a, M, l, delta, protectshells):
+    kf = 2 * np.pi / l
+    f = lesfreq(M)
+    k1 = kf * np.arange(a.shape[0])[:, None, None]
+    k2, k3 = kf * f[None, :, None], kf * f[None, None, :]
+    kk = k1 ** 2 + k2 ** 2 + k3 ** 2
+    w = np.where(kk < (kf * (protectshells + 1)) ** 2, 1.0, np.exp(-delta ** 2 * kk / 24))
+    return a * w[..., None] if a.ndim == 4 else a * w
+
+
+def les_nonlinearity(ub, M):
+    """σ2 = F[v_i v_j]/M³ from dealiased ub (M/2+1, M, M, 3)."""
+    vs = [np.fft.irfftn(les_twothirds(ub[..., i], M).transpose(2, 1, 0), s=(M, M, M), axes=(0, 1, 2)) * M ** 3
+          for i in range(3)]
+    out = []
+    for i, j in ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)):
+        out.append(np.fft.rfftn(vs[i] * vs[j], axes=(0, 1, 2)).transpose(2, 1, 0) / M ** 3)
+    return np.stack(out, -1)
+
+
+def sfs_sample_oracle(uh, sig, kcut, M, l, delta_fac=2.0, protectshells=2):
+    """ū and trace-free τ on the coarse rfft grid (Gaussian kernel)."""
+    delta = delta_fac * l / M
+    ub = les_cube(uh, kcut, M)
+    sb = les_cube(sig, kcut, M)
+    ubf = les_twothirds(les_gaussian(ub, M, l, delta, protectshells), M)
+    tau = les_gaussian(sb, M, l, delta, protectshells)
+    tau = les_twothirds(tau - les_nonlinearity(ubf, M), M)
+    tr = (tau[..., 0] + tau[..., 1] + tau[..., 2]) / 3
+    for c in range(3):
+        tau[..., c] = tau[..., c] - tr
+    return ubf, tau
+
+
+def les_spectrum(ub, M):
+    kc = M // 3
+    f = lesfreq(M)
+    s2 = np.arange(ub.shape[0])[:, None, None] ** 2 + f[None, :, None] ** 2 + f[None, None, :] ** 2
+    sh = np.floor(np.sqrt(s2)).astype(int)
+    # isqrt on integers
+    sh = np.array([[[int(np.sqrt(v)) if int(np.sqrt(v)) ** 2 <= v else int(np.sqrt(v)) - 1 for v in row]
+                    for row in plane] for plane in s2])
+    w = np.where((np.arange(ub.shape[0]) > 0) & (np.arange(ub.shape[0]) < ub.shape[0] - 1), 2.0, 1.0)[:, None, None]
+    u2 = (np.abs(ub) ** 2).sum(-1)
+    E = np.zeros(kc)
+    for s in range(1, kc + 1):
+        E[s - 1] = (w * u2 * (sh == s)).sum() / 2
+    return E

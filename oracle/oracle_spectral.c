@@ -386,3 +386,23 @@ double ospec_maxdiv(const ospec *h, const double complex *uh)
             }
     return mx;
 }
+
+/* σ̂ = F[v_i v_j]/n³ at the retained modes for the six products
+ * (xx, yy, zz, xy, yz, zx) of the physical velocity of uh -- the two product
+ * batches of spectral_rhs! (src/spectral/operators.jl:66-75) and of
+ * sfs_sample! (src/spectral/les.jl:323-337).  sig6: (K, m, m, 6). */
+void ospec_products(ospec *h, const double complex *uh, double complex *sig6)
+{
+    long n3 = (long)h->n * h->n * h->n;
+    long ns3 = (long)h->K * h->m * h->m * 3;
+    ospec_spec_to_phys(h, uh, h->v);
+    for (long t = 0; t < 3 * n3; ++t) h->rbuf[t] = h->v[t] * h->v[t];
+    ospec_phys_to_spec(h, h->rbuf, sig6);
+    for (long t = 0; t < n3; ++t) {
+        double a = h->v[t], b = h->v[t + n3], c = h->v[t + 2 * n3];
+        h->rbuf[t] = a * b;
+        h->rbuf[t + n3] = b * c;
+        h->rbuf[t + 2 * n3] = c * a;
+    }
+    ospec_phys_to_spec(h, h->rbuf, sig6 + ns3);
+}

@@ -1,0 +1,69 @@
+"""GPU BF16-MFMA CNN against the FP64 numpy oracle (same BF16-rounded
+parameters and inputs).  Tolerances are the BF16 bound: activations are
+rounded to 8 significant bits after every layer."""
+import numpy as np
+import pytest
+
+from distributednavierstokes_amd.cnn import SFSNet
+from oracle import oracle_cnn as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def setup_case(B=2, M=8, seed=0):
+    rng = np.random.default_rng(seed)
+    net = SFSNet(M=M, batch=B, seed=seed)
+    p = (rng.standard_normal(net.nparams) * 0.08).astype(np.float32)
+    net.set_params(p)
+    x = rng.standard_normal((B, M, M, M, 3)).astype(np.float32)
+    t = (rng.standard_normal((B, M, M, M, 6)) * 0.1).astype(np.float32)
+    return net, p, x, t
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle():
+    net, p, x, _ = setup_case()
+    y = net.forward(x)
+    yo, _, _ = oc.forward(oc.bf16_round(p), oc.bf16_round(x))
+    assert rel(y, yo) < 2e-2
+
+
+def test_gradients_match_oracle():
+    net, p, x, t = setup_case(seed=1)
+    loss = net.loss_grad(x, t)
+    g = net.grads()
+    lo, go = oc.loss_grad(oc.bf16_round(p), oc.bf16_round(x), t.astype(np.float64))
+    assert abs(loss - lo) < 2e-2 * lo
+    off = 0
+    for ci, co in oc.CH:
+        nw = 27 * co * ci
+        assert rel(g[off:off + nw], go[off:off + nw]) < 5e-2
+        assert rel(g[off + nw:off + nw + co], go[off + nw:off + nw + co]) < 5e-2
+        off += nw + co
+
+
+def test_adam_update():
+    net, p, x, t = setup_case(seed=2)
+    net.loss_grad(x, t)
+    g = net.grads().astype(np.float64)
+    net.adam_step(lr=1e-3)
+    p1 = net.params()
+    m = 0.1 * g
+    v = 0.001 * g * g
+    want = p - 1e-3 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    assert np.abs(p1 - want).max() < 1e-6
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(3)
+    net = SFSNet(M=8, batch=2, seed=3)
+    x = rng.standard_normal((2, 8, 8, 8, 3)).astype(np.float32)
+    t = (rng.standard_normal((2, 8, 8, 8, 6)) * 0.1).astype(np.float32)
+    l0 = net.loss_grad(x, t)
+    for _ in range(30):
+        net.loss_grad(x, t)
+        net.adam_step(lr=1e-3)
+    assert net.loss_grad(x, t) < 0.7 * l0
