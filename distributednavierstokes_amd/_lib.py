@@ -99,6 +99,19 @@ _SIGS = {
     "dns_cnn_loss_grad": (ct.c_int, [ct.c_void_p, _fp, _fp, _dp]),
     "dns_cnn_adam_step": (ct.c_int, [ct.c_void_p, ct.c_float, ct.c_float, ct.c_float, ct.c_float]),
     "dns_cnn_sync": (ct.c_int, [ct.c_void_p]),
+    "dns_cnn_add_sample": (ct.c_int, [ct.c_void_p, _fp, _fp]),
+    "dns_cnn_dataset_size": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32)]),
+    "dns_cnn_fit_scales": (ct.c_int, [ct.c_void_p, _dp, _dp]),
+    "dns_cnn_set_scales": (ct.c_int, [ct.c_void_p, ct.c_double, ct.c_double]),
+    "dns_cnn_get_scales": (ct.c_int, [ct.c_void_p, _dp, _dp]),
+    "dns_cnn_train": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_float, ct.c_float, ct.c_float, ct.c_float,
+                                 ct.c_uint64, ct.c_int32, _dp]),
+    "dns_cnn_eval": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32, _dp]),
+    "dns_cnn_predict": (ct.c_int, [ct.c_void_p, ct.c_int32, _fp]),
+    "dns_cnn_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "dns_cnn_timing_read": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.POINTER(ct.c_int64)]),
+    "dns_sfs_append": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "dns_les_closure_step": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_double, _dp]),
     "dns_spec_timing": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "dns_spec_timing_read": (ct.c_int, [ct.c_void_p, ct.c_int32, _dp, ct.POINTER(ct.c_int64)]),
 }

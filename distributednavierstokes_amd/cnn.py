@@ -65,3 +65,65 @@ class SFSNet:
 
     def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         check(lib().dns_cnn_adam_step(self.handle, lr, beta1, beta2, eps))
+
+    # ------------------------------------------------------------- dataset
+    def add_sample(self, ubar, tau):
+        """Adds a physical pair ū (M, M, M, 3), τ (M, M, M, 6) (x fastest)."""
+        M = self.M
+        if ubar.shape != (M, M, M, 3) or tau.shape != (M, M, M, 6):
+            raise DNSError("sample shapes must be (M, M, M, 3) and (M, M, M, 6)")
+        u = np.ascontiguousarray(np.asarray(ubar, dtype=np.float32).ravel(order="F"))
+        t = np.ascontiguousarray(np.asarray(tau, dtype=np.float32).ravel(order="F"))
+        check(lib().dns_cnn_add_sample(self.handle, fptr(u), fptr(t)))
+
+    @property
+    def nsamples(self):
+        n = ct.c_int32()
+        check(lib().dns_cnn_dataset_size(self.handle, ct.byref(n)))
+        return int(n.value)
+
+    def fit_scales(self):
+        su, st = np.zeros(1), np.zeros(1)
+        check(lib().dns_cnn_fit_scales(self.handle, dptr(su), dptr(st)))
+        return float(su[0]), float(st[0])
+
+    @property
+    def scales(self):
+        su, st = np.zeros(1), np.zeros(1)
+        check(lib().dns_cnn_get_scales(self.handle, dptr(su), dptr(st)))
+        return float(su[0]), float(st[0])
+
+    @scales.setter
+    def scales(self, v):
+        check(lib().dns_cnn_set_scales(self.handle, float(v[0]), float(v[1])))
+
+    def train(self, nsteps, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=0, ntrain=0):
+        """nsteps Adam steps on shuffled batches of the first ``ntrain``
+        dataset samples (all if 0); returns the normalised loss before each step."""
+        losses = np.zeros(max(1, int(nsteps)))
+        check(lib().dns_cnn_train(self.handle, int(nsteps), lr, beta1, beta2, eps, int(seed), int(ntrain),
+                                  dptr(losses)))
+        return losses[: int(nsteps)]
+
+    def evaluate(self, first=0, count=None):
+        """Normalised MSE over dataset samples [first, first + count); the
+        no-model closure (τ = 0) scores the mean square of the normalised τ."""
+        count = self.nsamples - first if count is None else count
+        out = np.zeros(1)
+        check(lib().dns_cnn_eval(self.handle, int(first), int(count), dptr(out)))
+        return float(out[0])
+
+    def predict(self, sample):
+        """Predicted τ (M, M, M, 6) of dataset sample ``sample`` (physical units)."""
+        M = self.M
+        out = np.zeros(6 * M ** 3, dtype=np.float32)
+        check(lib().dns_cnn_predict(self.handle, int(sample), fptr(out)))
+        return out.reshape((M, M, M, 6), order="F")
+
+    def timing(self, enable=True):
+        check(lib().dns_cnn_timing(self.handle, 1 if enable else 0))
+
+    def timing_read(self, family):
+        ms, cnt = np.zeros(1), ct.c_int64()
+        check(lib().dns_cnn_timing_read(self.handle, int(family), dptr(ms), ct.byref(cnt)))
+        return float(ms[0]), int(cnt.value)

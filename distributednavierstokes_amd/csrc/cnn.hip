@@ -9,6 +9,7 @@
 #include <random>
 #include <vector>
 
+#include "closure.hpp"
 #include "cnn_kernels.hpp"
 #include "host_util.hpp"
 
@@ -21,10 +22,78 @@ struct CNNLayer {
     unsigned short *W = nullptr, *WT = nullptr;
 };
 
+// Planar (c, Vs) → channels-last BF16 with scale (one sample into one batch slot).
+template <typename T>
+__global__ void k_planar_to_bf16(const T *x, int cin, long Vs, float scale, unsigned short *out, int ldc)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < Vs * ldc; e += (long)gridDim.x * blockDim.x) {
+        const long v = e / ldc;
+        const int c = (int)(e - v * ldc);
+        out[e] = c < cin ? f2bf(scale * (float)x[(size_t)c * Vs + v]) : (unsigned short)0;
+    }
+}
+
+// Planar (c, Vs) FP32 → interleaved (Vs, c) FP32 with scale.
+__global__ void k_planar_to_interleaved(const float *x, int c_, long Vs, float scale, float *out)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < Vs * c_; e += (long)gridDim.x * blockDim.x) {
+        const long v = e / c_;
+        const int c = (int)(e - v * c_);
+        out[e] = scale * x[(size_t)c * Vs + v];
+    }
+}
+
+// Interleaved (Vs, c) FP32 → planar (c, Vs) of T with scale.
+template <typename T> __global__ void k_interleaved_to_planar(const float *y, int c_, long Vs, double scale, T *out)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < Vs * c_; e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / Vs);
+        const long v = e - (long)c * Vs;
+        out[e] = (T)(scale * (double)y[v * c_ + c]);
+    }
+}
+
+template <typename T> __global__ void k_cast_f32(const T *x, long n, float *out)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        out[e] = (float)x[e];
+}
+
+__global__ void __launch_bounds__(256) k_sumsq(const float *x, long n, double *part)
+{
+    __shared__ double s[256];
+    double acc = 0;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        acc += (double)x[e] * x[e];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// dst = scale · Σ part[0..nparts) (one block).
+__global__ void __launch_bounds__(256) k_sum_final(const double *part, int nparts, double scale, double *dst)
+{
+    __shared__ double s[256];
+    double acc = 0;
+    for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += part[q];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dst = scale * s[0];
+}
+
 struct CNN {
     int M = 0, B = 0;
-    long V = 0;
+    long V = 0, Vs = 0; // B M³ voxels per batch, M³ per sample
     hipStream_t st = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
     CNNLayer L[4];
     size_t nparam = 0;
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr; // FP32 masters, Adam moments, gradient
@@ -33,26 +102,30 @@ struct CNN {
     unsigned short *x0 = nullptr, *z[3] = {}, *a[3] = {}, *dy = nullptr, *dz[2] = {};
     float *y = nullptr, *t = nullptr;
     float *wpart = nullptr, *bpart = nullptr;
-    double *lpart = nullptr, *pinned = nullptr;
+    double *lpart = nullptr, *lval = nullptr, *pinned = nullptr;
     int nsplit = 64, bblocks = 256;
+    static constexpr int kLossBlocks = 512;
     // dataset of filtered-DNS pairs (FP32 planar per sample: ū (3, M³), τ (6, M³))
     std::vector<float *> ds_u, ds_t;
-    double su = 1.0, st_ = 1.0; // normalisation: RMS of ū and τ over the dataset
+    double su = 1.0, stau = 1.0; // normalisation: RMS of ū and τ components
     LaunchTimer timer;
 
     ~CNN()
     {
         if (st) hipStreamSynchronize(st);
+        timer.flush();
         for (auto &l : L) {
             if (l.W) hipFree(l.W);
             if (l.WT) hipFree(l.WT);
         }
-        void *ptrs[] = {w, m, v, g, x0, z[0], z[1], z[2], a[0], a[1], a[2], dy, dz[0], dz[1], y, t, wpart, bpart, lpart};
+        void *ptrs[] = {w, m, v, g, x0, z[0], z[1], z[2], a[0], a[1], a[2], dy, dz[0], dz[1], y, t, wpart, bpart, lpart, lval};
         for (void *p : ptrs)
             if (p) hipFree(p);
         for (float *p : ds_u) hipFree(p);
         for (float *p : ds_t) hipFree(p);
         if (pinned) hipHostFree(pinned);
+        if (ev_in) hipEventDestroy(ev_in);
+        if (ev_out) hipEventDestroy(ev_out);
         if (st) hipStreamDestroy(st);
     }
 
@@ -60,8 +133,11 @@ struct CNN {
     {
         M = M_;
         B = B_;
-        V = (long)B * M * M * M;
+        Vs = (long)M * M * M;
+        V = (long)B * Vs;
         DNS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        DNS_HIP_CHECK(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+        DNS_HIP_CHECK(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
         timer.st = st;
         const int kin[4] = {3, 64, 64, 64}, nout[4] = {64, 64, 64, 6};
         size_t off = 0;
@@ -81,13 +157,13 @@ struct CNN {
             DNS_TRY(dalloc(&c.WT, (size_t)27 * c.kpt * c.noutp));
         }
         nparam = off;
-        // He-uniform initialisation from a seeded generator (host, once)
+        // uniform initialisation with variance 1/fan_in from a seeded generator (host, once)
         std::vector<float> h(nparam, 0.f);
         std::mt19937_64 gen(seed);
         for (int l = 0; l < 4; ++l) {
             const CNNLayer &c = L[l];
             const double fan_in = 27.0 * c.kin;
-            const double bound = std::sqrt(6.0 / fan_in) / std::sqrt(2.0);
+            const double bound = std::sqrt(3.0 / fan_in);
             std::uniform_real_distribution<double> U(-bound, bound);
             for (size_t t2 = 0; t2 < (size_t)27 * c.nout * c.kin; ++t2) h[c.woff + t2] = (float)U(gen);
         }
@@ -110,7 +186,8 @@ struct CNN {
         nsplit = (int)std::max<long>(1, std::min<long>(64, V / 4096));
         DNS_TRY(dalloc(&wpart, (size_t)nsplit * 27 * 64 * 64));
         DNS_TRY(dalloc(&bpart, (size_t)64 * bblocks));
-        DNS_TRY(dalloc(&lpart, 1024));
+        DNS_TRY(dalloc(&lpart, kLossBlocks));
+        DNS_TRY(dalloc(&lval, 1));
         DNS_HIP_CHECK(hipHostMalloc((void **)&pinned, sizeof(double) * 1024));
         return DNS_OK;
     }
@@ -126,18 +203,18 @@ struct CNN {
         return DNS_OK;
     }
 
-    ConvGeom geom() const { return ConvGeom{M, B, V}; }
+    ConvGeom geom(long Vn) const { return ConvGeom{M, (int)(Vn / Vs), Vn}; }
 
-    template <int WM, int WN, int EPI> void conv(const ConvArgs &ca, int np)
+    template <int WM, int WN, int EPI> void conv(const ConvArgs &ca, int np, long Vn, hipStream_t s)
     {
-        dim3 grd((unsigned)((V + 32 * WM - 1) / (32 * WM)), (unsigned)((np + 32 * WN - 1) / (32 * WN)));
-        hipLaunchKernelGGL((k_conv<WM, WN, EPI>), grd, dim3(256), 0, st, ca, geom());
+        dim3 grd((unsigned)((Vn + 32 * WM - 1) / (32 * WM)), (unsigned)((np + 32 * WN - 1) / (32 * WN)));
+        hipLaunchKernelGGL((k_conv<WM, WN, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
     }
 
-    // forward of the batch held in x0; the FP32 output lands in y (V × 6)
-    int forward()
+    // forward of the first nb batch slots of x0; the FP32 output lands in y (nb M³ × 6)
+    int forward(hipStream_t s, int nb)
     {
-        timer.begin(0);
+        const long Vn = (long)nb * Vs;
         for (int l = 0; l < 4; ++l) {
             const CNNLayer &c = L[l];
             ConvArgs ca{};
@@ -152,14 +229,13 @@ struct CNN {
                 ca.out = a[l];
                 ca.zbuf = z[l];
                 ca.ldo = 64;
-                conv<2, 2, 0>(ca, c.np);
+                conv<2, 2, 0>(ca, c.np, Vn, s);
             } else {
                 ca.yout = y;
                 ca.ldo = 6;
-                conv<4, 1, 1>(ca, c.np);
+                conv<4, 1, 1>(ca, c.np, Vn, s);
             }
         }
-        timer.end(0);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }
@@ -178,7 +254,7 @@ struct CNN {
         wa.kp = c.kpt;
         wa.chunk = (V + nsplit - 1) / nsplit;
         wa.chunk = (wa.chunk + 63) / 64 * 64;
-        hipLaunchKernelGGL(k_conv_wgrad, dim3(nsplit, 27), dim3(256), 0, st, wa, geom());
+        hipLaunchKernelGGL(k_conv_wgrad, dim3(nsplit, 27), dim3(256), 0, st, wa, geom(V));
         hipLaunchKernelGGL(k_wgrad_reduce, dim3(256), dim3(256), 0, st, wpart, nsplit, c.np, c.kpt, c.nout, c.kin,
                            g + c.woff);
         hipLaunchKernelGGL(k_bias_grad, dim3(bblocks, c.nout), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);
@@ -187,16 +263,24 @@ struct CNN {
         return DNS_OK;
     }
 
-    // loss of y against t and the full gradient into g
-    int backward(double *loss)
+    // MSE of y against t over the first nb slots into *ldst (device), and dy
+    int mse(int nb, double *ldst)
+    {
+        const long Vn = (long)nb * Vs;
+        hipLaunchKernelGGL(k_mse, dim3(kLossBlocks), dim3(256), 0, st, y, t, 6, Vn, dy, 8, lpart);
+        hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, lpart, kLossBlocks, 1.0 / ((double)Vn * 6), ldst);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    // loss of the full batch (y against t) into *ldst (device) and the gradient into g
+    int backward(double *ldst)
     {
         timer.begin(1);
-        const int nb = 512;
-        hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, st, y, t, 6, V, dy, 8, lpart);
+        DNS_TRY(mse(B, ldst));
         DNS_TRY(wgrad(3, dy, 8));
         // dz3 = (W4ᵀ * dy) ⊙ gelu'(z3), then layers 3, 2
         const unsigned short *dout = dy;
-        int ldd = 8;
         for (int l = 3; l >= 1; --l) {
             const CNNLayer &c = L[l];
             unsigned short *dst = dz[l & 1];
@@ -210,21 +294,20 @@ struct CNN {
             ca.zprev = z[l - 1];
             ca.out = dst;
             ca.sign = -1;
-            conv<2, 2, 2>(ca, c.kpt);
+            conv<2, 2, 2>(ca, c.kpt, V, st);
             DNS_TRY(wgrad(l - 1, dst, 64));
             dout = dst;
-            ldd = 64;
         }
-        (void)ldd;
         timer.end(1);
         DNS_HIP_CHECK(hipGetLastError());
-        if (loss) {
-            DNS_HIP_CHECK(hipMemcpyAsync(pinned, lpart, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
-            DNS_HIP_CHECK(hipStreamSynchronize(st));
-            double s = 0;
-            for (int q = 0; q < nb; ++q) s += pinned[q];
-            *loss = s / ((double)V * 6);
-        }
+        return DNS_OK;
+    }
+
+    int read_scalar(const double *dsrc, double *out)
+    {
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, dsrc, sizeof(double), hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        *out = pinned[0];
         return DNS_OK;
     }
 
@@ -250,54 +333,203 @@ struct CNN {
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }
+
+    // ------------------------------------------------------------ dataset
+    template <typename T> int append(const T *ub, const T *tau, hipStream_t s)
+    {
+        float *du_ = nullptr, *dt_ = nullptr;
+        DNS_TRY(dalloc(&du_, (size_t)3 * Vs));
+        DNS_TRY(dalloc(&dt_, (size_t)6 * Vs));
+        ds_u.push_back(du_);
+        ds_t.push_back(dt_);
+        hipLaunchKernelGGL((k_cast_f32<T>), dim3(1024), dim3(256), 0, s, ub, 3 * Vs, du_);
+        hipLaunchKernelGGL((k_cast_f32<T>), dim3(1024), dim3(256), 0, s, tau, 6 * Vs, dt_);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    // RMS of the ū and τ components over the dataset
+    int fit_scales()
+    {
+        if (ds_u.empty()) {
+            set_error("empty dataset");
+            return DNS_EINVAL;
+        }
+        double acc[2] = {0, 0};
+        for (int which = 0; which < 2; ++which) {
+            const auto &set = which ? ds_t : ds_u;
+            const long cnt = (which ? 6 : 3) * Vs;
+            for (size_t q = 0; q < set.size(); ++q) {
+                hipLaunchKernelGGL(k_sumsq, dim3(kLossBlocks), dim3(256), 0, st, set[q], cnt, lpart);
+                hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, lpart, kLossBlocks, 1.0, lval);
+                double s_ = 0;
+                DNS_TRY(read_scalar(lval, &s_));
+                acc[which] += s_;
+            }
+            acc[which] /= (double)set.size() * cnt;
+        }
+        if (!(acc[0] > 0) || !(acc[1] > 0)) {
+            set_error("dataset has a zero field");
+            return DNS_EINVAL;
+        }
+        su = std::sqrt(acc[0]);
+        stau = std::sqrt(acc[1]);
+        return DNS_OK;
+    }
+
+    // stages dataset samples idx[0..nb) into batch slots 0..nb (normalised)
+    void stage(const int *idx, int nb)
+    {
+        for (int b = 0; b < nb; ++b) {
+            hipLaunchKernelGGL((k_planar_to_bf16<float>), dim3(512), dim3(256), 0, st, ds_u[idx[b]], 3, Vs,
+                               (float)(1.0 / su), x0 + (size_t)b * Vs * 8, 8);
+            hipLaunchKernelGGL(k_planar_to_interleaved, dim3(512), dim3(256), 0, st, ds_t[idx[b]], 6, Vs,
+                               (float)(1.0 / stau), t + (size_t)b * Vs * 6);
+        }
+    }
+
+    // nsteps Adam steps on batches drawn from epoch-wise shuffles of the
+    // dataset's first ntrain samples (all if ntrain <= 0; mt19937_64(seed));
+    // the normalised loss of each step before its update goes to losses
+    // (host, may be null).
+    int train(int nsteps, float lr, float b1, float b2, float eps, uint64_t seed, int ntrain, double *losses)
+    {
+        const int nds = ntrain > 0 ? std::min(ntrain, (int)ds_u.size()) : (int)ds_u.size();
+        if (nds == 0) {
+            set_error("empty dataset");
+            return DNS_EINVAL;
+        }
+        double *dl = nullptr;
+        DNS_TRY(dalloc(&dl, (size_t)std::max(1, nsteps)));
+        std::mt19937_64 gen(seed);
+        std::vector<int> perm(nds);
+        size_t pos = nds;
+        std::vector<int> idx(B);
+        for (int it = 0; it < nsteps; ++it) {
+            for (int b = 0; b < B; ++b) {
+                if (pos == (size_t)nds) {
+                    for (int q = 0; q < nds; ++q) perm[q] = q;
+                    std::shuffle(perm.begin(), perm.end(), gen);
+                    pos = 0;
+                }
+                idx[b] = perm[pos++];
+            }
+            stage(idx.data(), B);
+            timer.begin(0);
+            DNS_TRY(forward(st, B));
+            timer.end(0);
+            DNS_TRY(backward(dl + it));
+            DNS_TRY(adam(lr, b1, b2, eps));
+        }
+        int rc = DNS_OK;
+        if (losses && nsteps > 0) {
+            if (hipMemcpyAsync(losses, dl, sizeof(double) * nsteps, hipMemcpyDeviceToHost, st) != hipSuccess) rc = DNS_EHIP;
+        }
+        if (hipStreamSynchronize(st) != hipSuccess) rc = DNS_EHIP;
+        hipFree(dl);
+        if (rc != DNS_OK) set_error("HIP error in training loop");
+        return rc;
+    }
+
+    // normalised MSE of the network over dataset samples [first, first + count)
+    int eval(int first, int count, double *out)
+    {
+        if (first < 0 || count < 1 || first + count > (int)ds_u.size()) {
+            set_error("sample range outside the dataset");
+            return DNS_EINVAL;
+        }
+        double acc = 0;
+        std::vector<int> idx(B);
+        for (int q0 = 0; q0 < count; q0 += B) {
+            const int nb = std::min(B, count - q0);
+            for (int b = 0; b < nb; ++b) idx[b] = first + q0 + b;
+            stage(idx.data(), nb);
+            DNS_TRY(forward(st, nb));
+            DNS_TRY(mse(nb, lval));
+            double s_ = 0;
+            DNS_TRY(read_scalar(lval, &s_));
+            acc += s_ * nb;
+        }
+        *out = acc / count;
+        return DNS_OK;
+    }
+
+    // predicted τ (6, M³ planar, physical units) of dataset sample q (host)
+    int predict(int q, float *tau)
+    {
+        if (q < 0 || q >= (int)ds_u.size()) {
+            set_error("sample index outside the dataset");
+            return DNS_EINVAL;
+        }
+        stage(&q, 1);
+        DNS_TRY(forward(st, 1));
+        float *tmp = reinterpret_cast<float *>(dz[0]); // free between training steps
+        hipLaunchKernelGGL((k_interleaved_to_planar<float>), dim3(512), dim3(256), 0, st, y, 6, Vs, stau, tmp);
+        DNS_HIP_CHECK(hipMemcpyAsync(tau, tmp, sizeof(float) * 6 * Vs, hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+
+    // τ of a physical velocity on stream s (slot 0 of the batch buffers)
+    template <typename T> int stress(const T *vel, T *tau, hipStream_t s)
+    {
+        DNS_HIP_CHECK(hipEventRecord(ev_in, st));
+        DNS_HIP_CHECK(hipStreamWaitEvent(s, ev_in, 0));
+        hipLaunchKernelGGL((k_planar_to_bf16<T>), dim3(512), dim3(256), 0, s, vel, 3, Vs, (float)(1.0 / su), x0, 8);
+        DNS_TRY(forward(s, 1));
+        hipLaunchKernelGGL((k_interleaved_to_planar<T>), dim3(512), dim3(256), 0, s, y, 6, Vs, stau, tau);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipEventRecord(ev_out, s));
+        DNS_HIP_CHECK(hipStreamWaitEvent(st, ev_out, 0));
+        return DNS_OK;
+    }
 };
-
-// Planar (c, V) → channels-last BF16 with scale (one sample into batch slot b).
-__global__ void k_planar_to_bf16(const float *x, int cin, long Vs, float scale, unsigned short *out, int ldc)
-{
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < Vs * ldc; e += (long)gridDim.x * blockDim.x) {
-        const long v = e / ldc;
-        const int c = (int)(e - v * ldc);
-        out[e] = c < cin ? f2bf(scale * x[(size_t)c * Vs + v]) : (unsigned short)0;
-    }
-}
-
-// Planar (c, V) FP32 → interleaved (V, c) FP32 with scale.
-__global__ void k_planar_to_interleaved(const float *x, int c_, long Vs, float scale, float *out)
-{
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < Vs * c_; e += (long)gridDim.x * blockDim.x) {
-        const long v = e / c_;
-        const int c = (int)(e - v * c_);
-        out[e] = scale * x[(size_t)c * Vs + v];
-    }
-}
-
-template <typename T> __global__ void k_cast_f32(const T *x, long n, float *out)
-{
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
-        out[e] = (float)x[e];
-}
-
-__global__ void k_sumsq(const float *x, long n, double *part)
-{
-    __shared__ double s[256];
-    double acc = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
-        acc += (double)x[e] * x[e];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
-}
 
 }  // namespace dns
 
 struct dns_cnn {
     std::unique_ptr<dns::CNN> impl;
 };
+
+namespace dns {
+
+template <typename T> int cnn_stress(dns_cnn *h, const T *vel, T *tau, int n, hipStream_t st)
+{
+    if (!h || !h->impl) {
+        set_error("null dns_cnn handle");
+        return DNS_EINVAL;
+    }
+    if (n != h->impl->M) {
+        set_error("LES grid and network grid differ");
+        return DNS_EINVAL;
+    }
+    return h->impl->stress<T>(vel, tau, st);
+}
+
+template <typename T> int cnn_append(dns_cnn *h, const T *ubar, const T *tau, int n, hipStream_t st)
+{
+    if (!h || !h->impl) {
+        set_error("null dns_cnn handle");
+        return DNS_EINVAL;
+    }
+    if (n != h->impl->M) {
+        set_error("LES grid and network grid differ");
+        return DNS_EINVAL;
+    }
+    CNN &c = *h->impl;
+    DNS_HIP_CHECK(hipEventRecord(c.ev_in, c.st));
+    DNS_HIP_CHECK(hipStreamWaitEvent(st, c.ev_in, 0));
+    DNS_TRY(c.append<T>(ubar, tau, st));
+    DNS_HIP_CHECK(hipStreamSynchronize(st));
+    return DNS_OK;
+}
+
+template int cnn_stress<double>(dns_cnn *, const double *, double *, int, hipStream_t);
+template int cnn_stress<float>(dns_cnn *, const float *, float *, int, hipStream_t);
+template int cnn_append<double>(dns_cnn *, const double *, const double *, int, hipStream_t);
+template int cnn_append<float>(dns_cnn *, const float *, const float *, int, hipStream_t);
+
+}  // namespace dns
 
 using namespace dns;
 
@@ -370,7 +602,7 @@ int dns_cnn_forward(dns_cnn *h, const float *x, float *yout)
     DNS_TRY(dalloc(&tmp, (size_t)c.V * 3));
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, x, sizeof(float) * c.V * 3, hipMemcpyHostToDevice, c.st));
     hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 8);
-    DNS_TRY(c.forward());
+    DNS_TRY(c.forward(c.st, c.B));
     DNS_HIP_CHECK(hipMemcpyAsync(yout, c.y, sizeof(float) * c.V * 6, hipMemcpyDeviceToHost, c.st));
     DNS_HIP_CHECK(hipStreamSynchronize(c.st));
     hipFree(tmp);
@@ -387,10 +619,98 @@ int dns_cnn_loss_grad(dns_cnn *h, const float *x, const float *tgt, double *loss
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, x, sizeof(float) * c.V * 3, hipMemcpyHostToDevice, c.st));
     DNS_HIP_CHECK(hipMemcpyAsync(c.t, tgt, sizeof(float) * c.V * 6, hipMemcpyHostToDevice, c.st));
     hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 8);
-    DNS_TRY(c.forward());
-    DNS_TRY(c.backward(loss));
+    DNS_TRY(c.forward(c.st, c.B));
+    DNS_TRY(c.backward(c.lval));
+    double l = 0;
+    DNS_TRY(c.read_scalar(c.lval, &l));
+    if (loss) *loss = l;
+    hipFree(tmp);
+    return DNS_OK;
+}
+
+int dns_cnn_add_sample(dns_cnn *h, const float *ubar, const float *tau)
+{
+    if (!h || !ubar || !tau) return DNS_EINVAL;
+    CNN &c = *h->impl;
+    float *tmp = nullptr;
+    DNS_TRY(dalloc(&tmp, (size_t)c.Vs * 9));
+    DNS_HIP_CHECK(hipMemcpyAsync(tmp, ubar, sizeof(float) * c.Vs * 3, hipMemcpyHostToDevice, c.st));
+    DNS_HIP_CHECK(hipMemcpyAsync(tmp + 3 * c.Vs, tau, sizeof(float) * c.Vs * 6, hipMemcpyHostToDevice, c.st));
+    DNS_TRY(c.append<float>(tmp, tmp + 3 * c.Vs, c.st));
     DNS_HIP_CHECK(hipStreamSynchronize(c.st));
     hipFree(tmp);
+    return DNS_OK;
+}
+
+int dns_cnn_dataset_size(dns_cnn *h, int32_t *n)
+{
+    if (!h || !n) return DNS_EINVAL;
+    *n = (int32_t)h->impl->ds_u.size();
+    return DNS_OK;
+}
+
+int dns_cnn_fit_scales(dns_cnn *h, double *su, double *stau)
+{
+    if (!h) return DNS_EINVAL;
+    DNS_TRY(h->impl->fit_scales());
+    if (su) *su = h->impl->su;
+    if (stau) *stau = h->impl->stau;
+    return DNS_OK;
+}
+
+int dns_cnn_set_scales(dns_cnn *h, double su, double stau)
+{
+    if (!h || !(su > 0) || !(stau > 0)) return DNS_EINVAL;
+    h->impl->su = su;
+    h->impl->stau = stau;
+    return DNS_OK;
+}
+
+int dns_cnn_get_scales(dns_cnn *h, double *su, double *stau)
+{
+    if (!h || !su || !stau) return DNS_EINVAL;
+    *su = h->impl->su;
+    *stau = h->impl->stau;
+    return DNS_OK;
+}
+
+int dns_cnn_train(dns_cnn *h, int32_t nsteps, float lr, float beta1, float beta2, float eps, uint64_t seed,
+                  int32_t ntrain, double *losses)
+{
+    if (!h || nsteps < 0) return DNS_EINVAL;
+    return h->impl->train(nsteps, lr, beta1, beta2, eps, seed, ntrain, losses);
+}
+
+int dns_cnn_eval(dns_cnn *h, int32_t first, int32_t count, double *loss)
+{
+    if (!h || !loss) return DNS_EINVAL;
+    return h->impl->eval(first, count, loss);
+}
+
+int dns_cnn_predict(dns_cnn *h, int32_t sample, float *tau)
+{
+    if (!h || !tau) return DNS_EINVAL;
+    return h->impl->predict(sample, tau);
+}
+
+int dns_cnn_timing(dns_cnn *h, int32_t enable)
+{
+    if (!h) return DNS_EINVAL;
+    h->impl->timer.flush();
+    for (int f = 0; f < 8; ++f) {
+        h->impl->timer.ms[f] = 0;
+        h->impl->timer.cnt[f] = 0;
+    }
+    h->impl->timer.on = enable != 0;
+    return DNS_OK;
+}
+
+int dns_cnn_timing_read(dns_cnn *h, int32_t family, double *total_ms, int64_t *launches)
+{
+    if (!h || family < 0 || family >= 8 || !total_ms || !launches) return DNS_EINVAL;
+    h->impl->timer.flush();
+    *total_ms = h->impl->timer.ms[family];
+    *launches = h->impl->timer.cnt[family];
     return DNS_OK;
 }
 

@@ -97,4 +97,26 @@ __device__ inline dd dd_add1(dd a, double b)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Block-strided loop over [0, total) that keeps U independent loads in
+// flight per thread: ld(idx) for U indices into registers, then st(idx, v)
+// for them.  A plain loop whose body loads from HBM and stores to LDS waits
+// out each load's latency in turn.
+template <int U, typename V, typename LD, typename ST>
+__device__ __forceinline__ void batched_for(int total, LD ld, ST st)
+{
+    for (int base = threadIdx.x; base < total; base += U * (int)blockDim.x) {
+        V r[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int idx = base + q * (int)blockDim.x;
+            if (idx < total) r[q] = ld(idx);
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int idx = base + q * (int)blockDim.x;
+            if (idx < total) st(idx, r[q]);
+        }
+    }
+}
+
 }  // namespace dns

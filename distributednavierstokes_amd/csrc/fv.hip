@@ -70,6 +70,8 @@ template <typename T> struct FVImpl : FVBase {
     T *u[3] = {}, *v[3] = {}, *q[3] = {}, *p = nullptr, *delta = nullptr;
     cplx<T> *A = nullptr;
     double2 *Dth = nullptr;
+    int thomas_tl = 0; // lines per block of the LDS Thomas kernel (0: sequential kernel)
+    size_t ldst = 0;
     double *Dcn = nullptr;
     T *dy_t = nullptr, *rdy_t = nullptr, *rdyu_t = nullptr;
     double *dy_d = nullptr, *dyu_d = nullptr, *rdy_d = nullptr, *rdyu_d = nullptr;
@@ -223,7 +225,6 @@ template <typename T> struct FVImpl : FVBase {
                     }
                 }
             DNS_TRY(dupload(&iptab, ip));
-            DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
         } else {
             std::vector<double> l2(ny);
             for (int k = 0; k < ny; ++k) l2[k] = modlam(k, ny, dyh[1]);
@@ -278,6 +279,19 @@ template <typename T> struct FVImpl : FVBase {
         if (!g.wall) {
             tby = pick_tb(ny);
             ldsy = (size_t)tby * (ny + 1) * sizeof(cplx<T>);
+        }
+        if (g.wall) {
+            // LDS-resident segmented Thomas: TL lines per block within ~80 KB so
+            // that two blocks share a CU; DNS_THOMAS=sequential selects the
+            // one-thread-per-line kernel with an HBM d' scratch.
+            const size_t cap = std::min<size_t>((size_t)maxlds, 80 * 1024);
+            const char *env = getenv("DNS_THOMAS");
+            thomas_tl = 16;
+            auto need = [&](int tl) { return (size_t)ny * (tl + 1) * sizeof(double2) + (size_t)NTHR * 40; };
+            while (thomas_tl > 1 && need(thomas_tl) > cap) thomas_tl /= 2;
+            if (need(thomas_tl) > cap || (env && !strcmp(env, "sequential"))) thomas_tl = 0;
+            ldst = thomas_tl ? need(thomas_tl) : 0;
+            if (!thomas_tl) DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
         }
         if (ldsz > (size_t)maxlds || ldsy > (size_t)maxlds) {
             set_error("FFT tile does not fit the %d-byte LDS limit", maxlds);
@@ -478,9 +492,14 @@ template <typename T> struct FVImpl : FVBase {
         timer.begin(FAM_Y);
         if (g.wall) {
             double scale = 1.0 / ((double)g.nx * g.nz);
-            int lines = g.ldc * g.nz;
-            hipLaunchKernelGGL((k_thomas_y<T>), dim3(ceil_div(lines, NTHR)), dim3(NTHR), 0, st, A, Dth, iptab, ay,
-                               cy, lam1, lam3, scale, g);
+            if (thomas_tl) {
+                hipLaunchKernelGGL((k_thomas_tile<T>), dim3(ceil_div(g.nxc, thomas_tl), g.nz), dim3(NTHR), ldst, st, A,
+                                   iptab, ay, cy, lam1, lam3, scale, thomas_tl, g);
+            } else {
+                int lines = g.ldc * g.nz;
+                hipLaunchKernelGGL((k_thomas_y<T>), dim3(ceil_div(lines, NTHR)), dim3(NTHR), 0, st, A, Dth, iptab, ay,
+                                   cy, lam1, lam3, scale, g);
+            }
         } else {
             double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,

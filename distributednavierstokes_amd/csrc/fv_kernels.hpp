@@ -295,30 +295,33 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
-    for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
-        int b = idx / N, t = idx - b * N;
-        int L = L0 + b;
-        cplx<T> z = mk<T>(T(0), T(0));
-        if (L < nlines) {
-            int j = L % g.ny, k = L / g.ny;
-            int i0 = 2 * t, i1 = 2 * t + 1;
-            if (MODE == 0) {
-                int km = wrapm(k, g.nz), jj = j + 1;
-                T rdyj = a.rdy[jj];
-                T x0 = a.ux[IX(i0, j, k, g)], x1 = a.ux[IX(i1, j, k, g)], xm = a.ux[IX(wrapm(i0, g.nx), j, k, g)];
-                T y0 = ldf(a.uy, i0, jj, k, g) - ldf(a.uy, i0, jj - 1, k, g);
-                T y1 = ldf(a.uy, i1, jj, k, g) - ldf(a.uy, i1, jj - 1, k, g);
-                T z0 = a.uz[IX(i0, j, k, g)] - a.uz[IX(i0, j, km, g)];
-                T z1 = a.uz[IX(i1, j, k, g)] - a.uz[IX(i1, j, km, g)];
-                z.x = (x0 - xm) * a.rdx + y0 * rdyj + z0 * a.rdz;
-                z.y = (x1 - x0) * a.rdx + y1 * rdyj + z1 * a.rdz;
-            } else {
-                z.x = a.ux[IX(i0, j, k, g)];
-                z.y = a.ux[IX(i1, j, k, g)];
+    batched_for<EPT, cplx<T>>(
+        lpb * N,
+        [&](int idx) {
+            int b = idx / N, t = idx - b * N;
+            int L = L0 + b;
+            cplx<T> z = mk<T>(T(0), T(0));
+            if (L < nlines) {
+                int j = L % g.ny, k = L / g.ny;
+                int i0 = 2 * t, i1 = 2 * t + 1;
+                if (MODE == 0) {
+                    int km = wrapm(k, g.nz), jj = j + 1;
+                    T rdyj = a.rdy[jj];
+                    T x0 = a.ux[IX(i0, j, k, g)], x1 = a.ux[IX(i1, j, k, g)], xm = a.ux[IX(wrapm(i0, g.nx), j, k, g)];
+                    T y0 = ldf(a.uy, i0, jj, k, g) - ldf(a.uy, i0, jj - 1, k, g);
+                    T y1 = ldf(a.uy, i1, jj, k, g) - ldf(a.uy, i1, jj - 1, k, g);
+                    T z0 = a.uz[IX(i0, j, k, g)] - a.uz[IX(i0, j, km, g)];
+                    T z1 = a.uz[IX(i1, j, k, g)] - a.uz[IX(i1, j, km, g)];
+                    z.x = (x0 - xm) * a.rdx + y0 * rdyj + z0 * a.rdz;
+                    z.y = (x1 - x0) * a.rdx + y1 * rdyj + z1 * a.rdz;
+                } else {
+                    z.x = a.ux[IX(i0, j, k, g)];
+                    z.y = a.ux[IX(i1, j, k, g)];
+                }
             }
-        }
-        buf[idx] = z;
-    }
+            return z;
+        },
+        [&](int idx, cplx<T> v) { buf[idx] = v; });
     __syncthreads();
     fft_lds<T, EPT, false>(buf, lpb, N, d, a.tw);
     const int nxc = N + 1;
@@ -357,25 +360,28 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
-    for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
-        int b = idx / N, k = idx - b * N;
-        int L = L0 + b;
-        cplx<T> Z = mk<T>(T(0), T(0));
-        if (L < nlines) {
-            cplx<T> Xk = a.A[k + (size_t)g.ldc * L];
-            cplx<T> Xn = a.A[(N - k) + (size_t)g.ldc * L];
-            if (k == 0) {
-                Xk.y = T(0);
-                Xn.y = T(0);
+    batched_for<EPT, cplx<T>>(
+        lpb * N,
+        [&](int idx) {
+            int b = idx / N, k = idx - b * N;
+            int L = L0 + b;
+            cplx<T> Z = mk<T>(T(0), T(0));
+            if (L < nlines) {
+                cplx<T> Xk = a.A[k + (size_t)g.ldc * L];
+                cplx<T> Xn = a.A[(N - k) + (size_t)g.ldc * L];
+                if (k == 0) {
+                    Xk.y = T(0);
+                    Xn.y = T(0);
+                }
+                cplx<T> Xc = cconj<T>(Xn);
+                cplx<T> ev = cadd<T>(Xk, Xc);                     // 2 Ev
+                cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]); // 2 Od = w^-k (Xk − Xc)
+                // Z = Ev + i Od, and brfft = 2 IDFT(Z): fold the 2 in
+                Z = mk<T>(ev.x - od.y, ev.y + od.x);
             }
-            cplx<T> Xc = cconj<T>(Xn);
-            cplx<T> ev = cadd<T>(Xk, Xc);              // 2 Ev
-            cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]); // 2 Od = w^-k (Xk − Xc)
-            // Z = Ev + i Od, and brfft = 2 IDFT(Z): fold the 2 in
-            Z = mk<T>(ev.x - od.y, ev.y + od.x);
-        }
-        buf[idx] = Z;
-    }
+            return Z;
+        },
+        [&](int idx, cplx<T> v) { buf[idx] = v; });
     __syncthreads();
     fft_lds<T, EPT, true>(buf, lpb, N, d, a.tw);
     for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
@@ -398,11 +404,16 @@ __global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
     const int kx0 = blockIdx.x * tb, j = blockIdx.y;
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb;
-        int kx = kx0 + b;
-        buf[b * ls + t] = kx < g.nxc ? A[CX(kx, j, t, g)] : mk<T>(T(0), T(0));
-    }
+    batched_for<EPT, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            int t = idx / tb, kx = kx0 + (idx - t * tb);
+            return kx < g.nxc ? A[CX(kx, j, t, g)] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> v) {
+            int t = idx / tb, b = idx - t * tb;
+            buf[b * ls + t] = v;
+        });
     __syncthreads();
     fft_lds<T, EPT, INV>(buf, tb, ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
@@ -428,11 +439,16 @@ __global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
     const int kx0 = blockIdx.x * tb, kz = blockIdx.y;
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb;
-        int kx = kx0 + b;
-        buf[b * ls + t] = kx < g.nxc ? A[CX(kx, t, kz, g)] : mk<T>(T(0), T(0));
-    }
+    batched_for<EPT, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            int t = idx / tb, kx = kx0 + (idx - t * tb);
+            return kx < g.nxc ? A[CX(kx, t, kz, g)] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> v) {
+            int t = idx / tb, b = idx - t * tb;
+            buf[b * ls + t] = v;
+        });
     __syncthreads();
     fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
     if (DIVIDE) {
@@ -529,6 +545,130 @@ __global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *__restrict__ A, doubl
                 xn = make_double2(dr[q].x - cr[q] * xn.x, dr[q].y - cr[q] * xn.y);
                 A[base + j * cs] = mk<T>(T(xn.x), T(xn.y));
             }
+        }
+    }
+}
+
+// LDS-resident variant of the same solve: a block owns TL consecutive kx
+// lines of one kz plane (all ny rows staged in LDS as FP64), and each line's
+// forward sweep and back substitution -- both first-order affine recurrences
+// x_j = α_j x_{j∓1} + β_j -- run as S = blockDim/TL segments in parallel:
+// every segment solves its rows from a zero inflow while accumulating Π α,
+// one thread per line chains the S segment maps, then every segment adds
+// (Π α) x_in.  HBM traffic: r read and x written once, the pivot table read
+// twice (d' never leaves LDS).  Same pivots and mean-mode rule as k_thomas_y.
+// grid (ceil(nxc / TL), nz), LDS ny (TL + 1) double2 + 3 S TL scalars.
+template <typename T>
+__global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, const double *__restrict__ ip,
+                                                     const double *__restrict__ ay, const double *__restrict__ cy,
+                                                     const double *__restrict__ lam1, const double *__restrict__ lam3,
+                                                     double scale, int TL, FVGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int ny = g.ny, TLp = TL + 1, S = blockDim.x / TL;
+    double2 *X = reinterpret_cast<double2 *>(smem_raw);
+    double2 *Bs = X + (size_t)ny * TLp;            // [S][TL] segment offsets / inflows
+    double *As = reinterpret_cast<double *>(Bs + (size_t)S * TL);
+    double2 *Xin = reinterpret_cast<double2 *>(As + (size_t)S * TL);
+    const int kx0 = blockIdx.x * TL, kz = blockIdx.y;
+    const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
+    const size_t abase = (size_t)g.ldc * (size_t)ny * kz, ibase = (size_t)g.nxc * (size_t)ny * kzs;
+    batched_for<8, cplx<T>>(
+        ny * TL,
+        [&](int idx) {
+            const int j = idx / TL, kx = kx0 + (idx - j * TL);
+            return kx < g.nxc ? A[abase + kx + (size_t)g.ldc * j] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> a) {
+            const int j = idx / TL, l = idx - j * TL;
+            X[j * TLp + l] = make_double2((double)a.x * scale, (double)a.y * scale);
+        });
+    const int l = threadIdx.x % TL, sg = threadIdx.x / TL, kx = kx0 + l;
+    const bool valid = kx < g.nxc;
+    const bool mean = valid && (lam1[kx] + lam3[kz]) == 0.0;
+    const int R = (ny + S - 1) / S, j0 = min(ny, sg * R), j1 = min(ny, j0 + R);
+    const double *ipl = ip + ibase + kx;
+    __syncthreads();
+    // forward sweep d'_j = −a_j ip_j d'_{j−1} + ip_j r_j (row 0: no coupling; 0 for the mean mode)
+    {
+        double P = 1.0;
+        double2 y = make_double2(0.0, 0.0);
+        for (int j = j0; j < j1; ++j) {
+            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
+            const double al = j == 0 ? 0.0 : -ay[j] * w;
+            const double2 r = X[j * TLp + l];
+            const double be = (j == 0 && mean) ? 0.0 : w;
+            y = make_double2(al * y.x + be * r.x, al * y.y + be * r.y);
+            X[j * TLp + l] = y;
+            P *= al;
+        }
+        As[sg * TL + l] = P;
+        Bs[sg * TL + l] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double2 xi = make_double2(0.0, 0.0);
+        for (int q = 0; q < S; ++q) {
+            Xin[q * TL + l] = xi;
+            const double a = As[q * TL + l];
+            const double2 b = Bs[q * TL + l];
+            xi = make_double2(a * xi.x + b.x, a * xi.y + b.y);
+        }
+    }
+    __syncthreads();
+    {
+        const double2 xi = Xin[sg * TL + l];
+        double P = 1.0;
+        for (int j = j0; j < j1; ++j) {
+            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
+            P *= j == 0 ? 0.0 : -ay[j] * w;
+            const double2 y = X[j * TLp + l];
+            X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+        }
+    }
+    __syncthreads();
+    // back substitution x_j = d'_j − c'_j x_{j+1}, c'_j = c_j ip_j (0 on the last row and row 0 of the mean mode)
+    {
+        double P = 1.0;
+        double2 y = make_double2(0.0, 0.0);
+        for (int j = j1 - 1; j >= j0; --j) {
+            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
+            const double al = (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -cy[j] * w;
+            const double2 d = X[j * TLp + l];
+            y = make_double2(al * y.x + d.x, al * y.y + d.y);
+            X[j * TLp + l] = y;
+            P *= al;
+        }
+        As[sg * TL + l] = P;
+        Bs[sg * TL + l] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double2 xi = make_double2(0.0, 0.0);
+        for (int q = S - 1; q >= 0; --q) {
+            Xin[q * TL + l] = xi;
+            const double a = As[q * TL + l];
+            const double2 b = Bs[q * TL + l];
+            xi = make_double2(a * xi.x + b.x, a * xi.y + b.y);
+        }
+    }
+    __syncthreads();
+    {
+        const double2 xi = Xin[sg * TL + l];
+        double P = 1.0;
+        for (int j = j1 - 1; j >= j0; --j) {
+            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
+            P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -cy[j] * w;
+            const double2 y = X[j * TLp + l];
+            X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < ny * TL; idx += blockDim.x) {
+        const int j = idx / TL, l2 = idx - j * TL, kx2 = kx0 + l2;
+        if (kx2 < g.nxc) {
+            const double2 x = X[j * TLp + l2];
+            A[abase + kx2 + (size_t)g.ldc * j] = mk<T>(T(x.x), T(x.y));
         }
     }
 }

@@ -6,6 +6,7 @@
 #include <memory>
 #include <vector>
 
+#include "closure.hpp"
 #include "host_util.hpp"
 #include "spectral_kernels.hpp"
 
@@ -61,7 +62,7 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         if (st) hipStreamSynchronize(st);
         timer.flush();
-        void *ptrs[] = {uh, ustart, du, P, Q, phys, twN, twx, twn, red, redout, factors, aux};
+        void *ptrs[] = {uh, ustart, du, P, Q, phys, twN, twx, twn, red, redout, factors, aux, tphys};
         for (void *p_ : ptrs)
             if (p_) hipFree(p_);
         if (pinned) hipHostFree(pinned);
@@ -204,11 +205,11 @@ template <typename T> struct SpecImpl : SpecBase {
 
     // vpart: per-block velocity bounds (stage 1 and vsummax read them; later
     // stages write to the scratch half of the reduction buffer)
-    template <int MODE> int xpass(double *vpart)
+    template <int MODE> int xpass(double *vpart, T *physbuf = nullptr)
     {
         XMidArgs<T> a;
         a.P = P;
-        a.phys = phys;
+        a.phys = physbuf ? physbuf : phys;
         a.vpart = vpart;
         a.tw = twN;
         a.twx = twx;
@@ -231,13 +232,14 @@ template <typename T> struct SpecImpl : SpecBase {
         return DNS_OK;
     }
 
-    template <int MODE> int ystage(int stage, double dt)
+    template <int MODE> int ystage(int stage, double dt, const cplx<T> *extra = nullptr)
     {
         StageArgs<T> s;
         s.Q = Q;
         s.uh = uh;
         s.ustart = ustart;
         s.du = du;
+        s.extra = extra;
         s.tw = twn;
         s.tb = tbs;
         s.first = stage == 0;
@@ -280,6 +282,7 @@ template <typename T> struct SpecImpl : SpecBase {
         sa.uh = uh;
         sa.ustart = ustart;
         sa.du = P;
+        sa.extra = nullptr;
         sa.tw = twn;
         sa.tb = tbs;
         sa.first = 1;
@@ -319,6 +322,62 @@ template <typename T> struct SpecImpl : SpecBase {
                            tb, g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
+    }
+
+    // Forward transform of three physical components at src (n³ planar each)
+    // into the compact spectra at dst (K, m, m, 3).
+    int phys_to_spec_into(T *src, cplx<T> *dst)
+    {
+        DNS_TRY(xpass<XMID_R2C>(red, src));
+        DNS_TRY(zfwd(3));
+        hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, dst, twn,
+                           tb, g, fdn);
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    // LES step with a learned sub-filter stress: each stage evaluates the
+    // network on the physical velocity, transforms τ to (K, m, m, 6) and adds
+    // it to σ̂ inside the fused stage (du_i = −ik_j(σ̂_ij + τ̂_ij)).
+    T *tphys = nullptr; // physical τ (n³ × 6)
+    int closure_step(dns_cnn *net, double dt, double *vmax)
+    {
+        DNS_TRY(ensure_phys());
+        DNS_TRY(ensure_aux());
+        if (!du) DNS_TRY(dalloc(&du, 3 * ns));
+        const size_t n3 = (size_t)g.n * g.n * g.n;
+        if (!tphys) DNS_TRY(dalloc(&tphys, 6 * n3));
+        for (int s = 0; s < 3; ++s) {
+            DNS_TRY(backward_yz());
+            DNS_TRY(xpass<XMID_C2R>(red + redcap / 2));
+            DNS_TRY(cnn_stress<T>(net, phys, tphys, g.n, st));
+            DNS_TRY(phys_to_spec_into(tphys, aux));
+            DNS_TRY(phys_to_spec_into(tphys + 3 * n3, aux + 3 * ns));
+            DNS_TRY(backward_yz());
+            DNS_TRY(xpass<XMID_RHS>(s == 0 ? red : red + redcap / 2));
+            DNS_TRY(zfwd(6));
+            DNS_TRY(ystage<0>(s, dt, aux));
+        }
+        if (vmax) DNS_TRY(vmax_final(ceil_div((long)g.n * g.n, lpb), vmax));
+        return DNS_OK;
+    }
+
+    // Appends the physical (ū, τ) of the last SFS sample to the network's dataset.
+    int append_sample(dns_cnn *net)
+    {
+        if (!aux) {
+            set_error("no SFS sample on this handle");
+            return DNS_EINVAL;
+        }
+        DNS_TRY(ensure_phys());
+        const size_t n3 = (size_t)g.n * g.n * g.n;
+        if (!tphys) DNS_TRY(dalloc(&tphys, 6 * n3));
+        DNS_TRY(spec_to_phys_from(aux));
+        DNS_HIP_CHECK(hipMemcpyAsync(tphys, phys, sizeof(T) * 3 * n3, hipMemcpyDeviceToDevice, st));
+        DNS_TRY(spec_to_phys_from(aux + 3 * ns));
+        DNS_HIP_CHECK(hipMemcpyAsync(tphys + 3 * n3, phys, sizeof(T) * 3 * n3, hipMemcpyDeviceToDevice, st));
+        DNS_TRY(spec_to_phys_from(uh));
+        return cnn_append<T>(net, phys, tphys, g.n, st);
     }
 
     int project() override
@@ -752,6 +811,30 @@ int dns_sfs_physical(dns_spec *les, double *ubar, double *tau)
     SP_CHECK(les);
     if (auto *l64 = dynamic_cast<SpecImpl<double> *>(les->impl.get())) return sfs_physical(l64, ubar, tau);
     if (auto *l32 = dynamic_cast<SpecImpl<float> *>(les->impl.get())) return sfs_physical(l32, ubar, tau);
+    return DNS_EINVAL;
+}
+
+int dns_les_closure_step(dns_spec *les, dns_cnn *net, double dt, double *vmax)
+{
+    SP_CHECK(les);
+    if (!net) {
+        set_error("null dns_cnn handle");
+        return DNS_EINVAL;
+    }
+    if (auto *l64 = dynamic_cast<SpecImpl<double> *>(les->impl.get())) return l64->closure_step(net, dt, vmax);
+    if (auto *l32 = dynamic_cast<SpecImpl<float> *>(les->impl.get())) return l32->closure_step(net, dt, vmax);
+    return DNS_EINVAL;
+}
+
+int dns_sfs_append(dns_spec *les, dns_cnn *net)
+{
+    SP_CHECK(les);
+    if (!net) {
+        set_error("null dns_cnn handle");
+        return DNS_EINVAL;
+    }
+    if (auto *l64 = dynamic_cast<SpecImpl<double> *>(les->impl.get())) return l64->append_sample(net);
+    if (auto *l32 = dynamic_cast<SpecImpl<float> *>(les->impl.get())) return l32->append_sample(net);
     return DNS_EINVAL;
 }
 

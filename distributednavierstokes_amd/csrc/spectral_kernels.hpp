@@ -50,13 +50,19 @@ __global__ void __launch_bounds__(256) k_sp_yback(const cplx<T> *uh, cplx<T> *B1
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, kz = blockIdx.y, c = blockIdx.z;
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb, i = i0 + b;
-        int jc = sp_compact(t, g);
-        cplx<T> v = mk<T>(T(0), T(0));
-        if (i < K && jc >= 0) v = uh[i + (size_t)K * (jc + (size_t)m * (kz + (size_t)m * c))];
-        buf[b * ls + t] = v;
-    }
+    batched_for<EPT, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            int jc = sp_compact(t, g);
+            cplx<T> v = mk<T>(T(0), T(0));
+            if (i < K && jc >= 0) v = uh[i + (size_t)K * (jc + (size_t)m * (kz + (size_t)m * c))];
+            return v;
+        },
+        [&](int idx, cplx<T> v) {
+            int t = idx / tb, b = idx - t * tb;
+            buf[b * ls + t] = v;
+        });
     __syncthreads();
     fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
@@ -75,13 +81,19 @@ __global__ void __launch_bounds__(256) k_sp_zback(const cplx<T> *B1, cplx<T> *P,
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, y = blockIdx.y, c = blockIdx.z;
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb, i = i0 + b;
-        int kc = sp_compact(t, g);
-        cplx<T> v = mk<T>(T(0), T(0));
-        if (i < K && kc >= 0) v = B1[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * c))];
-        buf[b * ls + t] = v;
-    }
+    batched_for<EPT, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            int kc = sp_compact(t, g);
+            cplx<T> v = mk<T>(T(0), T(0));
+            if (i < K && kc >= 0) v = B1[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * c))];
+            return v;
+        },
+        [&](int idx, cplx<T> v) {
+            int t = idx / tb, b = idx - t * tb;
+            buf[b * ls + t] = v;
+        });
     __syncthreads();
     fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
@@ -100,10 +112,16 @@ __global__ void __launch_bounds__(256) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, c
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, y = blockIdx.y, p = blockIdx.z;
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb, i = i0 + b;
-        buf[b * ls + t] = i < K ? P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * p))] : mk<T>(T(0), T(0));
-    }
+    batched_for<EPT, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            return i < K ? P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * p))] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> v) {
+            int t = idx / tb, b = idx - t * tb;
+            buf[b * ls + t] = v;
+        });
     __syncthreads();
     fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
@@ -123,10 +141,16 @@ __global__ void __launch_bounds__(256) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out,
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, kc = blockIdx.y, p = blockIdx.z;
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb, i = i0 + b;
-        buf[b * ls + t] = i < K ? Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
-    }
+    batched_for<EPT, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            return i < K ? Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> v) {
+            int t = idx / tb, b = idx - t * tb;
+            buf[b * ls + t] = v;
+        });
     __syncthreads();
     fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
     const T inv = T(1.0 / ((double)n * n * n));
@@ -168,24 +192,27 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     const size_t pstride = (size_t)K * n * n; // component stride of P
     if (MODE != XMID_R2C) {
         for (int c = 0; c < 3; ++c) {
-            for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
-                int b = idx / N, k = idx - b * N, L = L0 + b;
-                cplx<T> Z = mk<T>(T(0), T(0));
-                if (L < nlines) {
-                    const cplx<T> *row = a.P + (size_t)L * K + c * pstride;
-                    cplx<T> Xk = k < K ? row[k] : mk<T>(T(0), T(0));
-                    cplx<T> Xn = (N - k) < K ? row[N - k] : mk<T>(T(0), T(0));
-                    if (k == 0) {
-                        Xk.y = T(0);
-                        Xn.y = T(0);
+            batched_for<EPT, cplx<T>>(
+                lpb * N,
+                [&](int idx) {
+                    int b = idx / N, k = idx - b * N, L = L0 + b;
+                    cplx<T> Z = mk<T>(T(0), T(0));
+                    if (L < nlines) {
+                        const cplx<T> *row = a.P + (size_t)L * K + c * pstride;
+                        cplx<T> Xk = k < K ? row[k] : mk<T>(T(0), T(0));
+                        cplx<T> Xn = (N - k) < K ? row[N - k] : mk<T>(T(0), T(0));
+                        if (k == 0) {
+                            Xk.y = T(0);
+                            Xn.y = T(0);
+                        }
+                        cplx<T> Xc = cconj<T>(Xn);
+                        cplx<T> ev = cadd<T>(Xk, Xc);
+                        cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]);
+                        Z = mk<T>(ev.x - od.y, ev.y + od.x);
                     }
-                    cplx<T> Xc = cconj<T>(Xn);
-                    cplx<T> ev = cadd<T>(Xk, Xc);
-                    cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]);
-                    Z = mk<T>(ev.x - od.y, ev.y + od.x);
-                }
-                work[idx] = Z;
-            }
+                    return Z;
+                },
+                [&](int idx, cplx<T> v) { work[idx] = v; });
             __syncthreads();
             fft_lds<T, EPT, true>(work, lpb, N, d, a.tw);
             for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
@@ -229,18 +256,21 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     for (int p = 0; p < nout; ++p) {
         const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
         const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
-        for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
-            int b = idx / N, t = idx - b * N, L = L0 + b;
-            cplx<T> z = mk<T>(T(0), T(0));
-            if (MODE == XMID_RHS) {
-                const T *va = v + ((size_t)ca * lpb + b) * n, *vb = v + ((size_t)cb * lpb + b) * n;
-                z = mk<T>(va[2 * t] * vb[2 * t], va[2 * t + 1] * vb[2 * t + 1]);
-            } else if (L < nlines) {
-                const T *src = a.phys + (size_t)n * L + (size_t)p * n * nlines;
-                z = mk<T>(src[2 * t], src[2 * t + 1]);
-            }
-            work[idx] = z;
-        }
+        batched_for<EPT, cplx<T>>(
+            lpb * N,
+            [&](int idx) {
+                int b = idx / N, t = idx - b * N, L = L0 + b;
+                cplx<T> z = mk<T>(T(0), T(0));
+                if (MODE == XMID_RHS) {
+                    const T *va = v + ((size_t)ca * lpb + b) * n, *vb = v + ((size_t)cb * lpb + b) * n;
+                    z = mk<T>(va[2 * t] * vb[2 * t], va[2 * t + 1] * vb[2 * t + 1]);
+                } else if (L < nlines) {
+                    const T *src = a.phys + (size_t)n * L + (size_t)p * n * nlines;
+                    z = mk<T>(src[2 * t], src[2 * t + 1]);
+                }
+                return z;
+            },
+            [&](int idx, cplx<T> v2) { work[idx] = v2; });
         __syncthreads();
         fft_lds<T, EPT, false>(work, lpb, N, d, a.tw);
         for (int idx = threadIdx.x; idx < lpb * K; idx += blockDim.x) {
@@ -269,6 +299,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
 template <typename T> struct StageArgs {
     const cplx<T> *Q;
     cplx<T> *uh, *ustart, *du;
+    const cplx<T> *extra; // (K, m, m, 6) stress added to σ̂ before the divergence, or null
     const cplx<T> *tw;
     int tb;
     int first;   // stage 1: the stage start is uh itself
@@ -290,10 +321,16 @@ __global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, F
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[o][c] = mk<T>(T(0), T(0));
     for (int p = 0; p < 6; ++p) {
-        for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-            int t = idx / tb, b = idx - t * tb, i = i0 + b;
-            buf[b * ls + t] = i < K ? s.Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
-        }
+        batched_for<EPT, cplx<T>>(
+            tb * n,
+            [&](int idx) {
+                int t = idx / tb, i = i0 + (idx - t * tb);
+                return i < K ? s.Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
+            },
+            [&](int idx, cplx<T> v) {
+                int t = idx / tb, b = idx - t * tb;
+                buf[b * ls + t] = v;
+            });
         __syncthreads();
         fft_lds<T, EPT, false>(buf, tb, ls, d, s.tw);
 #pragma unroll
@@ -306,6 +343,7 @@ __global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, F
                         cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
                 } else if (i < K) {
                     cplx<T> sg = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
+                    if (s.extra) sg = cadd<T>(sg, s.extra[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))]);
                     T kx = T(g.kfac * i), ky = T(g.kfac * sp_freq(jc, g));
                     // −i k σ  =  (k σ.y, −k σ.x)
                     auto mi = [](T k, cplx<T> z) { return mk<T>(k * z.y, -k * z.x); };

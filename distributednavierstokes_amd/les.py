@@ -37,6 +37,12 @@ class LESSampler:
         self.nsamples += 1
         return self
 
+    def append_to(self, net):
+        """Adds the last sample's physical (ū, τ) to the device dataset of the
+        closure network ``net`` (an ``SFSNet`` with M equal to this grid)."""
+        check(lib().dns_sfs_append(self.les.handle, net.handle))
+        return self
+
     def spectral(self):
         """ū (K, m, m, 3) and τ̂ (K, m, m, 6) on the LES compact grid."""
         s = self.les

@@ -293,10 +293,13 @@ class SpecState:
         self.uh = DeviceState(s)
 
 
-def spectral_solve_(*, uh, setup, tlims, dt=None, cfl=0.85, forcing=None, tstops=(), nstart=0, processors=None):
+def spectral_solve_(*, uh, setup, tlims, dt=None, cfl=0.85, forcing=None, tstops=(), nstart=0, processors=None,
+                    closure=None):
     """``spectral_solve!`` -- src/spectral/step.jl:118-170.  The state stays
     on the GPU; a processor returning ``"stop"`` ends the solve after the
-    current step.  ``uh`` is updated in place at the end."""
+    current step.  ``uh`` is updated in place at the end.  ``closure`` (an
+    ``SFSNet`` on this grid) adds its learned sub-filter stress to every
+    stage (an LES step); None is the no-model solve."""
     s = setup
     L = lib()
     processors = dict(processors or {})
@@ -329,7 +332,10 @@ def spectral_solve_(*, uh, setup, tlims, dt=None, cfl=0.85, forcing=None, tstops
             istop += 1
         tnext = stops[istop] if istop < len(stops) else tend
         dtn = min(dt if dt is not None else cfl * math.sqrt(3) / (kmax * (vmax + eps)), tnext - t)
-        check(L.dns_spec_step(s.handle, float(dtn), dptr(out) if dt is None else None))
+        if closure is None:
+            check(L.dns_spec_step(s.handle, float(dtn), dptr(out) if dt is None else None))
+        else:
+            check(L.dns_les_closure_step(s.handle, closure.handle, float(dtn), dptr(out) if dt is None else None))
         if forcing is not None:
             forcing(dev, s)
         if dt is None:

@@ -234,6 +234,36 @@ int dns_cnn_loss_grad(dns_cnn *h, const float *x, const float *t, double *loss);
 /* One Adam update of the parameters with the current gradient. */
 int dns_cnn_adam_step(dns_cnn *h, float lr, float beta1, float beta2, float eps);
 int dns_cnn_sync(dns_cnn *h);
+/* Device dataset of (ū, τ) pairs, planar FP32 host arrays (3·M³, 6·M³). */
+int dns_cnn_add_sample(dns_cnn *h, const float *ubar, const float *tau);
+int dns_cnn_dataset_size(dns_cnn *h, int32_t *n);
+/* Normalisation: ū/su in, τ/stau out.  fit_scales sets the RMS of the
+ * dataset's components; set/get handle saved values. */
+int dns_cnn_fit_scales(dns_cnn *h, double *su, double *stau);
+int dns_cnn_set_scales(dns_cnn *h, double su, double stau);
+int dns_cnn_get_scales(dns_cnn *h, double *su, double *stau);
+/* nsteps Adam steps on batches of epoch-wise shuffles (mt19937_64(seed)) of
+ * the first ntrain dataset samples (all if ntrain <= 0); losses[nsteps]
+ * (normalised MSE before each update) may be NULL. */
+int dns_cnn_train(dns_cnn *h, int32_t nsteps, float lr, float beta1, float beta2, float eps, uint64_t seed,
+                  int32_t ntrain, double *losses);
+/* Normalised MSE over dataset samples [first, first + count). */
+int dns_cnn_eval(dns_cnn *h, int32_t first, int32_t count, double *loss);
+/* Predicted τ (6·M³ planar, physical units) of one dataset sample. */
+int dns_cnn_predict(dns_cnn *h, int32_t sample, float *tau);
+/* Kernel timing, families 0 forward, 1 backward, 2 Adam + weight packing. */
+int dns_cnn_timing(dns_cnn *h, int32_t enable);
+int dns_cnn_timing_read(dns_cnn *h, int32_t family, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------ closure LES (no counterpart
+ * in the reference; it ends at the SFS data, src/spectral/les.jl:313-382)
+ * Appends the physical (ū, τ) of the last dns_sfs_sample on `les` to the
+ * network's dataset, on the device. */
+int dns_sfs_append(dns_spec *les, dns_cnn *net);
+/* spectral_step! (src/spectral/step.jl:68-92) of an LES handle (n = the
+ * network's M) with the learned stress: each stage adds τ̂ = F[net(ū)] to σ̂,
+ * du_i = −ik_j(σ̂_ij + τ̂_ij).  vmax as dns_spec_step. */
+int dns_les_closure_step(dns_spec *les, dns_cnn *net, double dt, double *vmax);
 
 #ifdef __cplusplus
 }
