@@ -140,6 +140,159 @@ def run_spectral(n, kcut, precision, steps, warmup, cfl=0.5):
             "kernels": fam, "checks": {"energy": e, "maxdiv": md}}
 
 
+CNN_MACS_PER_VOXEL = 27 * (3 * 64 + 64 * 64 + 64 * 64 + 64 * 6)   # 236,736
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def run_cnn_train(steps, warmup, M=64, batch=16, nsamples=16):
+    """Adam steps of the closure CNN on batches of 16 × 64³ (synthetic
+    pairs); FLOPs per step = 3 × 2 × MACs (forward, data and weight grads)."""
+    from distributednavierstokes_amd.cnn import SFSNet
+
+    net = SFSNet(M=M, batch=batch, seed=0)
+    rng = np.random.default_rng(0)
+    for _ in range(nsamples):
+        net.add_sample(rng.standard_normal((M, M, M, 3)).astype(np.float32),
+                       (0.1 * rng.standard_normal((M, M, M, 6))).astype(np.float32))
+    net.fit_scales()
+    net.train(warmup, lr=1e-3, seed=1)
+    net.timing(True)
+    t0 = time.perf_counter()
+    losses = net.train(steps, lr=1e-3, seed=2)
+    t1 = time.perf_counter()
+    fam = {nm: net.timing_read(f) for f, nm in ((0, "forward"), (1, "backward"), (2, "adam_pack"))}
+    net.timing(False)
+    sps = steps / (t1 - t0)
+    flops = 3 * 2 * CNN_MACS_PER_VOXEL * batch * M ** 3
+    fwd_ms = fam["forward"][0] / max(1, fam["forward"][1])
+    bwd_ms = fam["backward"][0] / max(1, fam["backward"][1])
+    return {"workload": f"CNN 3-64-64-64-6, 3x3x3 periodic, batch {batch} x {M}^3, BF16 MFMA / FP32 accumulate, Adam",
+            "adam_steps_per_s": sps, "ms_per_step": 1e3 / sps, "flop_per_step": flops,
+            "achieved_tflops": flops * sps / 1e12, "frac_bf16_peak": flops * sps / 1e12 / BF16_PEAK_TFLOPS,
+            "forward_ms": fwd_ms, "forward_tflops": flops / 3 / (fwd_ms * 1e-3) / 1e12 if fwd_ms > 0 else None,
+            "backward_ms": bwd_ms, "backward_tflops": 2 * flops / 3 / (bwd_ms * 1e-3) / 1e12 if bwd_ms > 0 else None,
+            "loss_first_last": [float(losses[0]), float(losses[-1])]}
+
+
+def run_les_closure(steps, warmup, M=64):
+    """64³ spectral LES steps with the learned closure (CNN on every stage)
+    and without it (no-model fallback), random IC, fixed Δt."""
+    from distributednavierstokes_amd import _lib
+    from distributednavierstokes_amd.cnn import SFSNet
+    import distributednavierstokes_amd as dns
+
+    s = dns.spectral_setup(n=M, kcut=M // 3, visc=2e-4)
+    dns.spectral_randomfield_(dns.spectral.DeviceState(s), s, totalenergy=1.0, kpeak=4, seed=0)
+    net = SFSNet(M=M, batch=1, seed=0)
+    net.scales = (1.0, 0.01)
+    L = _lib.lib()
+    out = np.zeros(1)
+    res = {}
+    for name, step in (("no_model", lambda: L.dns_spec_step(s.handle, 1e-3, _lib.dptr(out))),
+                       ("closure", lambda: L.dns_les_closure_step(s.handle, net.handle, 1e-3, _lib.dptr(out)))):
+        for _ in range(warmup):
+            _lib.check(step())
+        _lib.check(L.dns_spec_sync(s.handle))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _lib.check(step())
+        _lib.check(L.dns_spec_sync(s.handle))
+        res[name] = steps / (time.perf_counter() - t0)
+    return {"workload": f"spectral LES {M}^3 kcut={M // 3}, FP64, learned closure vs no model",
+            "closure_steps_per_s": res["closure"], "no_model_steps_per_s": res["no_model"]}
+
+
+def run_closure_experiment(n=512, M=64, nsnap=64, every=40, spinup_t=2.0, train_steps=2000, batch=16,
+                           nheld=8, window=0.5, nwin=5, dns_precision="mixed", cfl=0.5, log=print):
+    """Learned-closure experiment: forced HIT DNS (random IC k^4 exp(-2(k/4)^2),
+    E = 1, seed 0, shells 1-2 clamped, nu = 2e-4) spun up to t = spinup_t,
+    then nsnap SFS samples (Gaussian, filter width 2 l/M) every `every` steps
+    on the M^3 grid; the CNN trains on the first nsnap - nheld (batch 16,
+    Adam lr 1e-3, seed 0); a-priori score on the held-out samples; then M^3
+    LES (FP64) with and without the closure from the filtered DNS at the
+    last sample, over `window` time units, against filtered-DNS spectra at
+    nwin equally spaced times."""
+    from distributednavierstokes_amd import closure as cl
+    from distributednavierstokes_amd.cnn import SFSNet
+    from distributednavierstokes_amd.spectral import DeviceState, _shells
+    import distributednavierstokes_amd as dns
+
+    T = np.float32 if dns_precision == "mixed" else np.float64
+    s = dns.spectral_setup(n=n, visc=2e-4, T=T)
+    dev = DeviceState(s)
+    dns.spectral_randomfield_(dev, s, totalenergy=1.0, kpeak=4, seed=0)
+    forcing = dns.shellforcing(dev, s, shells=(1, 2))
+    t_start = time.perf_counter()
+
+    def progress(state, s_):
+        if state.n % 200 == 0:
+            log(f"dns spin-up step {state.n} t={state.t:.3f} ({time.perf_counter() - t_start:.0f} s)")
+
+    st = dns.spectral_solve_(uh=dev, setup=s, tlims=(0.0, spinup_t), cfl=cfl, forcing=forcing,
+                             processors={"log": progress})
+    log(f"spin-up done: {st.n} steps to t={st.t:.3f}")
+    net = SFSNet(M=M, batch=batch, seed=0)
+    kept_idx = tuple(range(nsnap - nheld, nsnap))
+    sam, st2, times, kept = cl.collect_pairs(s, dev, net, M=M, nsnap=nsnap, every=every, spinup=0, cfl=cfl,
+                                             forcing=forcing, keep_physical=kept_idx, t0=st.t)
+    log(f"{net.nsamples} samples at t={times[0]:.3f}..{times[-1]:.3f} ({time.perf_counter() - t_start:.0f} s)")
+    su, stau = net.fit_scales()
+    # LES initial condition: filtered DNS at the last sample
+    ub0, _ = sam.spectral()
+    # continue the DNS over the window, filtered spectra at the stop times
+    stops = [times[-1] + window * (q + 1) / nwin for q in range(nwin)]
+    E_dns = []
+
+    def dns_proc(state, s_):
+        if any(abs(state.t - x) < 1e-9 for x in stops):
+            sam.sample(None)
+            E_dns.append(np.asarray(sam.spectrum()))
+
+    dns.spectral_solve_(uh=dev, setup=s, tlims=(times[-1], stops[-1]), cfl=cfl, forcing=forcing, tstops=stops,
+                        processors={"spec": dns_proc})
+    log(f"DNS window done ({time.perf_counter() - t_start:.0f} s)")
+    # a-priori: untrained vs trained on held-out samples, no model = mean square of normalised tau
+    nofit = float(np.mean([np.mean((kept[q][1] / stau) ** 2) for q in kept_idx]))
+    t0 = time.perf_counter()
+    losses = net.train(train_steps, lr=1e-3, seed=0, ntrain=nsnap - nheld)
+    train_s = time.perf_counter() - t0
+    held = net.evaluate(nsnap - nheld, nheld)
+    corr = float(np.mean([cl.stress_correlation(net.predict(q), kept[q][1]) for q in kept_idx]))
+    log(f"trained {train_steps} steps in {train_s:.1f} s: held-out MSE {held:.4f} (no model {nofit:.4f}), corr {corr:.3f}")
+
+    def les(netarg):
+        les_s = cl.les_setup_like(sam, T=np.float64)
+        E = []
+        uh = np.array(ub0, order="F")
+        fr = dns.shellforcing(uh, les_s, shells=(1, 2))
+
+        def proc(state, s_):
+            if any(abs(state.t - x) < 1e-9 for x in stops):
+                E.append(np.asarray(_shells(s_, range(1, les_s.kcut + 1))))
+
+        dns.spectral_solve_(uh=uh, setup=les_s, tlims=(times[-1], stops[-1]), cfl=cfl, forcing=fr, tstops=stops,
+                            processors={"spec": proc}, closure=netarg)
+        return E
+
+    E_nm, E_cl = les(None), les(net)
+    Ed, Enm, Ecl = np.mean(E_dns, axis=0), np.mean(E_nm, axis=0), np.mean(E_cl, axis=0)
+    err_nm = np.abs(Enm / Ed - 1.0)
+    err_cl = np.abs(Ecl / Ed - 1.0)
+    log(f"LES done ({time.perf_counter() - t_start:.0f} s)")
+    return {"workload": f"HIT {n}^3 DNS ({dns_precision}) -> {nsnap} SFS samples on {M}^3 (Gaussian, 2l/M) -> "
+                        f"CNN {train_steps} Adam steps batch {batch} -> {M}^3 LES closure vs no model",
+            "sample_times": [times[0], times[-1]], "scales": [su, stau],
+            "train_seconds": train_s, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+            "a_priori": {"held_out_mse_trained": held, "held_out_mse_no_model": nofit, "tau_correlation": corr},
+            "a_posteriori": {"window": [times[-1], stops[-1]], "times": nwin,
+                             "spectrum_rel_err_max_closure": float(err_cl.max()),
+                             "spectrum_rel_err_max_no_model": float(err_nm.max()),
+                             "spectrum_rel_err_mean_closure": float(err_cl.mean()),
+                             "spectrum_rel_err_mean_no_model": float(err_nm.mean()),
+                             "E_filtered_dns": Ed.tolist(), "E_les_closure": Ecl.tolist(),
+                             "E_les_no_model": Enm.tolist()}}
+
+
 def fv_form(prob, steps, warmup, fixed_dt=None, ic=None):
     r = run_gpu(prob, steps, warmup, fixed_dt=fixed_dt, ic=ic)
     N = int(np.prod(prob.n))
@@ -271,7 +424,18 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the other configurations")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scales the FETCH/WRITE counters)")
+    ap.add_argument("--closure-experiment", action="store_true",
+                    help="run only the learned-closure experiment (DNS -> SFS data -> training -> LES)")
+    ap.add_argument("--dns-n", type=int, default=512)
+    ap.add_argument("--les-m", type=int, default=64)
+    ap.add_argument("--nsnap", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=2000)
     args = ap.parse_args()
+    if args.closure_experiment:
+        res = run_closure_experiment(n=args.dns_n, M=args.les_m, nsnap=args.nsnap, train_steps=args.train_steps,
+                                     log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(res), flush=True)
+        return
     if args.copy_calibration:
         import torch
 
@@ -320,6 +484,8 @@ def main():
             forms[f"channel_cn_512x384x512_{prec}"] = fv_form(channel_problem((512, 384, 512), "implicit", prec), 10, 2)
         for prec in ("fp64", "mixed"):
             forms[f"hit512_{prec}"] = run_spectral(512, 170, prec, 5, 1)
+        forms["cnn_train_16x64cube"] = run_cnn_train(10, 2)
+        forms["les64_closure"] = run_les_closure(20, 2)
     line = {
         "metric": "RK3 steps/s & Mcells/s, FP64 vs mixed at stated err; %HBM roofline; LES err",
         "value": sps,
