@@ -293,6 +293,70 @@ def run_closure_experiment(n=512, M=64, nsnap=64, every=40, spinup_t=2.0, train_
                              "E_les_no_model": Enm.tolist()}}
 
 
+def run_mixed_accuracy(n=(512, 384, 512), steps=1000, checkpoints=(10, 100, 1000), cfl=0.5, log=print):
+    """Mixed (FP32 storage/stencils/FFTs, FP64 Thomas and reductions) against
+    FP64 on the CN channel from the same IC, the mixed run driven by the FP64
+    run's Δt sequence.  Errors at the checkpoints: relative L2 of the
+    velocity, kinetic energy, mean streamwise profile U(y) and Re_τ
+    (u_τ² = ν dU/dy from the first cell rows at both walls)."""
+    from distributednavierstokes_amd import _lib
+    from distributednavierstokes_amd.fv import ROCmBackend, Setup
+
+    L = _lib.lib()
+    res = {"workload": f"FV channel {n[0]}x{n[1]}x{n[2]}, CN y-diffusion, CFL {cfl}, {steps} steps; "
+                       f"mixed driven by the FP64 dt sequence", "checkpoints": {}}
+    snaps, dts = {}, []
+    for prec in ("fp64", "mixed"):
+        prob = channel_problem(n, "implicit", prec)
+        s = Setup(prob, ROCmBackend())
+        s.upload_u(channel_ic(prob))
+        zero = np.zeros(prob.ghosted_shape, order="F")
+        for name in ("p", "qx", "qy", "qz", "delta"):
+            s.upload(name, zero)
+        _lib.check(L.dns_fv_project(s.handle, _lib.FIELD["delta"]))
+        out = np.zeros(1)
+        t0 = time.perf_counter()
+        for it in range(1, steps + 1):
+            if prec == "fp64":
+                _lib.check(L.dns_fv_cfl_timestep(s.handle, cfl, _lib.dptr(out)))
+                dts.append(float(out[0]))
+            _lib.check(L.dns_fv_step(s.handle, dts[it - 1]))
+            if it % 100 == 0:
+                _lib.check(L.dns_fv_sync(s.handle))
+                log(f"{prec} step {it} ({time.perf_counter() - t0:.0f} s)")
+            if it in checkpoints:
+                u = [s.download(c)[1:-1, 1:-1, 1:-1].copy() for c in ("ux", "uy", "uz")]
+                ke = np.zeros(1)
+                _lib.check(L.dns_fv_kinetic_energy(s.handle, _lib.dptr(ke)))
+                md = np.zeros(1)
+                _lib.check(L.dns_fv_maxdiv(s.handle, _lib.dptr(md)))
+                if prec == "fp64":
+                    snaps[it] = (u, float(ke[0]), float(md[0]))
+                else:
+                    ref, ke_ref, md_ref = snaps.pop(it)
+                    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(u, ref))
+                    den = sum(float(np.sum(b ** 2)) for b in ref)
+                    U, Ur = u[0].mean(axis=(0, 2)), ref[0].mean(axis=(0, 2))
+                    yc = 0.5 * (prob.yfaces[:-1] + prob.yfaces[1:])
+
+                    def retau(prof):
+                        dUdy = 0.5 * (prof[0] / (yc[0] - prob.yfaces[0]) + prof[-1] / (prob.yfaces[-1] - yc[-1]))
+                        return math.sqrt(prob.visc * dUdy) / prob.visc
+
+                    res["checkpoints"][str(it)] = {
+                        "t": float(sum(dts[:it])),
+                        "velocity_rel_l2": math.sqrt(num / den),
+                        "kinetic_energy_rel": abs(float(ke[0]) - ke_ref) / ke_ref,
+                        "mean_profile_rel": float(np.max(np.abs(U - Ur)) / np.max(np.abs(Ur))),
+                        "re_tau_rel": abs(retau(U) - retau(Ur)) / retau(Ur),
+                        "maxdiv_mixed": float(md[0]), "maxdiv_fp64": md_ref}
+                    log(f"checkpoint {it}: {res['checkpoints'][str(it)]}")
+        _lib.check(L.dns_fv_sync(s.handle))
+        res[f"{prec}_seconds"] = time.perf_counter() - t0
+        del s
+    return res
+
+
 def fv_form(prob, steps, warmup, fixed_dt=None, ic=None):
     r = run_gpu(prob, steps, warmup, fixed_dt=fixed_dt, ic=ic)
     N = int(np.prod(prob.n))
@@ -430,7 +494,17 @@ def main():
     ap.add_argument("--les-m", type=int, default=64)
     ap.add_argument("--nsnap", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--mixed-accuracy", action="store_true",
+                    help="run only the mixed-vs-FP64 CN channel comparison (same dt sequence)")
+    ap.add_argument("--accuracy-steps", type=int, default=1000)
     args = ap.parse_args()
+    if args.mixed_accuracy:
+        cps = tuple(c for c in (10, 100, 1000) if c <= args.accuracy_steps) or (args.accuracy_steps,)
+        res = run_mixed_accuracy(tuple(args.n) if tuple(args.n) != (256, 192, 256) else (512, 384, 512),
+                                 steps=args.accuracy_steps, checkpoints=cps,
+                                 log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(res), flush=True)
+        return
     if args.closure_experiment:
         res = run_closure_experiment(n=args.dns_n, M=args.les_m, nsnap=args.nsnap, train_steps=args.train_steps,
                                      log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -4,6 +4,7 @@
 // with FP32 accumulation (v_mfma_f32_32x32x8_bf16).  C ABI: dns_cnn_*.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -139,16 +140,18 @@ struct CNN {
         DNS_HIP_CHECK(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
         DNS_HIP_CHECK(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
         timer.st = st;
+        const char *env = getenv("DNS_CNN_MFMA");
+        mfma_k8 = env && !strcmp(env, "k8");
         const int kin[4] = {3, 64, 64, 64}, nout[4] = {64, 64, 64, 6};
         size_t off = 0;
         for (int l = 0; l < 4; ++l) {
             CNNLayer &c = L[l];
             c.kin = kin[l];
-            c.kinp = (kin[l] + 7) / 8 * 8;
+            c.kinp = (kin[l] + 15) / 16 * 16;
             c.nout = nout[l];
             c.np = (nout[l] + 31) / 32 * 32;
             c.kpt = (kin[l] + 31) / 32 * 32;
-            c.noutp = (nout[l] + 7) / 8 * 8;
+            c.noutp = (nout[l] + 15) / 16 * 16;
             c.woff = off;
             off += (size_t)27 * c.nout * c.kin;
             c.boff = off;
@@ -173,18 +176,21 @@ struct CNN {
         DNS_TRY(dupload(&v, h));
         DNS_TRY(dupload(&g, h));
         DNS_TRY(pack());
-        DNS_TRY(dalloc(&x0, (size_t)V * 8));
+        DNS_TRY(dalloc(&x0, (size_t)V * 16));
         for (int q = 0; q < 3; ++q) {
             DNS_TRY(dalloc(&z[q], (size_t)V * 64));
             DNS_TRY(dalloc(&a[q], (size_t)V * 64));
         }
-        DNS_TRY(dalloc(&dy, (size_t)V * 8));
+        DNS_TRY(dalloc(&dy, (size_t)V * 16));
         DNS_TRY(dalloc(&dz[0], (size_t)V * 64));
         DNS_TRY(dalloc(&dz[1], (size_t)V * 64));
         DNS_TRY(dalloc(&y, (size_t)V * 6));
         DNS_TRY(dalloc(&t, (size_t)V * 6));
         nsplit = (int)std::max<long>(1, std::min<long>(64, V / 4096));
-        DNS_TRY(dalloc(&wpart, (size_t)nsplit * 27 * 64 * 64));
+        // the row-staged weight-gradient kernel splits the B·M² x-rows over at most 256 blocks
+        row_wgrad = !mfma_k8 && M % 16 == 0 && M <= 256;
+        const int nsplit16 = row_wgrad ? (int)std::min<long>(256, (long)B * M * M) : 0;
+        DNS_TRY(dalloc(&wpart, (size_t)std::max(nsplit, nsplit16) * 27 * 64 * 64));
         DNS_TRY(dalloc(&bpart, (size_t)64 * bblocks));
         DNS_TRY(dalloc(&lpart, kLossBlocks));
         DNS_TRY(dalloc(&lval, 1));
@@ -205,10 +211,29 @@ struct CNN {
 
     ConvGeom geom(long Vn) const { return ConvGeom{M, (int)(Vn / Vs), Vn}; }
 
-    template <int WM, int WN, int EPI> void conv(const ConvArgs &ca, int np, long Vn, hipStream_t s)
+    // 3x3x3 convolution launch: 4 waves of (64 voxels × 32·NB channels) on
+    // the K = 16 MFMA; DNS_CNN_MFMA=k8 selects the 32x32x8 kernel instead.
+    bool mfma_k8 = false;
+    bool row_wgrad = false; // k_conv_wgrad16 (M % 16 == 0) instead of k_conv_wgrad
+    template <int EPI> void conv(const ConvArgs &ca, int np, long Vn, hipStream_t s)
     {
-        dim3 grd((unsigned)((Vn + 32 * WM - 1) / (32 * WM)), (unsigned)((np + 32 * WN - 1) / (32 * WN)));
-        hipLaunchKernelGGL((k_conv<WM, WN, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
+        if (mfma_k8) {
+            if (np % 64 == 0) {
+                dim3 grd((unsigned)((Vn + 63) / 64), (unsigned)(np / 64));
+                hipLaunchKernelGGL((k_conv<2, 2, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
+            } else {
+                dim3 grd((unsigned)((Vn + 127) / 128), (unsigned)((np + 31) / 32));
+                hipLaunchKernelGGL((k_conv<4, 1, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
+            }
+            return;
+        }
+        if (np % 64 == 0) {
+            dim3 grd((unsigned)((Vn + 255) / 256), (unsigned)(np / 64));
+            hipLaunchKernelGGL((k_conv16<2, 2, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
+        } else {
+            dim3 grd((unsigned)((Vn + 255) / 256), (unsigned)(np / 32));
+            hipLaunchKernelGGL((k_conv16<2, 1, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
+        }
     }
 
     // forward of the first nb batch slots of x0; the FP32 output lands in y (nb M³ × 6)
@@ -229,11 +254,11 @@ struct CNN {
                 ca.out = a[l];
                 ca.zbuf = z[l];
                 ca.ldo = 64;
-                conv<2, 2, 0>(ca, c.np, Vn, s);
+                conv<0>(ca, c.np, Vn, s);
             } else {
                 ca.yout = y;
                 ca.ldo = 6;
-                conv<4, 1, 1>(ca, c.np, Vn, s);
+                conv<1>(ca, c.np, Vn, s);
             }
         }
         DNS_HIP_CHECK(hipGetLastError());
@@ -252,10 +277,28 @@ struct CNN {
         wa.kin = c.kinp;
         wa.np = c.np;
         wa.kp = c.kpt;
-        wa.chunk = (V + nsplit - 1) / nsplit;
-        wa.chunk = (wa.chunk + 63) / 64 * 64;
-        hipLaunchKernelGGL(k_conv_wgrad, dim3(nsplit, 27), dim3(256), 0, st, wa, geom(V));
-        hipLaunchKernelGGL(k_wgrad_reduce, dim3(256), dim3(256), 0, st, wpart, nsplit, c.np, c.kpt, c.nout, c.kin,
+        int ns = nsplit;
+        if (row_wgrad) {
+            WGrad16Args w16;
+            w16.dout = dout;
+            w16.in = wa.in;
+            w16.part = wpart;
+            w16.ldd = ldd;
+            w16.nout = c.nout;
+            w16.kin = c.kinp;
+            w16.np = c.np;
+            w16.kp = c.kpt;
+            const long nrows = (long)B * M * M;
+            w16.rows_per_block = (int)((nrows + 255) / 256);
+            ns = (int)((nrows + w16.rows_per_block - 1) / w16.rows_per_block);
+            const size_t lds = sizeof(unsigned short) * 4 * 64 * (size_t)(M + 8);
+            hipLaunchKernelGGL(k_conv_wgrad16, dim3(ns, 9), dim3(256), lds, st, w16, geom(V));
+        } else {
+            wa.chunk = (V + nsplit - 1) / nsplit;
+            wa.chunk = (wa.chunk + 63) / 64 * 64;
+            hipLaunchKernelGGL(k_conv_wgrad, dim3(nsplit, 27), dim3(256), 0, st, wa, geom(V));
+        }
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(256), dim3(256), 0, st, wpart, ns, c.np, c.kpt, c.nout, c.kin,
                            g + c.woff);
         hipLaunchKernelGGL(k_bias_grad, dim3(bblocks, c.nout), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);
         hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, st, bpart, bblocks, c.nout, g + c.boff);
@@ -267,7 +310,7 @@ struct CNN {
     int mse(int nb, double *ldst)
     {
         const long Vn = (long)nb * Vs;
-        hipLaunchKernelGGL(k_mse, dim3(kLossBlocks), dim3(256), 0, st, y, t, 6, Vn, dy, 8, lpart);
+        hipLaunchKernelGGL(k_mse, dim3(kLossBlocks), dim3(256), 0, st, y, t, 6, Vn, dy, 16, lpart);
         hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, lpart, kLossBlocks, 1.0 / ((double)Vn * 6), ldst);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
@@ -278,7 +321,7 @@ struct CNN {
     {
         timer.begin(1);
         DNS_TRY(mse(B, ldst));
-        DNS_TRY(wgrad(3, dy, 8));
+        DNS_TRY(wgrad(3, dy, 16));
         // dz3 = (W4ᵀ * dy) ⊙ gelu'(z3), then layers 3, 2
         const unsigned short *dout = dy;
         for (int l = 3; l >= 1; --l) {
@@ -294,7 +337,7 @@ struct CNN {
             ca.zprev = z[l - 1];
             ca.out = dst;
             ca.sign = -1;
-            conv<2, 2, 2>(ca, c.kpt, V, st);
+            conv<2>(ca, c.kpt, V, st);
             DNS_TRY(wgrad(l - 1, dst, 64));
             dout = dst;
         }
@@ -382,7 +425,7 @@ struct CNN {
     {
         for (int b = 0; b < nb; ++b) {
             hipLaunchKernelGGL((k_planar_to_bf16<float>), dim3(512), dim3(256), 0, st, ds_u[idx[b]], 3, Vs,
-                               (float)(1.0 / su), x0 + (size_t)b * Vs * 8, 8);
+                               (float)(1.0 / su), x0 + (size_t)b * Vs * 16, 16);
             hipLaunchKernelGGL(k_planar_to_interleaved, dim3(512), dim3(256), 0, st, ds_t[idx[b]], 6, Vs,
                                (float)(1.0 / stau), t + (size_t)b * Vs * 6);
         }
@@ -475,7 +518,7 @@ struct CNN {
     {
         DNS_HIP_CHECK(hipEventRecord(ev_in, st));
         DNS_HIP_CHECK(hipStreamWaitEvent(s, ev_in, 0));
-        hipLaunchKernelGGL((k_planar_to_bf16<T>), dim3(512), dim3(256), 0, s, vel, 3, Vs, (float)(1.0 / su), x0, 8);
+        hipLaunchKernelGGL((k_planar_to_bf16<T>), dim3(512), dim3(256), 0, s, vel, 3, Vs, (float)(1.0 / su), x0, 16);
         DNS_TRY(forward(s, 1));
         hipLaunchKernelGGL((k_interleaved_to_planar<T>), dim3(512), dim3(256), 0, s, y, 6, Vs, stau, tau);
         DNS_HIP_CHECK(hipGetLastError());
@@ -601,7 +644,7 @@ int dns_cnn_forward(dns_cnn *h, const float *x, float *yout)
     float *tmp = nullptr;
     DNS_TRY(dalloc(&tmp, (size_t)c.V * 3));
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, x, sizeof(float) * c.V * 3, hipMemcpyHostToDevice, c.st));
-    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 8);
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 16);
     DNS_TRY(c.forward(c.st, c.B));
     DNS_HIP_CHECK(hipMemcpyAsync(yout, c.y, sizeof(float) * c.V * 6, hipMemcpyDeviceToHost, c.st));
     DNS_HIP_CHECK(hipStreamSynchronize(c.st));
@@ -618,7 +661,7 @@ int dns_cnn_loss_grad(dns_cnn *h, const float *x, const float *tgt, double *loss
     DNS_TRY(dalloc(&tmp, (size_t)c.V * 3));
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, x, sizeof(float) * c.V * 3, hipMemcpyHostToDevice, c.st));
     DNS_HIP_CHECK(hipMemcpyAsync(c.t, tgt, sizeof(float) * c.V * 6, hipMemcpyHostToDevice, c.st));
-    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 8);
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 16);
     DNS_TRY(c.forward(c.st, c.B));
     DNS_TRY(c.backward(c.lval));
     double l = 0;

@@ -18,6 +18,7 @@
 namespace dns {
 
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 __device__ inline unsigned short f2bf(float f)
@@ -135,6 +136,81 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a, ConvGeom g)
     }
 }
 
+// Same convolution on v_mfma_f32_32x32x16_bf16 (gfx950): each lane feeds 8
+// consecutive channels (one 16-byte load per operand), and each wave owns a
+// (32·MB voxels) × (32·NB channels) tile so every A load feeds NB MFMAs and
+// every B load MB.  Block: 4 waves stacked along the voxels.  KIN must be a
+// multiple of 16 and NP a multiple of 32·NB (one block column per 32·NB).
+// The k order inside an MFMA is the same permutation for A and B, so the
+// products do not depend on how the hardware assigns k to lanes.
+template <int MB, int NB, int EPI>
+__global__ void __launch_bounds__(256) k_conv16(ConvArgs a, ConvGeom g)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long vbase = ((long)blockIdx.x * 4 + wave) * (32 * MB);
+    const int n0 = blockIdx.y * (32 * NB);
+    const int row = lane & 31, kh = 8 * (lane >> 5);
+    long vrow[MB];
+    bool vok[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        vrow[mb] = vbase + 32 * mb + row;
+        vok[mb] = vrow[mb] < g.V;
+    }
+    f32x16 acc[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[mb][nb][r] = 0.0f;
+    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+    for (int o = 0; o < 27; ++o) {
+        const unsigned short *ap[MB];
+        const unsigned short *bp[NB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+            ap[mb] = a.in + (vok[mb] ? conv_nb(vrow[mb], o, a.sign, g) : 0) * a.kin + kh;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) bp[nb] = a.W + ((size_t)o * a.np + n0 + 32 * nb + row) * a.kin + kh;
+        for (int k0 = 0; k0 < a.kin; k0 += 16) {
+            bf16x8 av[MB], bv[NB];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+                av[mb] = __builtin_bit_cast(bf16x8, vok[mb] ? *reinterpret_cast<const uint4 *>(ap[mb] + k0) : zero4);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) bv[nb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(bp[nb] + k0));
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[mb], bv[nb], acc[mb][nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int col = n0 + 32 * nb + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long vv = vbase + 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (vv >= g.V || col >= a.nout) continue;
+                const float c = acc[mb][nb][r];
+                if (EPI == 0) {
+                    const float z = c + a.bias[col];
+                    a.zbuf[vv * a.ldo + col] = f2bf(z);
+                    a.out[vv * a.ldo + col] = f2bf(gelu_f(z));
+                } else if (EPI == 1) {
+                    a.yout[vv * a.ldo + col] = c + a.bias[col];
+                } else {
+                    const float zp = bf2f(a.zprev[vv * a.ldo + col]);
+                    a.out[vv * a.ldo + col] = f2bf(c * gelu_d(zp));
+                }
+            }
+        }
+}
+
 // Weight gradient of one offset per block-row: for o = blockIdx.y,
 // dW[o][n][k] += Σ_{v in chunk} dout[v][n] · in[v + d_o][k], tiles of 64
 // voxels staged (transposed) through LDS.  Each block covers a range of
@@ -194,6 +270,101 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(WGradArgs a, ConvGeom g)
         for (int r = 0; r < 16; ++r) {
             const int n = 32 * wn + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             a.part[(((size_t)blockIdx.x * 27 + o) * a.np + n) * a.kp + col] = acc[r];
+        }
+    }
+}
+
+// Weight gradient on the K = 16 MFMA, organised by x-rows: a block owns a
+// range of rows (y, z, b) and one (dy, dz) pair (blockIdx.y), stages each
+// row's output gradient dout[x][n] transposed as sd[n][x] and the matching
+// input row at (y + dy, z + dz) as three x-shifted copies si[dx][k][x], so the
+// three dx offsets are aligned 16-byte LDS reads; a wave owns up to three of
+// the (n-tile, k-tile, dx) 32×32 accumulators.  Partials
+// part[blockIdx.x][o][n][k] as k_conv_wgrad.  Needs M % 16 == 0, np and kp
+// in {32, 64}, ldd and kin multiples of 8.  LDS 4·64·(M + 8) halves.
+struct WGrad16Args {
+    const unsigned short *dout; // [V][ldd]
+    const unsigned short *in;   // [V][kin]
+    float *part;
+    int ldd, nout, kin, np, kp;
+    int rows_per_block;
+};
+
+__global__ void __launch_bounds__(256) k_conv_wgrad16(WGrad16Args a, ConvGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    unsigned short *lds = reinterpret_cast<unsigned short *>(smem_raw);
+    const int M = g.M, LD = M + 8;
+    unsigned short *sd = lds;           // [64][LD]
+    unsigned short *si = lds + 64 * LD; // [3][64][LD]
+    const int grp = blockIdx.y, dy = grp % 3 - 1, dz = grp / 3 - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, h = lane >> 5;
+    const int NT = a.np / 32, KT = a.kp / 32, ntiles = NT * KT * 3;
+    const int nc8 = a.np / 8, kc8 = a.kp / 8;
+    f32x16 acc[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+    const long nrows = (long)g.B * M * M;
+    const long r0 = (long)blockIdx.x * a.rows_per_block;
+    const long r1 = min(nrows, r0 + a.rows_per_block);
+    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+    for (long r = r0; r < r1; ++r) {
+        const int y = (int)(r % M);
+        const long t = r / M;
+        const int z = (int)(t % M);
+        const long b = t / M;
+        const int ys = y + dy < 0 ? y + dy + M : (y + dy >= M ? y + dy - M : y + dy);
+        const int zs = z + dz < 0 ? z + dz + M : (z + dz >= M ? z + dz - M : z + dz);
+        const long vrow = r * M, vsrc = (long)M * (ys + (long)M * (zs + (long)M * b));
+        __syncthreads();
+        for (int e = threadIdx.x; e < M * nc8; e += blockDim.x) {
+            const int x = e / nc8, c8 = (e - x * nc8) * 8;
+            const uint4 v = c8 + 8 <= a.ldd ? *reinterpret_cast<const uint4 *>(a.dout + (vrow + x) * a.ldd + c8) : zero4;
+            const unsigned short *hv = reinterpret_cast<const unsigned short *>(&v);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sd[(c8 + q) * LD + x] = hv[q];
+        }
+        for (int e = threadIdx.x; e < M * kc8; e += blockDim.x) {
+            const int x = e / kc8, c8 = (e - x * kc8) * 8;
+            const uint4 v = c8 + 8 <= a.kin ? *reinterpret_cast<const uint4 *>(a.in + (vsrc + x) * a.kin + c8) : zero4;
+            const unsigned short *hv = reinterpret_cast<const unsigned short *>(&v);
+            const int xm = x + 1 == M ? 0 : x + 1, xp = x == 0 ? M - 1 : x - 1; // dx = −1, +1 destinations
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                si[(0 * 64 + c8 + q) * LD + xm] = hv[q];
+                si[(1 * 64 + c8 + q) * LD + x] = hv[q];
+                si[(2 * 64 + c8 + q) * LD + xp] = hv[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ti = 0; ti < 3; ++ti) {
+            const int tile = wave + 4 * ti;
+            if (tile >= ntiles) break;
+            const int dxi = tile % 3, rest = tile / 3, nt = rest % NT, kt = rest / NT;
+            const unsigned short *pa = sd + (32 * nt + row) * LD + 8 * h;
+            const unsigned short *pb = si + (dxi * 64 + 32 * kt + row) * LD + 8 * h;
+            for (int x0 = 0; x0 < M; x0 += 16) {
+                const bf16x8 av = *reinterpret_cast<const bf16x8 *>(pa + x0);
+                const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(pb + x0);
+                acc[ti] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[ti], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int ti = 0; ti < 3; ++ti) {
+        const int tile = wave + 4 * ti;
+        if (tile >= ntiles) break;
+        const int dxi = tile % 3, rest = tile / 3, nt = rest % NT, kt = rest / NT;
+        const int o = dxi + 3 * grp;
+        const int k = 32 * kt + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int n = 32 * nt + 8 * (r >> 2) + 4 * h + (r & 3);
+            a.part[(((size_t)blockIdx.x * 27 + o) * a.np + n) * a.kp + k] = acc[ti][r];
         }
     }
 }

@@ -58,6 +58,7 @@ static const double RK_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 static const double RK_ALPHA[3] = {1.0 / 3.0, 5.0 / 12.0, 1.0 / 4.0};
 
 constexpr int X_EPT = 8, ZY_EPT = 8, NTHR = 256;
+constexpr int THOMAS_RMAX = 16, CN_RMAX = 32; // rows per segment of the LDS tridiagonal kernels
 constexpr int CH_NCHUNK = 16;
 constexpr int RED_BLOCKS = 1024;
 
@@ -72,6 +73,8 @@ template <typename T> struct FVImpl : FVBase {
     double2 *Dth = nullptr;
     int thomas_tl = 0; // lines per block of the LDS Thomas kernel (0: sequential kernel)
     size_t ldst = 0;
+    int cn_tl = 0; // x-lines per block of the LDS CN kernel (0: sequential kernel)
+    size_t ldscn = 0;
     double *Dcn = nullptr;
     T *dy_t = nullptr, *rdy_t = nullptr, *rdyu_t = nullptr;
     double *dy_d = nullptr, *dyu_d = nullptr, *rdy_d = nullptr, *rdyu_d = nullptr;
@@ -79,7 +82,7 @@ template <typename T> struct FVImpl : FVBase {
     double *ydc[6] = {}, *cncoef = nullptr;
     FFTDesc fdx{}, fdz{}, fdy{};
     cplx<T> *twN = nullptr, *twx = nullptr, *twz = nullptr, *twy = nullptr;
-    int lpb = 1, tbz = 1, tby = 1;
+    int lpb = 1, tbz = 1, tby = 1, tbzf = 1, tbyf = 1; // tile lines and FFT sub-batch lines
     size_t ldsx = 0, ldsz = 0, ldsy = 0;
     double *red = nullptr, *redout = nullptr, *upart = nullptr;
     size_t redcap = 0;
@@ -254,7 +257,6 @@ template <typename T> struct FVImpl : FVBase {
             std::vector<double> *sets[6] = {&at, &bt, &ct, &an, &bn, &cn};
             for (int s = 0; s < 6; ++s) DNS_TRY(dupload(&ydc[s], *sets[s]));
             DNS_TRY(dalloc(&cncoef, 4 * (size_t)ny));
-            DNS_TRY(dalloc(&Dcn, (size_t)N));
         }
 
         // FFT tiles
@@ -274,10 +276,18 @@ template <typename T> struct FVImpl : FVBase {
             while (tb > 1 && (size_t)tb * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tb;
             return std::min(tb, g.nxc);
         };
-        tbz = pick_tb(nz);
+        // a block may load more lines than one FFT sub-batch (tb >= tbf); wider
+        // row segments measured no faster here, so the tile is one sub-batch
+        auto tile_lines = [&](int tbf, int n) {
+            (void)n;
+            return tbf;
+        };
+        tbzf = pick_tb(nz);
+        tbz = tile_lines(tbzf, nz);
         ldsz = (size_t)tbz * (nz + 1) * sizeof(cplx<T>);
         if (!g.wall) {
-            tby = pick_tb(ny);
+            tbyf = pick_tb(ny);
+            tby = tile_lines(tbyf, ny);
             ldsy = (size_t)tby * (ny + 1) * sizeof(cplx<T>);
         }
         if (g.wall) {
@@ -287,11 +297,35 @@ template <typename T> struct FVImpl : FVBase {
             const size_t cap = std::min<size_t>((size_t)maxlds, 80 * 1024);
             const char *env = getenv("DNS_THOMAS");
             thomas_tl = 16;
-            auto need = [&](int tl) { return (size_t)ny * (tl + 1) * sizeof(double2) + (size_t)NTHR * 40; };
-            while (thomas_tl > 1 && need(thomas_tl) > cap) thomas_tl /= 2;
-            if (need(thomas_tl) > cap || (env && !strcmp(env, "sequential"))) thomas_tl = 0;
+            auto need = [&](int tl) {
+                return (size_t)ny * (tl + 1) * sizeof(double2) + (size_t)NTHR * 40 + (size_t)2 * ny * sizeof(double);
+            };
+            auto rows = [&](int tl) { return (ny + NTHR / tl - 1) / (NTHR / tl); };
+            while (thomas_tl > 1 && (need(thomas_tl) > cap || rows(thomas_tl) > THOMAS_RMAX)) thomas_tl /= 2;
+            if (need(thomas_tl) > cap || rows(thomas_tl) > THOMAS_RMAX || (env && !strcmp(env, "sequential")))
+                thomas_tl = 0;
             ldst = thomas_tl ? need(thomas_tl) : 0;
             if (!thomas_tl) DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
+            if (imexy && (size_t)6 * ny * sizeof(double) > (size_t)maxlds) {
+                set_error("unsupported grid: implicit y-diffusion coefficients need %zu bytes of LDS",
+                          (size_t)6 * ny * sizeof(double));
+                return DNS_EUNSUPPORTED;
+            }
+            if (imexy) {
+                // LDS segmented CN: up to 16 x-lines per block within ~80 KB;
+                // DNS_CN=sequential selects the one-thread-per-column kernel.
+                const char *envc = getenv("DNS_CN");
+                cn_tl = 16;
+                auto needc = [&](int tl) {
+                    return (size_t)ny * (tl + 1) * sizeof(double) + (size_t)3 * NTHR * sizeof(double) +
+                           (size_t)6 * ny * sizeof(double);
+                };
+                auto rowsc = [&](int tl) { return (ny + NTHR / tl - 1) / (NTHR / tl); };
+                while (cn_tl > 1 && (needc(cn_tl) > cap || rowsc(cn_tl) > CN_RMAX)) cn_tl /= 2;
+                if (needc(cn_tl) > cap || rowsc(cn_tl) > CN_RMAX || (envc && !strcmp(envc, "sequential"))) cn_tl = 0;
+                ldscn = cn_tl ? needc(cn_tl) : 0;
+                if (!cn_tl) DNS_TRY(dalloc(&Dcn, (size_t)N));
+            }
         }
         if (ldsz > (size_t)maxlds || ldsy > (size_t)maxlds) {
             set_error("FFT tile does not fit the %d-byte LDS limit", maxlds);
@@ -486,14 +520,15 @@ template <typename T> struct FVImpl : FVBase {
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
         hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
-                           twz, tbz, g, fdz);
+                           twz, tbz, tbzf, g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Y);
         if (g.wall) {
             double scale = 1.0 / ((double)g.nx * g.nz);
             if (thomas_tl) {
-                hipLaunchKernelGGL((k_thomas_tile<T>), dim3(ceil_div(g.nxc, thomas_tl), g.nz), dim3(NTHR), ldst, st, A,
+                hipLaunchKernelGGL((k_thomas_tile<T, THOMAS_RMAX>), dim3(ceil_div(g.nxc, thomas_tl), g.nz), dim3(NTHR),
+                                   ldst, st, A,
                                    iptab, ay, cy, lam1, lam3, scale, thomas_tl, g);
             } else {
                 int lines = g.ldc * g.nz;
@@ -503,13 +538,13 @@ template <typename T> struct FVImpl : FVBase {
         } else {
             double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,
-                               st, A, twy, lam1, lam2, lam3, scale, tby, g, fdy);
+                               st, A, twy, lam1, lam2, lam3, scale, tby, tbyf, g, fdy);
         }
         timer.end(FAM_Y);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
         hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
-                           twz, tbz, g, fdz);
+                           twz, tbz, tbzf, g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         XInvArgs<T> xi;
@@ -553,8 +588,8 @@ template <typename T> struct FVImpl : FVBase {
     {
         const double theta = alpha * dt * visc / 2;
         timer.begin(FAM_CN);
-        hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(64), 0, st, ydc[0], ydc[1], ydc[2], ydc[3], ydc[4], ydc[5], theta,
-                           g.ny, cncoef);
+        hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(NTHR), (size_t)6 * g.ny * sizeof(double), st, ydc[0], ydc[1], ydc[2],
+                           ydc[3], ydc[4], ydc[5], theta, g.ny, cncoef);
         for (int c = 0; c < 3; ++c) {
             CNArgs<T> a;
             a.u = u[c];
@@ -573,10 +608,17 @@ template <typename T> struct FVImpl : FVBase {
             a.theta = theta;
             a.rdx = 1.0 / dx;
             a.rdz = 1.0 / dz;
-            dim3 blk(64), grd(ceil_div(g.nx, 64), g.nz);
-            if (c == 0) hipLaunchKernelGGL((k_cn<T, 0>), grd, blk, 0, st, a, g);
-            else if (c == 1) hipLaunchKernelGGL((k_cn<T, 1>), grd, blk, 0, st, a, g);
-            else hipLaunchKernelGGL((k_cn<T, 2>), grd, blk, 0, st, a, g);
+            if (cn_tl) {
+                dim3 grd(ceil_div(g.nx, cn_tl), g.nz);
+                if (c == 0) hipLaunchKernelGGL((k_cn_tile<T, 0, CN_RMAX>), grd, dim3(NTHR), ldscn, st, a, g, cn_tl);
+                else if (c == 1) hipLaunchKernelGGL((k_cn_tile<T, 1, CN_RMAX>), grd, dim3(NTHR), ldscn, st, a, g, cn_tl);
+                else hipLaunchKernelGGL((k_cn_tile<T, 2, CN_RMAX>), grd, dim3(NTHR), ldscn, st, a, g, cn_tl);
+            } else {
+                dim3 blk(64), grd(ceil_div(g.nx, 64), g.nz);
+                if (c == 0) hipLaunchKernelGGL((k_cn<T, 0>), grd, blk, 0, st, a, g);
+                else if (c == 1) hipLaunchKernelGGL((k_cn<T, 1>), grd, blk, 0, st, a, g);
+                else hipLaunchKernelGGL((k_cn<T, 2>), grd, blk, 0, st, a, g);
+            }
         }
         timer.end(FAM_CN);
         DNS_HIP_CHECK(hipGetLastError());
@@ -621,8 +663,11 @@ template <typename T> struct FVImpl : FVBase {
     int cfl(double cflv, double *dt) override
     {
         if (umax_valid) {
-            // partial maxima left by the pressure-gradient pass that produced u
-            hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, upart, nb_pgrad, 3, redout);
+            // partial maxima left by the pressure-gradient pass that produced u,
+            // reduced in two levels (up to 512 blocks, then one)
+            const int G = std::min(512, std::max(1, nb_pgrad / 1024));
+            hipLaunchKernelGGL(k_max_mid, dim3(G), dim3(NTHR), 0, st, upart, nb_pgrad, 3, red);
+            hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, red, G, 3, redout);
         } else {
             const int nb = std::min(RED_BLOCKS, ceil_div(N, NTHR));
             hipLaunchKernelGGL((k_maxabs3<T>), dim3(nb), dim3(NTHR), 0, st, u[0], u[1], u[2], N, red);
@@ -716,9 +761,9 @@ template <typename T> struct FVImpl : FVBase {
             hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(g.ny * g.nz, lpb)), dim3(NTHR), ldsx, st, xa, g,
                                fdx);
             hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
-                               twz, tbz, g, fdz);
+                               twz, tbz, tbzf, g, fdz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,
-                               st, A, twy, lam1, lam2, lam3, 1.0, tby, g, fdy);
+                               st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, g, fdy);
             hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR), sizeof(double) * nbins, st, A, g, nbins,
                                1.0 / N2, red);
             hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nbins, NTHR)), dim3(NTHR), 0, st, red, nb, nbins, dout);

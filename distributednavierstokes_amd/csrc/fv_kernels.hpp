@@ -14,6 +14,13 @@
 #pragma once
 
 #include "common.hpp"
+
+// Loads kept in flight per thread by the HBM→LDS staging loops (capped by the
+// tile's elements per thread).
+#ifndef FV_LOAD_MAX
+#define FV_LOAD_MAX 16
+#endif
+#define FV_LOAD_U(e) ((e) < FV_LOAD_MAX ? (e) : FV_LOAD_MAX)
 #include "fft_lds.hpp"
 #include "reduce.hpp"
 
@@ -295,7 +302,7 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
-    batched_for<EPT, cplx<T>>(
+    batched_for<FV_LOAD_U(EPT), cplx<T>>(
         lpb * N,
         [&](int idx) {
             int b = idx / N, t = idx - b * N;
@@ -360,7 +367,7 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
-    batched_for<EPT, cplx<T>>(
+    batched_for<FV_LOAD_U(EPT), cplx<T>>(
         lpb * N,
         [&](int idx) {
             int b = idx / N, k = idx - b * N;
@@ -398,13 +405,13 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
 // consecutive kx at fixed j (coalesced tb·16-byte row segments).
 // ------------------------------------------------------------------------
 template <typename T, int EPT, bool INV>
-__global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, FVGeom g, FFTDesc d)
+__global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, int tbf, FVGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
     const int kx0 = blockIdx.x * tb, j = blockIdx.y;
-    batched_for<EPT, cplx<T>>(
+    batched_for<FV_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
             int t = idx / tb, kx = kx0 + (idx - t * tb);
@@ -415,7 +422,7 @@ __global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds<T, EPT, INV>(buf, tb, ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, INV>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb;
         int kx = kx0 + b;
@@ -433,13 +440,13 @@ __global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
 template <typename T, int EPT, bool DIVIDE>
 __global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw, const double *lam1,
                                                    const double *lam2, const double *lam3, double scale, int tb,
-                                                   FVGeom g, FFTDesc d)
+                                                   int tbf, FVGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
     const int kx0 = blockIdx.x * tb, kz = blockIdx.y;
-    batched_for<EPT, cplx<T>>(
+    batched_for<FV_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
             int t = idx / tb, kx = kx0 + (idx - t * tb);
@@ -450,7 +457,7 @@ __global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     if (DIVIDE) {
         for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
             int t = idx / tb, b = idx - t * tb;
@@ -462,7 +469,7 @@ __global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = mk<T>(T((double)v.x * f), T((double)v.y * f));
         }
         __syncthreads();
-        fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
+        for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     }
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb;
@@ -555,10 +562,12 @@ __global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *__restrict__ A, doubl
 // x_j = α_j x_{j∓1} + β_j -- run as S = blockDim/TL segments in parallel:
 // every segment solves its rows from a zero inflow while accumulating Π α,
 // one thread per line chains the S segment maps, then every segment adds
-// (Π α) x_in.  HBM traffic: r read and x written once, the pivot table read
-// twice (d' never leaves LDS).  Same pivots and mean-mode rule as k_thomas_y.
-// grid (ceil(nxc / TL), nz), LDS ny (TL + 1) double2 + 3 S TL scalars.
-template <typename T>
+// (Π α) x_in.  A thread keeps its rows' pivots in registers (at most RMAX
+// rows per segment) and the y coefficients sit in LDS, so no global load
+// waits inside a recurrence.  HBM traffic: r read and x written once, the
+// pivot table read once.  Same pivots and mean-mode rule as k_thomas_y.
+// grid (ceil(nxc / TL), nz); LDS ny (TL + 1) double2 + 2 ny + 5 blockDim doubles.
+template <typename T, int RMAX>
 __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, const double *__restrict__ ip,
                                                      const double *__restrict__ ay, const double *__restrict__ cy,
                                                      const double *__restrict__ lam1, const double *__restrict__ lam3,
@@ -567,43 +576,53 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
     extern __shared__ unsigned char smem_raw[];
     const int ny = g.ny, TLp = TL + 1, S = blockDim.x / TL;
     double2 *X = reinterpret_cast<double2 *>(smem_raw);
-    double2 *Bs = X + (size_t)ny * TLp;            // [S][TL] segment offsets / inflows
-    double *As = reinterpret_cast<double *>(Bs + (size_t)S * TL);
-    double2 *Xin = reinterpret_cast<double2 *>(As + (size_t)S * TL);
+    double2 *Bs = X + (size_t)ny * TLp; // [S][TL] segment offsets
+    double2 *Xin = Bs + blockDim.x;     // [S][TL] segment inflows
+    double *As = reinterpret_cast<double *>(Xin + blockDim.x);
+    double *sa = As + blockDim.x, *sc = sa + ny;
     const int kx0 = blockIdx.x * TL, kz = blockIdx.y;
     const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
     const size_t abase = (size_t)g.ldc * (size_t)ny * kz, ibase = (size_t)g.nxc * (size_t)ny * kzs;
-    batched_for<8, cplx<T>>(
-        ny * TL,
-        [&](int idx) {
-            const int j = idx / TL, kx = kx0 + (idx - j * TL);
-            return kx < g.nxc ? A[abase + kx + (size_t)g.ldc * j] : mk<T>(T(0), T(0));
-        },
-        [&](int idx, cplx<T> a) {
-            const int j = idx / TL, l = idx - j * TL;
-            X[j * TLp + l] = make_double2((double)a.x * scale, (double)a.y * scale);
-        });
     const int l = threadIdx.x % TL, sg = threadIdx.x / TL, kx = kx0 + l;
     const bool valid = kx < g.nxc;
-    const bool mean = valid && (lam1[kx] + lam3[kz]) == 0.0;
     const int R = (ny + S - 1) / S, j0 = min(ny, sg * R), j1 = min(ny, j0 + R);
-    const double *ipl = ip + ibase + kx;
+    double w[RMAX];
+#pragma unroll
+    for (int q = 0; q < RMAX; ++q) w[q] = (valid && j0 + q < j1) ? ip[ibase + kx + (size_t)g.nxc * (j0 + q)] : 0.0;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        sa[j] = ay[j];
+        sc[j] = cy[j];
+    }
+    batched_for<FV_LOAD_U(8), cplx<T>>(
+        ny * TL,
+        [&](int idx) {
+            const int j = idx / TL, kx2 = kx0 + (idx - j * TL);
+            return kx2 < g.nxc ? A[abase + kx2 + (size_t)g.ldc * j] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> a) {
+            const int j = idx / TL, l2 = idx - j * TL;
+            X[j * TLp + l2] = make_double2((double)a.x * scale, (double)a.y * scale);
+        });
+    const bool mean = valid && (lam1[kx] + lam3[kz]) == 0.0;
     __syncthreads();
     // forward sweep d'_j = −a_j ip_j d'_{j−1} + ip_j r_j (row 0: no coupling; 0 for the mean mode)
     {
         double P = 1.0;
         double2 y = make_double2(0.0, 0.0);
-        for (int j = j0; j < j1; ++j) {
-            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
-            const double al = j == 0 ? 0.0 : -ay[j] * w;
-            const double2 r = X[j * TLp + l];
-            const double be = (j == 0 && mean) ? 0.0 : w;
-            y = make_double2(al * y.x + be * r.x, al * y.y + be * r.y);
-            X[j * TLp + l] = y;
-            P *= al;
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double al = j == 0 ? 0.0 : -sa[j] * w[q];
+                const double be = (j == 0 && mean) ? 0.0 : w[q];
+                const double2 r = X[j * TLp + l];
+                y = make_double2(al * y.x + be * r.x, al * y.y + be * r.y);
+                X[j * TLp + l] = y;
+                P *= al;
+            }
         }
-        As[sg * TL + l] = P;
-        Bs[sg * TL + l] = y;
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
     }
     __syncthreads();
     if (sg == 0) {
@@ -617,13 +636,16 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
     }
     __syncthreads();
     {
-        const double2 xi = Xin[sg * TL + l];
+        const double2 xi = Xin[threadIdx.x];
         double P = 1.0;
-        for (int j = j0; j < j1; ++j) {
-            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
-            P *= j == 0 ? 0.0 : -ay[j] * w;
-            const double2 y = X[j * TLp + l];
-            X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                P *= j == 0 ? 0.0 : -sa[j] * w[q];
+                const double2 y = X[j * TLp + l];
+                X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+            }
         }
     }
     __syncthreads();
@@ -631,16 +653,19 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
     {
         double P = 1.0;
         double2 y = make_double2(0.0, 0.0);
-        for (int j = j1 - 1; j >= j0; --j) {
-            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
-            const double al = (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -cy[j] * w;
-            const double2 d = X[j * TLp + l];
-            y = make_double2(al * y.x + d.x, al * y.y + d.y);
-            X[j * TLp + l] = y;
-            P *= al;
+#pragma unroll
+        for (int q = RMAX - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double al = (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
+                const double2 d = X[j * TLp + l];
+                y = make_double2(al * y.x + d.x, al * y.y + d.y);
+                X[j * TLp + l] = y;
+                P *= al;
+            }
         }
-        As[sg * TL + l] = P;
-        Bs[sg * TL + l] = y;
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
     }
     __syncthreads();
     if (sg == 0) {
@@ -654,13 +679,16 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
     }
     __syncthreads();
     {
-        const double2 xi = Xin[sg * TL + l];
+        const double2 xi = Xin[threadIdx.x];
         double P = 1.0;
-        for (int j = j1 - 1; j >= j0; --j) {
-            const double w = valid ? ipl[(size_t)g.nxc * j] : 0.0;
-            P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -cy[j] * w;
-            const double2 y = X[j * TLp + l];
-            X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+#pragma unroll
+        for (int q = RMAX - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
+                const double2 y = X[j * TLp + l];
+                X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+            }
         }
     }
     __syncthreads();
@@ -795,22 +823,195 @@ __global__ void __launch_bounds__(256) k_cn(CNArgs<T> a, FVGeom g)
     }
 }
 
+// LDS-resident variant of k_cn: a block owns TL consecutive x-lines of one
+// z plane; each line's rows are split into S = blockDim/TL segments (at most
+// RMAX rows each) that run the forward sweep (d'_j = θ a_j ip_j d'_{j−1} +
+// ip_j rhs_j) and the back substitution (x_j = d'_j − c'_j x_{j+1}) as
+// segment-parallel affine recurrences (local solve from zero inflow with
+// Π α, chained per line by one thread, then corrected), with d' held in LDS
+// as FP64 and the row coefficients staged in LDS.  Each thread loads CH rows
+// of u, p, q ahead of its recurrence.  u is updated in place: a line's rows
+// are read only by this block, and all reads precede the first barrier.
+// grid (ceil(nx / TL), nz); LDS ny (TL + 1) + 3 blockDim + 6 ny doubles.
+template <typename T, int COMP, int RMAX>
+__global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int CH = 8;
+    const int ny = g.ny, TLp = TL + 1, S = blockDim.x / TL;
+    double *X = reinterpret_cast<double *>(smem_raw);
+    double *As = X + (size_t)ny * TLp, *Bs = As + blockDim.x, *Xin = Bs + blockDim.x;
+    double *ca = Xin + blockDim.x, *cb = ca + ny, *cc = cb + ny, *cf = cc + ny, *ci = cf + ny, *cq = ci + ny;
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        ca[j] = a.a[j];
+        cb[j] = a.b[j];
+        cc[j] = a.c[j];
+        cf[j] = j == 0 ? 0.0 : a.theta * a.a[j] * a.ip[j]; // forward α
+        ci[j] = a.ip[j];
+        cq[j] = j == ny - 1 ? 0.0 : -a.cp[j]; // backward α
+    }
+    const int l = threadIdx.x % TL, sg = threadIdx.x / TL;
+    const int i = blockIdx.x * TL + l, k = blockIdx.y;
+    const bool valid = i < g.nx;
+    const int ii = valid ? i : 0; // loads of an idle lane stay in bounds; its results are discarded
+    const int ip1 = wrapp(ii, g.nx), kp1 = wrapp(k, g.nz);
+    const int jact = (COMP == 1 && g.wall) ? ny - 1 : ny;
+    const int R = (ny + S - 1) / S, j0 = min(ny, sg * R), j1 = min(ny, j0 + R);
+    auto uat = [&](int j) -> double { return (j >= 0 && j < ny) ? (double)a.u[IX(ii, j, k, g)] : 0.0; };
+    // right-hand sides of the segment's rows: rhs_j = u_j + B q_j − αΔt ∂p + θ (Dy u)_j
+    double rhs[RMAX];
+    {
+        double um = uat(j0 - 1);
+#pragma unroll
+        for (int c0 = 0; c0 < RMAX; c0 += CH) {
+            double ur[CH + 1], vr[CH];
+#pragma unroll
+            for (int q = 0; q <= CH; ++q) ur[q] = (j0 + c0 + q <= j1) ? uat(j0 + c0 + q) : 0.0;
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                const int j = j0 + c0 + q;
+                vr[q] = 0.0;
+                if (j < j1 && j < jact) {
+                    const int c = IX(ii, j, k, g);
+                    double dpdc;
+                    if (COMP == 0) dpdc = ((double)a.p[IX(ip1, j, k, g)] - (double)a.p[c]) * a.rdx;
+                    else if (COMP == 1) dpdc = ((double)ldc(a.p, ii, j + 2, k, g, T(1)) - (double)a.p[c]) * a.rdyu[j + 1];
+                    else dpdc = ((double)a.p[IX(ii, j, kp1, g)] - (double)a.p[c]) * a.rdz;
+                    vr[q] = a.B * (double)a.q[c] - a.adt * dpdc;
+                }
+            }
+            __syncthreads(); // first pass: the coefficient arrays are staged before use
+#pragma unroll
+            for (int q = 0; q < CH; ++q) {
+                const int j = j0 + c0 + q;
+                rhs[c0 + q] = 0.0;
+                if (j < j1) {
+                    const double u0 = ur[q], up = ur[q + 1];
+                    double lap = cb[j] * u0;
+                    if (j > 0) lap += ca[j] * um;
+                    if (j < ny - 1) lap += cc[j] * up;
+                    rhs[c0 + q] = u0 + vr[q] + a.theta * lap;
+                    um = u0;
+                }
+            }
+        }
+    }
+    // forward sweep, local to the segment
+    {
+        double P = 1.0, y = 0.0;
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                y = cf[j] * y + ci[j] * rhs[q];
+                X[j * TLp + l] = y;
+                P *= cf[j];
+            }
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double xi = 0.0;
+        for (int q = 0; q < S; ++q) {
+            Xin[q * TL + l] = xi;
+            xi = As[q * TL + l] * xi + Bs[q * TL + l];
+        }
+    }
+    __syncthreads();
+    {
+        const double xi = Xin[threadIdx.x];
+        double P = 1.0;
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                P *= cf[j];
+                X[j * TLp + l] += P * xi;
+            }
+        }
+    }
+    __syncthreads();
+    // back substitution, local to the segment
+    {
+        double P = 1.0, y = 0.0;
+#pragma unroll
+        for (int q = RMAX - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                y = cq[j] * y + X[j * TLp + l];
+                X[j * TLp + l] = y;
+                P *= cq[j];
+            }
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double xi = 0.0;
+        for (int q = S - 1; q >= 0; --q) {
+            Xin[q * TL + l] = xi;
+            xi = As[q * TL + l] * xi + Bs[q * TL + l];
+        }
+    }
+    __syncthreads();
+    {
+        const double xi = Xin[threadIdx.x];
+        double P = 1.0;
+#pragma unroll
+        for (int q = RMAX - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                P *= cq[j];
+                const double x = X[j * TLp + l] + P * xi;
+                if (valid) a.u[IX(i, j, k, g)] = T(x);
+            }
+        }
+    }
+}
+
 // Pivots of I − θ tridiag(a, b, c) for the tangential (set 0) and normal
 // (set 1) rows: out[set][0][j] = c'_j, out[set][1][j] = 1/β_j.
-__global__ void k_cn_coeffs(const double *at, const double *bt, const double *ct, const double *an,
-                            const double *bn, const double *cn, double theta, int ny, double *out)
+__global__ void __launch_bounds__(256) k_cn_coeffs(const double *__restrict__ at, const double *__restrict__ bt,
+                                                   const double *__restrict__ ct, const double *__restrict__ an,
+                                                   const double *__restrict__ bn, const double *__restrict__ cn,
+                                                   double theta, int ny, double *__restrict__ out)
 {
-    const int set = threadIdx.x;
-    if (set > 1) return;
-    const double *a = set ? an : at, *b = set ? bn : bt, *c = set ? cn : ct;
-    double *cp = out + set * 2 * ny, *ip = cp + ny;
-    double beta = 1.0 - theta * b[0];
-    cp[0] = -theta * c[0] / beta;
-    ip[0] = 1.0 / beta;
-    for (int j = 1; j < ny; ++j) {
-        beta = (1.0 - theta * b[j]) + theta * a[j] * cp[j - 1];
-        cp[j] = -theta * c[j] / beta;
-        ip[j] = 1.0 / beta;
+    // the rows are staged in LDS so the two sequential recurrences (one per
+    // set) wait on LDS, not on global memory; dynamic LDS 6·ny doubles
+    extern __shared__ unsigned char smem_raw[];
+    double *rows = reinterpret_cast<double *>(smem_raw); // [set][a, b, c][ny], reused for [set][c', 1/β][ny]
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        rows[0 * ny + j] = at[j];
+        rows[1 * ny + j] = bt[j];
+        rows[2 * ny + j] = ct[j];
+        rows[3 * ny + j] = an[j];
+        rows[4 * ny + j] = bn[j];
+        rows[5 * ny + j] = cn[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int set = threadIdx.x;
+        double *a = rows + 3 * set * ny, *b = a + ny, *c = b + ny;
+        double beta = 1.0 - theta * b[0];
+        double cp = -theta * c[0] / beta;
+        c[0] = cp;     // c'_0 (row c is consumed at the same index)
+        b[0] = 1.0 / beta; // 1/β_0
+        for (int j = 1; j < ny; ++j) {
+            beta = (1.0 - theta * b[j]) + theta * a[j] * cp;
+            cp = -theta * c[j] / beta;
+            c[j] = cp;
+            b[j] = 1.0 / beta;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        for (int set = 0; set < 2; ++set) {
+            out[set * 2 * ny + j] = rows[(3 * set + 2) * ny + j];      // c'_j
+            out[set * 2 * ny + ny + j] = rows[(3 * set + 1) * ny + j]; // 1/β_j
+        }
     }
 }
 

@@ -7,6 +7,27 @@
 
 namespace dns {
 
+// First level of a two-level max: block b reduces partials
+// [b·chunk, (b+1)·chunk) of each slot into mid[b·nslots + slot].
+static __global__ void __launch_bounds__(256) k_max_mid(const double *part, int nblocks, int nslots, double *mid)
+{
+    __shared__ double s[256];
+    const int chunk = (nblocks + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * chunk, b1 = min(nblocks, b0 + chunk);
+    for (int slot = 0; slot < nslots; ++slot) {
+        double m = 0;
+        for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) m = fmax(m, part[(size_t)b * nslots + slot]);
+        s[threadIdx.x] = m;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) mid[blockIdx.x * nslots + slot] = s[0];
+        __syncthreads();
+    }
+}
+
 // Block max of `cnt` partial maxima per slot; writes out[slot].
 static __global__ void k_max_final(const double *part, int nblocks, int nslots, double *out)
 {

@@ -53,7 +53,7 @@ template <typename T> struct SpecImpl : SpecBase {
     T *phys = nullptr;
     cplx<T> *twN = nullptr, *twx = nullptr, *twn = nullptr;
     FFTDesc fdn{}, fdh{};
-    int tb = 1, tbs = 1, lpb = 1;
+    int tb = 1, tbf = 1, tbs = 1, lpb = 1;
     size_t lds = 0, ldss = 0, ldsx = 0;
     double *red = nullptr, *redout = nullptr, *factors = nullptr, *pinned = nullptr;
     size_t redcap = 0;
@@ -102,9 +102,12 @@ template <typename T> struct SpecImpl : SpecBase {
         hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (maxlds <= 0) maxlds = 65536;
         const size_t sz = sizeof(cplx<T>);
-        tb = std::max(1, std::min(64, SP_EPT * SP_NTHR / g.n));
-        while (tb > 1 && (size_t)tb * (g.n + 1) * sz > (size_t)maxlds) --tb;
-        tb = std::min(tb, g.K);
+        // y/z pencil tiles: FFT sub-batches of tbf lines; a block may load
+        // tb >= tbf lines (wider rows measured no faster, so tb = tbf)
+        tbf = std::max(1, std::min(64, SP_EPT * SP_NTHR / g.n));
+        while (tbf > 1 && (size_t)tbf * (g.n + 1) * sz > (size_t)maxlds) --tbf;
+        tbf = std::min(tbf, g.K);
+        tb = tbf;
         lds = (size_t)tb * (g.n + 1) * sz;
         tbs = std::max(1, std::min({64, SP_SEPT * SP_NTHR / g.n, SP_MAXO * SP_NTHR / g.m}));
         while (tbs > 1 && (size_t)tbs * (g.n + 1) * sz > (size_t)maxlds) --tbs;
@@ -193,11 +196,11 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         timer.begin(SFAM_Y);
         hipLaunchKernelGGL((k_sp_yback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st,
-                           src ? src : uh, Q, twn, tb, g, fdn);
+                           src ? src : uh, Q, twn, tb, tbf, g, fdn);
         timer.end(SFAM_Y);
         timer.begin(SFAM_Z);
         hipLaunchKernelGGL((k_sp_zback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, 3), dim3(SP_NTHR), lds, st, Q, P,
-                           twn, tb, g, fdn);
+                           twn, tb, tbf, g, fdn);
         timer.end(SFAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
@@ -226,7 +229,7 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         timer.begin(SFAM_Z);
         hipLaunchKernelGGL((k_sp_zfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, ncomp), dim3(SP_NTHR), lds, st, P, Q,
-                           twn, tb, g, fdn);
+                           twn, tb, tbf, g, fdn);
         timer.end(SFAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
@@ -319,7 +322,7 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(xpass<XMID_R2C>(red));
         DNS_TRY(zfwd(3));
         hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, uh, twn,
-                           tb, g, fdn);
+                           tb, tbf, g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }
@@ -331,7 +334,7 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(xpass<XMID_R2C>(red, src));
         DNS_TRY(zfwd(3));
         hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, dst, twn,
-                           tb, g, fdn);
+                           tb, tbf, g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }
@@ -399,7 +402,10 @@ template <typename T> struct SpecImpl : SpecBase {
 
     int vmax_final(int nparts, double *out)
     {
-        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(SP_NTHR), 0, st, red, nparts, 1, redout);
+        // two levels: up to 256 blocks into the scratch half of red, then one
+        const int G = std::min(256, std::max(1, nparts / 1024));
+        hipLaunchKernelGGL(k_max_mid, dim3(G), dim3(SP_NTHR), 0, st, red, nparts, 1, red + redcap / 2);
+        hipLaunchKernelGGL(k_max_final, dim3(1), dim3(SP_NTHR), 0, st, red + redcap / 2, G, 1, redout);
         DNS_HIP_CHECK(hipGetLastError());
         DNS_HIP_CHECK(hipMemcpyAsync(pinned, redout, sizeof(double), hipMemcpyDeviceToHost, st));
         DNS_HIP_CHECK(hipStreamSynchronize(st));

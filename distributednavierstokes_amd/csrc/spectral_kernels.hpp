@@ -18,6 +18,14 @@
 #pragma once
 
 #include "common.hpp"
+
+// Loads kept in flight per thread by the HBM→LDS staging loops (capped by the
+// tile's elements per thread).
+// (FP64 tiles keep fewer in flight: their FFT registers already fill 256 VGPRs.)
+#ifndef SP_LOAD_MAX
+#define SP_LOAD_MAX 16
+#endif
+#define SP_LOAD_U(e) ((e) < SP_LOAD_MAX / (int)(sizeof(T) / 4) ? (e) : SP_LOAD_MAX / (int)(sizeof(T) / 4))
 #include "fft_lds.hpp"
 #include "reduce.hpp"
 
@@ -43,14 +51,14 @@ __device__ inline int sp_compact(int f, const SpecGeom &g)
 // Backward y pass: B1[i, y, kz, c] = Σ_ky uh[i, ky, kz, c] e^{+i ky y}.
 // grid (ceil(K/tb), m, 3); LDS tb·(n+1).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb,
+__global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb, int tbf,
                                                   SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, kz = blockIdx.y, c = blockIdx.z;
-    batched_for<EPT, cplx<T>>(
+    batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
             int t = idx / tb, b = idx - t * tb, i = i0 + b;
@@ -64,7 +72,7 @@ __global__ void __launch_bounds__(256) k_sp_yback(const cplx<T> *uh, cplx<T> *B1
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb, i = i0 + b;
         if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
@@ -74,14 +82,14 @@ __global__ void __launch_bounds__(256) k_sp_yback(const cplx<T> *uh, cplx<T> *B1
 // Backward z pass: P[i, y, z, c] = Σ_kz B1[i, y, kz, c] e^{+i kz z}.
 // grid (ceil(K/tb), n, 3).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, SpecGeom g,
+__global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, int tbf, SpecGeom g,
                                                   FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, y = blockIdx.y, c = blockIdx.z;
-    batched_for<EPT, cplx<T>>(
+    batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
             int t = idx / tb, b = idx - t * tb, i = i0 + b;
@@ -95,7 +103,7 @@ __global__ void __launch_bounds__(256) k_sp_zback(const cplx<T> *B1, cplx<T> *P,
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds<T, EPT, true>(buf, tb, ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb, i = i0 + b;
         if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
@@ -105,14 +113,14 @@ __global__ void __launch_bounds__(256) k_sp_zback(const cplx<T> *B1, cplx<T> *P,
 // Forward z pass: Q[i, y, kc, p] = Σ_z P[i, y, z, p] e^{−i kz z} at the
 // retained kz.  grid (ceil(K/tb), n, ncomp).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, SpecGeom g,
+__global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, int tbf, SpecGeom g,
                                                  FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, y = blockIdx.y, p = blockIdx.z;
-    batched_for<EPT, cplx<T>>(
+    batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
             int t = idx / tb, b = idx - t * tb, i = i0 + b;
@@ -123,7 +131,7 @@ __global__ void __launch_bounds__(256) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, c
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int kc = idx / tb, b = idx - kc * tb, i = i0 + b;
         if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
@@ -134,14 +142,14 @@ __global__ void __launch_bounds__(256) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, c
 // src/spectral/fft.jl:190-193): out[i, jc, kc, p] (K, m, m, ncomp).
 // grid (ceil(K/tb), m, ncomp).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, SpecGeom g,
+__global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, int tbf, SpecGeom g,
                                                  FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
     const int i0 = blockIdx.x * tb, kc = blockIdx.y, p = blockIdx.z;
-    batched_for<EPT, cplx<T>>(
+    batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
             int t = idx / tb, b = idx - t * tb, i = i0 + b;
@@ -152,7 +160,7 @@ __global__ void __launch_bounds__(256) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out,
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds<T, EPT, false>(buf, tb, ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
     const T inv = T(1.0 / ((double)n * n * n));
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int jc = idx / tb, b = idx - jc * tb, i = i0 + b;
@@ -192,7 +200,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     const size_t pstride = (size_t)K * n * n; // component stride of P
     if (MODE != XMID_R2C) {
         for (int c = 0; c < 3; ++c) {
-            batched_for<EPT, cplx<T>>(
+            batched_for<SP_LOAD_U(EPT), cplx<T>>(
                 lpb * N,
                 [&](int idx) {
                     int b = idx / N, k = idx - b * N, L = L0 + b;
@@ -256,7 +264,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     for (int p = 0; p < nout; ++p) {
         const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
         const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
-        batched_for<EPT, cplx<T>>(
+        batched_for<SP_LOAD_U(EPT), cplx<T>>(
             lpb * N,
             [&](int idx) {
                 int b = idx / N, t = idx - b * N, L = L0 + b;
@@ -307,7 +315,7 @@ template <typename T> struct StageArgs {
 };
 
 template <typename T, int EPT, int MAXO, int MODE>
-__global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, FFTDesc d)
+__global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -321,7 +329,7 @@ __global__ void __launch_bounds__(256) k_sp_ystage(StageArgs<T> s, SpecGeom g, F
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[o][c] = mk<T>(T(0), T(0));
     for (int p = 0; p < 6; ++p) {
-        batched_for<EPT, cplx<T>>(
+        batched_for<SP_LOAD_U(EPT), cplx<T>>(
             tb * n,
             [&](int idx) {
                 int t = idx / tb, i = i0 + (idx - t * tb);

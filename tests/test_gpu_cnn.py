@@ -24,15 +24,19 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_forward_matches_oracle():
-    net, p, x, _ = setup_case()
+@pytest.mark.parametrize("M", [8, 16])
+def test_forward_matches_oracle(M):
+    net, p, x, _ = setup_case(M=M)
     y = net.forward(x)
     yo, _, _ = oc.forward(oc.bf16_round(p), oc.bf16_round(x))
     assert rel(y, yo) < 2e-2
 
 
-def test_gradients_match_oracle():
-    net, p, x, t = setup_case(seed=1)
+@pytest.mark.parametrize("M", [8, 16])
+def test_gradients_match_oracle(M):
+    """M = 8 takes the voxel-tile weight-gradient kernel, M = 16 the
+    row-staged one (x-rows of a multiple of 16)."""
+    net, p, x, t = setup_case(M=M, seed=1)
     loss = net.loss_grad(x, t)
     g = net.grads()
     lo, go = oc.loss_grad(oc.bf16_round(p), oc.bf16_round(x), t.astype(np.float64))
