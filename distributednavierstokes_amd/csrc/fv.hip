@@ -90,7 +90,7 @@ template <typename T> struct FVImpl : FVBase {
     int N = 0;
     int nb_pgrad = 0;
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
-    bool tiled_momentum = false; // DNS_MOMENTUM_TILED=1 selects the LDS-tiled z-marching stencil kernel
+    int momentum_kind = 3; // 3 slab-ordered (k_momentum_slab), 0 column-ordered, 1 direct, 2 LDS-tiled z march
 
     ~FVImpl() override
     {
@@ -119,8 +119,9 @@ template <typename T> struct FVImpl : FVBase {
         g.wall = P->ywall ? 1 : 0;
         imexy = P->ydiffusion ? 1 : 0;
         {
-            const char *e = getenv("DNS_MOMENTUM_TILED");
-            tiled_momentum = e && e[0] == '1';
+            // DNS_MOMENTUM selects the stencil kernel: slab (default), col, direct, tiled
+            const char *e = getenv("DNS_MOMENTUM");
+            momentum_kind = !e ? 3 : !strcmp(e, "direct") ? 1 : !strcmp(e, "tiled") ? 2 : !strcmp(e, "col") ? 0 : 3;
         }
         visc = P->visc;
         for (int c = 0; c < 3; ++c) f[c] = P->bodyforce[c];
@@ -470,7 +471,40 @@ template <typename T> struct FVImpl : FVBase {
         a.rdz2 = (T)(1.0 / (dz * dz));
         timer.begin(FAM_MOM);
         const bool rq = Ac != 0.0;
-        if (tiled_momentum) {
+        if (momentum_kind == 3) {
+            // y-slabs whose three-plane u window (3 planes × 3 components) stays within ~2.5 MB of L2
+            const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 4);
+            const size_t rowbytes = (size_t)g.nx * 4 * 9 * sizeof(T); // one block row, 3 planes × 3 components
+            const int nslab = ceil_div(nby, std::max(1, std::min(nby, (int)((size_t)(2560 * 1024) / rowbytes))));
+            const int nbys = ceil_div(nby, nslab);
+            dim3 blk(64, 4, 1), grd((unsigned)nbx * nbys * g.nz * nslab);
+            if (write_v) {
+                if (rq) hipLaunchKernelGGL((k_momentum_slab<T, true, true>), grd, blk, 0, st, a, g, nbx, nbys);
+                else hipLaunchKernelGGL((k_momentum_slab<T, false, true>), grd, blk, 0, st, a, g, nbx, nbys);
+            } else {
+                if (rq) hipLaunchKernelGGL((k_momentum_slab<T, true, false>), grd, blk, 0, st, a, g, nbx, nbys);
+                else hipLaunchKernelGGL((k_momentum_slab<T, false, false>), grd, blk, 0, st, a, g, nbx, nbys);
+            }
+            timer.end(FAM_MOM);
+            DNS_HIP_CHECK(hipGetLastError());
+            return DNS_OK;
+        }
+        if (momentum_kind == 0) {
+            constexpr int RY = sizeof(T) == 8 ? 2 : 4;
+            const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 4 * RY);
+            dim3 blk(64, 4, 1), grd((unsigned)nbx * nby * g.nz);
+            if (write_v) {
+                if (rq) hipLaunchKernelGGL((k_momentum_col<T, true, true, RY>), grd, blk, 0, st, a, g, nbx);
+                else hipLaunchKernelGGL((k_momentum_col<T, false, true, RY>), grd, blk, 0, st, a, g, nbx);
+            } else {
+                if (rq) hipLaunchKernelGGL((k_momentum_col<T, true, false, RY>), grd, blk, 0, st, a, g, nbx);
+                else hipLaunchKernelGGL((k_momentum_col<T, false, false, RY>), grd, blk, 0, st, a, g, nbx);
+            }
+            timer.end(FAM_MOM);
+            DNS_HIP_CHECK(hipGetLastError());
+            return DNS_OK;
+        }
+        if (momentum_kind == 2) {
             const int zc = std::min(g.nz, 32);
             dim3 tgrd(ceil_div(g.nx, 64), ceil_div(g.ny, 8), ceil_div(g.nz, zc));
             if (write_v) {

@@ -108,7 +108,17 @@ template <typename T> struct MomentumArgs {
 
 // The three momentum components at one cell from its 27 stencil values
 // (uXabc: component X at offsets a, b, c in x, y, z; p = +1, m = -1).
-template <typename T, bool READ_Q, bool WRITE_V>
+// NT: the q and u′ streams (read or written once per launch) bypass L2
+// residency with non-temporal accesses, leaving L2 to the u planes that
+// neighbouring z planes re-read.
+template <typename T> __device__ inline T ld_stream(const T *p, bool nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
+template <typename T> __device__ inline void st_stream(T v, T *p, bool nt)
+{
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <typename T, bool READ_Q, bool WRITE_V, bool NT = false>
 __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, int c, int j, int jj, T ux000,
                                      T uxp00, T uxm00, T ux0p0, T ux0m0, T ux00p, T ux00m, T uxmp0, T uxm0p, T uy000,
                                      T uyp00, T uym00, T uy0p0, T uy0m0, T uy00p, T uy00m, T uypm0, T uy0mp, T uz000,
@@ -130,9 +140,9 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
                  a.epsy * ((ux0p0 - ux000) * rDyuj - (ux000 - ux0m0) * rDyum) * rDyj +
                  (ux00p - T(2) * ux000 + ux00m) * a.rdz2;
         T q = a.dt * (-conv + a.nu * diff + a.fx);
-        if (READ_Q) q += a.A * a.qx[c];
-        a.qx[c] = q;
-        if (WRITE_V) a.vx[c] = ux000 + a.B * q;
+        if (READ_Q) q += a.A * ld_stream(a.qx + c, NT);
+        st_stream(q, a.qx + c, NT);
+        if (WRITE_V) st_stream(ux000 + a.B * q, a.vx + c, NT);
     }
     // y momentum at the top y-face; the staggered volume straddles cells
     // jj and jj+1, hence the Δy-weighted transporting velocities
@@ -150,11 +160,11 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
                      a.epsy * ((uy0p0 - uy000) * rDyp - (uy000 - uy0m0) * rDyj) * rDyuj +
                      (uy00p - T(2) * uy000 + uy00m) * a.rdz2;
             T q = a.dt * (-conv + a.nu * diff + a.fy);
-            if (READ_Q) q += a.A * a.qy[c];
-            a.qy[c] = q;
-            if (WRITE_V) a.vy[c] = uy000 + a.B * q;
+            if (READ_Q) q += a.A * ld_stream(a.qy + c, NT);
+            st_stream(q, a.qy + c, NT);
+            if (WRITE_V) st_stream(uy000 + a.B * q, a.vy + c, NT);
         } else if (WRITE_V) {
-            a.vy[c] = uy000;
+            st_stream(uy000, a.vy + c, NT);
         }
     }
     // z momentum at the back z-face
@@ -169,9 +179,9 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
                  a.epsy * ((uz0p0 - uz000) * rDyuj - (uz000 - uz0m0) * rDyum) * rDyj +
                  (uz00p - T(2) * uz000 + uz00m) * a.rdz2;
         T q = a.dt * (-conv + a.nu * diff + a.fz);
-        if (READ_Q) q += a.A * a.qz[c];
-        a.qz[c] = q;
-        if (WRITE_V) a.vz[c] = uz000 + a.B * q;
+        if (READ_Q) q += a.A * ld_stream(a.qz + c, NT);
+        st_stream(q, a.qz + c, NT);
+        if (WRITE_V) st_stream(uz000 + a.B * q, a.vz + c, NT);
     }
 }
 
@@ -208,6 +218,113 @@ __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
     momentum_cell<T, READ_Q, WRITE_V>(a, g, IX(i, j, k, g), j, jj, ux000, uxp00, uxm00, ux0p0, ux0m0, ux00p, ux00m,
                                       uxmp0, uxm0p, uy000, uyp00, uym00, uy0p0, uy0m0, uy00p, uy00m, uypm0, uy0mp,
                                       uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m, uz0pm);
+}
+
+// Slab-ordered variant of k_momentum: blocks are ordered x-fastest within
+// y-slabs of `nbys` block rows, then z, then slab, and each XCD takes a
+// contiguous run of that order, so one XCD streams through consecutive z
+// planes of one slab whose three-plane u window fits its L2; q and u′ are
+// streamed non-temporally.  1D grid of nbx · nbys · nz · nslab blocks.
+template <typename T, bool READ_Q, bool WRITE_V>
+__global__ void __launch_bounds__(256) k_momentum_slab(MomentumArgs<T> a, FVGeom g, int nbx, int nbys)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    int t = lb;
+    const int bx = t % nbx;
+    t /= nbx;
+    const int byl = t % nbys;
+    t /= nbys;
+    const int k = t % g.nz, slab = t / g.nz;
+    const int i = bx * 64 + threadIdx.x;
+    const int j = (slab * nbys + byl) * 4 + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
+    const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
+    const int jj = j + 1;
+    const T m1 = T(-1);
+    const T *ux = a.ux, *uy = a.uy, *uz = a.uz;
+
+    const T ux000 = ux[IX(i, j, k, g)], uxp00 = ux[IX(ip, j, k, g)], uxm00 = ux[IX(im, j, k, g)];
+    const T ux0p0 = ldc(ux, i, jj + 1, k, g, m1), ux0m0 = ldc(ux, i, jj - 1, k, g, m1);
+    const T ux00p = ux[IX(i, j, kp, g)], ux00m = ux[IX(i, j, km, g)];
+    const T uxmp0 = ldc(ux, im, jj + 1, k, g, m1), uxm0p = ux[IX(im, j, kp, g)];
+
+    const T uy000 = ldf(uy, i, jj, k, g), uyp00 = ldf(uy, ip, jj, k, g), uym00 = ldf(uy, im, jj, k, g);
+    const T uy0p0 = ldf(uy, i, jj + 1, k, g), uy0m0 = ldf(uy, i, jj - 1, k, g);
+    const T uy00p = ldf(uy, i, jj, kp, g), uy00m = ldf(uy, i, jj, km, g);
+    const T uypm0 = ldf(uy, ip, jj - 1, k, g), uy0mp = ldf(uy, i, jj - 1, kp, g);
+
+    const T uz000 = uz[IX(i, j, k, g)], uzp00 = uz[IX(ip, j, k, g)], uzm00 = uz[IX(im, j, k, g)];
+    const T uz0p0 = ldc(uz, i, jj + 1, k, g, m1), uz0m0 = ldc(uz, i, jj - 1, k, g, m1);
+    const T uz00p = uz[IX(i, j, kp, g)], uz00m = uz[IX(i, j, km, g)];
+    const T uzp0m = uz[IX(ip, j, km, g)], uz0pm = ldc(uz, i, jj + 1, km, g, m1);
+
+    momentum_cell<T, READ_Q, WRITE_V, true>(a, g, IX(i, j, k, g), j, jj, ux000, uxp00, uxm00, ux0p0, ux0m0, ux00p,
+                                            ux00m, uxmp0, uxm0p, uy000, uyp00, uym00, uy0p0, uy0m0, uy00p, uy00m, uypm0,
+                                            uy0mp, uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m, uz0pm);
+}
+
+// Column-ordered variant: each thread computes RY consecutive y rows, so
+// the y-neighbour values of one row are the centre values of the next and
+// each column of the stencil is loaded once; the blocks (64 × 4·RY cells)
+// are ordered z-fastest within an (x, y) tile column and each XCD takes a
+// contiguous run of that order, so the blocks resident on one XCD are
+// consecutive z planes of one tile column and their z-neighbour planes hit
+// in its L2.  1D grid of nbx · nby · nz blocks of (64, 4) threads.
+template <typename T, bool READ_Q, bool WRITE_V, int RY>
+__global__ void __launch_bounds__(256) k_momentum_col(MomentumArgs<T> a, FVGeom g, int nbx)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int k = lb % g.nz, tcol = lb / g.nz, bx = tcol % nbx, by = tcol / nbx;
+    const int i = bx * 64 + threadIdx.x;
+    const int j0 = (by * 4 + threadIdx.y) * RY;
+    if (i >= g.nx || j0 >= g.ny) return;
+    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
+    const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
+    const T m1 = T(-1);
+    const T *ux = a.ux, *uy = a.uy, *uz = a.uz;
+    // values at interior row r (r = -1 and r = ny are ghost rows); rows past ny are never used
+    auto cx = [&](const T *f, int ii, int r, int kk) -> T { return r <= g.ny ? ldc(f, ii, r + 1, kk, g, m1) : T(0); };
+    auto fy = [&](int ii, int r, int kk) -> T { return r <= g.ny ? ldf(uy, ii, r + 1, kk, g) : T(0); };
+    T uxc[RY + 2], uxp[RY], uxm[RY + 1], uxkp[RY], uxkm[RY], uxmkp[RY];
+    T uyc[RY + 2], uyp[RY + 1], uym[RY], uykp[RY + 1], uykm[RY];
+    T uzc[RY + 2], uzp[RY], uzm[RY], uzkp[RY], uzkm[RY + 1], uzpkm[RY];
+#pragma unroll
+    for (int r = 0; r < RY + 2; ++r) {
+        uxc[r] = cx(ux, i, j0 - 1 + r, k);
+        uyc[r] = fy(i, j0 - 1 + r, k);
+        uzc[r] = cx(uz, i, j0 - 1 + r, k);
+    }
+#pragma unroll
+    for (int r = 0; r < RY + 1; ++r) {
+        uxm[r] = cx(ux, im, j0 + r, k);
+        uyp[r] = fy(ip, j0 - 1 + r, k);
+        uykp[r] = fy(i, j0 - 1 + r, kp);
+        uzkm[r] = cx(uz, i, j0 + r, km);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        uxp[r] = cx(ux, ip, j0 + r, k);
+        uxkp[r] = cx(ux, i, j0 + r, kp);
+        uxkm[r] = cx(ux, i, j0 + r, km);
+        uxmkp[r] = cx(ux, im, j0 + r, kp);
+        uym[r] = fy(im, j0 + r, k);
+        uykm[r] = fy(i, j0 + r, km);
+        uzp[r] = cx(uz, ip, j0 + r, k);
+        uzm[r] = cx(uz, im, j0 + r, k);
+        uzkp[r] = cx(uz, i, j0 + r, kp);
+        uzpkm[r] = cx(uz, ip, j0 + r, km);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int j = j0 + r;
+        if (j >= g.ny) break;
+        momentum_cell<T, READ_Q, WRITE_V>(a, g, IX(i, j, k, g), j, j + 1, uxc[r + 1], uxp[r], uxm[r], uxc[r + 2],
+                                          uxc[r], uxkp[r], uxkm[r], uxm[r + 1], uxmkp[r], uyc[r + 1], uyp[r + 1],
+                                          uym[r], uyc[r + 2], uyc[r], uykp[r + 1], uykm[r], uyp[r], uykp[r],
+                                          uzc[r + 1], uzp[r], uzm[r], uzc[r + 2], uzc[r], uzkp[r], uzkm[r], uzpkm[r],
+                                          uzkm[r + 1]);
+    }
 }
 
 // LDS-tiled variant: a block owns a 64 × 8 (x, y) tile and marches over
