@@ -58,7 +58,8 @@ static const double RK_B[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 static const double RK_ALPHA[3] = {1.0 / 3.0, 5.0 / 12.0, 1.0 / 4.0};
 
 constexpr int X_EPT = 8, ZY_EPT = 8, NTHR = 256;
-constexpr int THOMAS_RMAX = 16, CN_RMAX = 32; // rows per segment of the LDS tridiagonal kernels
+constexpr int THOMAS_RMAX = 24, CN_RMAX = 32; // rows per thread segment of the tridiagonal kernels
+// (a half-size variant of each kernel serves segments of at most RMAX / 2 rows)
 constexpr int CH_NCHUNK = 16;
 constexpr int RED_BLOCKS = 1024;
 
@@ -71,9 +72,9 @@ template <typename T> struct FVImpl : FVBase {
     T *u[3] = {}, *v[3] = {}, *q[3] = {}, *p = nullptr, *delta = nullptr;
     cplx<T> *A = nullptr;
     double2 *Dth = nullptr;
-    int thomas_tl = 0; // lines per block of the LDS Thomas kernel (0: sequential kernel)
+    int thomas_tl = 0, thomas_nthr = 256; // lines and threads per block of the LDS Thomas kernel (0: sequential)
     size_t ldst = 0;
-    int cn_tl = 0; // x-lines per block of the LDS CN kernel (0: sequential kernel)
+    int cn_tl = 0, cn_nthr = 256; // x-lines and threads per block of the LDS CN kernel (0: sequential)
     size_t ldscn = 0;
     double *Dcn = nullptr;
     T *dy_t = nullptr, *rdy_t = nullptr, *rdyu_t = nullptr;
@@ -292,20 +293,33 @@ template <typename T> struct FVImpl : FVBase {
             ldsy = (size_t)tby * (ny + 1) * sizeof(cplx<T>);
         }
         if (g.wall) {
-            // LDS-resident segmented Thomas: TL lines per block within ~80 KB so
-            // that two blocks share a CU; DNS_THOMAS=sequential selects the
-            // one-thread-per-line kernel with an HBM d' scratch.
-            const size_t cap = std::min<size_t>((size_t)maxlds, 80 * 1024);
-            const char *env = getenv("DNS_THOMAS");
-            thomas_tl = 16;
-            auto need = [&](int tl) {
-                return (size_t)ny * (tl + 1) * sizeof(double2) + (size_t)NTHR * 40 + (size_t)2 * ny * sizeof(double);
+            // segmented tridiagonal kernels: a block of 256 threads owns TL lines
+            // (128-byte rows where the segment length allows), each thread one
+            // segment of at most RMAX rows in registers; DNS_THOMAS / DNS_CN =
+            // sequential select the one-thread-per-line kernels with an HBM d'
+            // scratch.
+            auto pick_tile = [&](int elem_bytes, int rmax, auto need, int &tl, int &nthr, size_t &lds) {
+                // widest 128-byte rows first; 256 threads; segments within rmax rows
+                for (int t = std::max(1, 128 / elem_bytes); t >= 1; t /= 2) {
+                    const int nt = 256;
+                    const int rows = (ny + nt / t - 1) / (nt / t);
+                    if (rows <= rmax && need(t, nt) <= (size_t)maxlds) {
+                        tl = t;
+                        nthr = nt;
+                        lds = need(t, nt);
+                        return;
+                    }
+                }
+                tl = 0;
             };
-            auto rows = [&](int tl) { return (ny + NTHR / tl - 1) / (NTHR / tl); };
-            while (thomas_tl > 1 && (need(thomas_tl) > cap || rows(thomas_tl) > THOMAS_RMAX)) thomas_tl /= 2;
-            if (need(thomas_tl) > cap || rows(thomas_tl) > THOMAS_RMAX || (env && !strcmp(env, "sequential")))
-                thomas_tl = 0;
-            ldst = thomas_tl ? need(thomas_tl) : 0;
+            const char *env = getenv("DNS_THOMAS");
+            pick_tile((int)sizeof(cplx<T>), THOMAS_RMAX,
+                      [&](int tl, int nt) {
+                          (void)tl;
+                          return (size_t)nt * 40 + (size_t)2 * ny * sizeof(double);
+                      },
+                      thomas_tl, thomas_nthr, ldst);
+            if (env && !strcmp(env, "sequential")) thomas_tl = 0;
             if (!thomas_tl) DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
             if (imexy && (size_t)6 * ny * sizeof(double) > (size_t)maxlds) {
                 set_error("unsupported grid: implicit y-diffusion coefficients need %zu bytes of LDS",
@@ -313,18 +327,14 @@ template <typename T> struct FVImpl : FVBase {
                 return DNS_EUNSUPPORTED;
             }
             if (imexy) {
-                // LDS segmented CN: up to 16 x-lines per block within ~80 KB;
-                // DNS_CN=sequential selects the one-thread-per-column kernel.
                 const char *envc = getenv("DNS_CN");
-                cn_tl = 16;
-                auto needc = [&](int tl) {
-                    return (size_t)ny * (tl + 1) * sizeof(double) + (size_t)3 * NTHR * sizeof(double) +
-                           (size_t)6 * ny * sizeof(double);
-                };
-                auto rowsc = [&](int tl) { return (ny + NTHR / tl - 1) / (NTHR / tl); };
-                while (cn_tl > 1 && (needc(cn_tl) > cap || rowsc(cn_tl) > CN_RMAX)) cn_tl /= 2;
-                if (needc(cn_tl) > cap || rowsc(cn_tl) > CN_RMAX || (envc && !strcmp(envc, "sequential"))) cn_tl = 0;
-                ldscn = cn_tl ? needc(cn_tl) : 0;
+                pick_tile((int)sizeof(T), CN_RMAX,
+                          [&](int tl, int nt) {
+                              (void)tl;
+                              return (size_t)3 * nt * sizeof(double) + (size_t)6 * ny * sizeof(double);
+                          },
+                          cn_tl, cn_nthr, ldscn);
+                if (envc && !strcmp(envc, "sequential")) cn_tl = 0;
                 if (!cn_tl) DNS_TRY(dalloc(&Dcn, (size_t)N));
             }
         }
@@ -561,9 +571,14 @@ template <typename T> struct FVImpl : FVBase {
         if (g.wall) {
             double scale = 1.0 / ((double)g.nx * g.nz);
             if (thomas_tl) {
-                hipLaunchKernelGGL((k_thomas_tile<T, THOMAS_RMAX>), dim3(ceil_div(g.nxc, thomas_tl), g.nz), dim3(NTHR),
-                                   ldst, st, A,
-                                   iptab, ay, cy, lam1, lam3, scale, thomas_tl, g);
+                const dim3 grd(ceil_div(g.nxc, thomas_tl), g.nz), blk(thomas_nthr);
+                const int rows = ceil_div(g.ny, thomas_nthr / thomas_tl);
+                if (rows <= THOMAS_RMAX / 2)
+                    hipLaunchKernelGGL((k_thomas_tile<T, THOMAS_RMAX / 2>), grd, blk, ldst, st, A, iptab, ay, cy, lam1,
+                                       lam3, scale, thomas_tl, g);
+                else
+                    hipLaunchKernelGGL((k_thomas_tile<T, THOMAS_RMAX>), grd, blk, ldst, st, A, iptab, ay, cy, lam1, lam3,
+                                       scale, thomas_tl, g);
             } else {
                 int lines = g.ldc * g.nz;
                 hipLaunchKernelGGL((k_thomas_y<T>), dim3(ceil_div(lines, NTHR)), dim3(NTHR), 0, st, A, Dth, iptab, ay,
@@ -644,9 +659,16 @@ template <typename T> struct FVImpl : FVBase {
             a.rdz = 1.0 / dz;
             if (cn_tl) {
                 dim3 grd(ceil_div(g.nx, cn_tl), g.nz);
-                if (c == 0) hipLaunchKernelGGL((k_cn_tile<T, 0, CN_RMAX>), grd, dim3(NTHR), ldscn, st, a, g, cn_tl);
-                else if (c == 1) hipLaunchKernelGGL((k_cn_tile<T, 1, CN_RMAX>), grd, dim3(NTHR), ldscn, st, a, g, cn_tl);
-                else hipLaunchKernelGGL((k_cn_tile<T, 2, CN_RMAX>), grd, dim3(NTHR), ldscn, st, a, g, cn_tl);
+                const dim3 blk(cn_nthr);
+                if (ceil_div(g.ny, cn_nthr / cn_tl) <= CN_RMAX / 2) {
+                    if (c == 0) hipLaunchKernelGGL((k_cn_tile<T, 0, CN_RMAX / 2>), grd, blk, ldscn, st, a, g, cn_tl);
+                    else if (c == 1) hipLaunchKernelGGL((k_cn_tile<T, 1, CN_RMAX / 2>), grd, blk, ldscn, st, a, g, cn_tl);
+                    else hipLaunchKernelGGL((k_cn_tile<T, 2, CN_RMAX / 2>), grd, blk, ldscn, st, a, g, cn_tl);
+                } else {
+                    if (c == 0) hipLaunchKernelGGL((k_cn_tile<T, 0, CN_RMAX>), grd, blk, ldscn, st, a, g, cn_tl);
+                    else if (c == 1) hipLaunchKernelGGL((k_cn_tile<T, 1, CN_RMAX>), grd, blk, ldscn, st, a, g, cn_tl);
+                    else hipLaunchKernelGGL((k_cn_tile<T, 2, CN_RMAX>), grd, blk, ldscn, st, a, g, cn_tl);
+                }
             } else {
                 dim3 blk(64), grd(ceil_div(g.nx, 64), g.nz);
                 if (c == 0) hipLaunchKernelGGL((k_cn<T, 0>), grd, blk, 0, st, a, g);

@@ -673,17 +673,18 @@ __global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *__restrict__ A, doubl
     }
 }
 
-// LDS-resident variant of the same solve: a block owns TL consecutive kx
-// lines of one kz plane (all ny rows staged in LDS as FP64), and each line's
-// forward sweep and back substitution -- both first-order affine recurrences
-// x_j = α_j x_{j∓1} + β_j -- run as S = blockDim/TL segments in parallel:
-// every segment solves its rows from a zero inflow while accumulating Π α,
-// one thread per line chains the S segment maps, then every segment adds
-// (Π α) x_in.  A thread keeps its rows' pivots in registers (at most RMAX
-// rows per segment) and the y coefficients sit in LDS, so no global load
-// waits inside a recurrence.  HBM traffic: r read and x written once, the
-// pivot table read once.  Same pivots and mean-mode rule as k_thomas_y.
-// grid (ceil(nxc / TL), nz); LDS ny (TL + 1) double2 + 2 ny + 5 blockDim doubles.
+// Segmented variant of the same solve: a block owns TL consecutive kx lines
+// of one kz plane, and each line's rows are split into S = blockDim/TL
+// segments of at most RMAX rows held in registers by one thread (the data,
+// then d', then x, all FP64, with the rows' pivots).  The forward sweep and
+// the back substitution -- both first-order affine recurrences
+// x_j = α_j x_{j∓1} + β_j -- run segment-parallel: every segment solves its
+// rows from a zero inflow while accumulating Π α, one thread per line chains
+// the S segment maps through LDS, then every segment adds (Π α) x_in.  HBM
+// traffic: r read and x written once, the pivot table read once; LDS holds
+// only the y coefficients and the segment maps, so several blocks share a
+// CU.  Same pivots and mean-mode rule as k_thomas_y.
+// grid (ceil(nxc / TL), nz); LDS 2 ny + 5 blockDim doubles.
 template <typename T, int RMAX>
 __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, const double *__restrict__ ip,
                                                      const double *__restrict__ ay, const double *__restrict__ cy,
@@ -691,35 +692,35 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
                                                      double scale, int TL, FVGeom g)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int ny = g.ny, TLp = TL + 1, S = blockDim.x / TL;
-    double2 *X = reinterpret_cast<double2 *>(smem_raw);
-    double2 *Bs = X + (size_t)ny * TLp; // [S][TL] segment offsets
-    double2 *Xin = Bs + blockDim.x;     // [S][TL] segment inflows
+    const int ny = g.ny, S = blockDim.x / TL;
+    double2 *Bs = reinterpret_cast<double2 *>(smem_raw); // [S][TL] segment offsets
+    double2 *Xin = Bs + blockDim.x;                      // [S][TL] segment inflows
     double *As = reinterpret_cast<double *>(Xin + blockDim.x);
     double *sa = As + blockDim.x, *sc = sa + ny;
     const int kx0 = blockIdx.x * TL, kz = blockIdx.y;
     const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
-    const size_t abase = (size_t)g.ldc * (size_t)ny * kz, ibase = (size_t)g.nxc * (size_t)ny * kzs;
     const int l = threadIdx.x % TL, sg = threadIdx.x / TL, kx = kx0 + l;
     const bool valid = kx < g.nxc;
     const int R = (ny + S - 1) / S, j0 = min(ny, sg * R), j1 = min(ny, j0 + R);
+    cplx<T> *Al = A + (size_t)g.ldc * (size_t)ny * kz + (valid ? kx : 0);
+    const double *ipl = ip + (size_t)g.nxc * (size_t)ny * kzs + (valid ? kx : 0);
+    double2 x[RMAX];
     double w[RMAX];
 #pragma unroll
-    for (int q = 0; q < RMAX; ++q) w[q] = (valid && j0 + q < j1) ? ip[ibase + kx + (size_t)g.nxc * (j0 + q)] : 0.0;
+    for (int q = 0; q < RMAX; ++q) {
+        const int j = j0 + q;
+        x[q] = make_double2(0.0, 0.0);
+        w[q] = 0.0;
+        if (valid && j < j1) {
+            const cplx<T> a = Al[(size_t)g.ldc * j];
+            x[q] = make_double2((double)a.x * scale, (double)a.y * scale);
+            w[q] = ipl[(size_t)g.nxc * j];
+        }
+    }
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         sa[j] = ay[j];
         sc[j] = cy[j];
     }
-    batched_for<FV_LOAD_U(8), cplx<T>>(
-        ny * TL,
-        [&](int idx) {
-            const int j = idx / TL, kx2 = kx0 + (idx - j * TL);
-            return kx2 < g.nxc ? A[abase + kx2 + (size_t)g.ldc * j] : mk<T>(T(0), T(0));
-        },
-        [&](int idx, cplx<T> a) {
-            const int j = idx / TL, l2 = idx - j * TL;
-            X[j * TLp + l2] = make_double2((double)a.x * scale, (double)a.y * scale);
-        });
     const bool mean = valid && (lam1[kx] + lam3[kz]) == 0.0;
     __syncthreads();
     // forward sweep d'_j = −a_j ip_j d'_{j−1} + ip_j r_j (row 0: no coupling; 0 for the mean mode)
@@ -732,9 +733,8 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
             if (j < j1) {
                 const double al = j == 0 ? 0.0 : -sa[j] * w[q];
                 const double be = (j == 0 && mean) ? 0.0 : w[q];
-                const double2 r = X[j * TLp + l];
-                y = make_double2(al * y.x + be * r.x, al * y.y + be * r.y);
-                X[j * TLp + l] = y;
+                y = make_double2(al * y.x + be * x[q].x, al * y.y + be * x[q].y);
+                x[q] = y;
                 P *= al;
             }
         }
@@ -760,12 +760,11 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
             const int j = j0 + q;
             if (j < j1) {
                 P *= j == 0 ? 0.0 : -sa[j] * w[q];
-                const double2 y = X[j * TLp + l];
-                X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
+                x[q] = make_double2(x[q].x + P * xi.x, x[q].y + P * xi.y);
             }
         }
     }
-    __syncthreads();
+    __syncthreads(); // the segment maps of the forward chain are read; reuse As/Bs
     // back substitution x_j = d'_j − c'_j x_{j+1}, c'_j = c_j ip_j (0 on the last row and row 0 of the mean mode)
     {
         double P = 1.0;
@@ -775,9 +774,8 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
             const int j = j0 + q;
             if (j < j1) {
                 const double al = (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
-                const double2 d = X[j * TLp + l];
-                y = make_double2(al * y.x + d.x, al * y.y + d.y);
-                X[j * TLp + l] = y;
+                y = make_double2(al * y.x + x[q].x, al * y.y + x[q].y);
+                x[q] = y;
                 P *= al;
             }
         }
@@ -795,25 +793,14 @@ __global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, co
         }
     }
     __syncthreads();
-    {
-        const double2 xi = Xin[threadIdx.x];
-        double P = 1.0;
+    const double2 xi = Xin[threadIdx.x];
+    double P = 1.0;
 #pragma unroll
-        for (int q = RMAX - 1; q >= 0; --q) {
-            const int j = j0 + q;
-            if (j < j1) {
-                P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
-                const double2 y = X[j * TLp + l];
-                X[j * TLp + l] = make_double2(y.x + P * xi.x, y.y + P * xi.y);
-            }
-        }
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < ny * TL; idx += blockDim.x) {
-        const int j = idx / TL, l2 = idx - j * TL, kx2 = kx0 + l2;
-        if (kx2 < g.nxc) {
-            const double2 x = X[j * TLp + l2];
-            A[abase + kx2 + (size_t)g.ldc * j] = mk<T>(T(x.x), T(x.y));
+    for (int q = RMAX - 1; q >= 0; --q) {
+        const int j = j0 + q;
+        if (j < j1) {
+            P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
+            if (valid) Al[(size_t)g.ldc * j] = mk<T>(T(x[q].x + P * xi.x), T(x[q].y + P * xi.y));
         }
     }
 }
@@ -940,24 +927,24 @@ __global__ void __launch_bounds__(256) k_cn(CNArgs<T> a, FVGeom g)
     }
 }
 
-// LDS-resident variant of k_cn: a block owns TL consecutive x-lines of one
-// z plane; each line's rows are split into S = blockDim/TL segments (at most
-// RMAX rows each) that run the forward sweep (d'_j = θ a_j ip_j d'_{j−1} +
-// ip_j rhs_j) and the back substitution (x_j = d'_j − c'_j x_{j+1}) as
-// segment-parallel affine recurrences (local solve from zero inflow with
-// Π α, chained per line by one thread, then corrected), with d' held in LDS
-// as FP64 and the row coefficients staged in LDS.  Each thread loads CH rows
-// of u, p, q ahead of its recurrence.  u is updated in place: a line's rows
-// are read only by this block, and all reads precede the first barrier.
-// grid (ceil(nx / TL), nz); LDS ny (TL + 1) + 3 blockDim + 6 ny doubles.
+// Segmented variant of k_cn: a block owns TL consecutive x-lines of one z
+// plane; each line's rows are split into S = blockDim/TL segments of at most
+// RMAX rows held in registers by one thread (rhs, then d', then x, FP64).
+// The forward sweep (d'_j = θ a_j ip_j d'_{j−1} + ip_j rhs_j) and the back
+// substitution (x_j = d'_j − c'_j x_{j+1}) run as segment-parallel affine
+// recurrences (local solve from zero inflow with Π α, chained per line by
+// one thread through LDS, then corrected); the row coefficients are staged
+// in LDS.  Each thread loads CH rows of u, p, q ahead of use.  u is updated
+// in place: a line's rows are read only by this block, and all reads
+// precede the first barrier.
+// grid (ceil(nx / TL), nz); LDS 6 ny + 3 blockDim doubles.
 template <typename T, int COMP, int RMAX>
 __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
 {
     extern __shared__ unsigned char smem_raw[];
     constexpr int CH = 8;
-    const int ny = g.ny, TLp = TL + 1, S = blockDim.x / TL;
-    double *X = reinterpret_cast<double *>(smem_raw);
-    double *As = X + (size_t)ny * TLp, *Bs = As + blockDim.x, *Xin = Bs + blockDim.x;
+    const int ny = g.ny, S = blockDim.x / TL;
+    double *As = reinterpret_cast<double *>(smem_raw), *Bs = As + blockDim.x, *Xin = Bs + blockDim.x;
     double *ca = Xin + blockDim.x, *cb = ca + ny, *cc = cb + ny, *cf = cc + ny, *ci = cf + ny, *cq = ci + ny;
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         ca[j] = a.a[j];
@@ -976,7 +963,7 @@ __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
     const int R = (ny + S - 1) / S, j0 = min(ny, sg * R), j1 = min(ny, j0 + R);
     auto uat = [&](int j) -> double { return (j >= 0 && j < ny) ? (double)a.u[IX(ii, j, k, g)] : 0.0; };
     // right-hand sides of the segment's rows: rhs_j = u_j + B q_j − αΔt ∂p + θ (Dy u)_j
-    double rhs[RMAX];
+    double x[RMAX];
     {
         double um = uat(j0 - 1);
 #pragma unroll
@@ -997,17 +984,17 @@ __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
                     vr[q] = a.B * (double)a.q[c] - a.adt * dpdc;
                 }
             }
-            __syncthreads(); // first pass: the coefficient arrays are staged before use
+            if (c0 == 0) __syncthreads(); // the coefficient rows are staged before first use
 #pragma unroll
             for (int q = 0; q < CH; ++q) {
                 const int j = j0 + c0 + q;
-                rhs[c0 + q] = 0.0;
+                x[c0 + q] = 0.0;
                 if (j < j1) {
                     const double u0 = ur[q], up = ur[q + 1];
                     double lap = cb[j] * u0;
                     if (j > 0) lap += ca[j] * um;
                     if (j < ny - 1) lap += cc[j] * up;
-                    rhs[c0 + q] = u0 + vr[q] + a.theta * lap;
+                    x[c0 + q] = u0 + vr[q] + a.theta * lap;
                     um = u0;
                 }
             }
@@ -1020,8 +1007,8 @@ __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
         for (int q = 0; q < RMAX; ++q) {
             const int j = j0 + q;
             if (j < j1) {
-                y = cf[j] * y + ci[j] * rhs[q];
-                X[j * TLp + l] = y;
+                y = cf[j] * y + ci[j] * x[q];
+                x[q] = y;
                 P *= cf[j];
             }
         }
@@ -1045,11 +1032,11 @@ __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
             const int j = j0 + q;
             if (j < j1) {
                 P *= cf[j];
-                X[j * TLp + l] += P * xi;
+                x[q] += P * xi;
             }
         }
     }
-    __syncthreads();
+    __syncthreads(); // the forward chain's maps are consumed; reuse As/Bs
     // back substitution, local to the segment
     {
         double P = 1.0, y = 0.0;
@@ -1057,8 +1044,8 @@ __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
         for (int q = RMAX - 1; q >= 0; --q) {
             const int j = j0 + q;
             if (j < j1) {
-                y = cq[j] * y + X[j * TLp + l];
-                X[j * TLp + l] = y;
+                y = cq[j] * y + x[q];
+                x[q] = y;
                 P *= cq[j];
             }
         }
@@ -1074,17 +1061,14 @@ __global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
         }
     }
     __syncthreads();
-    {
-        const double xi = Xin[threadIdx.x];
-        double P = 1.0;
+    const double xi = Xin[threadIdx.x];
+    double P = 1.0;
 #pragma unroll
-        for (int q = RMAX - 1; q >= 0; --q) {
-            const int j = j0 + q;
-            if (j < j1) {
-                P *= cq[j];
-                const double x = X[j * TLp + l] + P * xi;
-                if (valid) a.u[IX(i, j, k, g)] = T(x);
-            }
+    for (int q = RMAX - 1; q >= 0; --q) {
+        const int j = j0 + q;
+        if (j < j1) {
+            P *= cq[j];
+            if (valid) a.u[IX(i, j, k, g)] = T(x[q] + P * xi);
         }
     }
 }
