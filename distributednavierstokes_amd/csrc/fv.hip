@@ -84,6 +84,7 @@ template <typename T> struct FVImpl : FVBase {
     FFTDesc fdx{}, fdz{}, fdy{};
     cplx<T> *twN = nullptr, *twx = nullptr, *twz = nullptr, *twy = nullptr;
     int lpb = 1, tbz = 1, tby = 1, tbzf = 1, tbyf = 1; // tile lines and FFT sub-batch lines
+    int nthrz = 256, nthry = 256;                      // threads of the z / periodic-y FFT blocks
     size_t ldsx = 0, ldsz = 0, ldsy = 0;
     double *red = nullptr, *redout = nullptr, *upart = nullptr;
     size_t redcap = 0;
@@ -273,43 +274,44 @@ template <typename T> struct FVImpl : FVBase {
         if (maxlds <= 0) maxlds = 65536;
         lpb = std::max(1, std::min(64, X_EPT * NTHR / std::max(1, Nh)));
         ldsx = (size_t)lpb * Nh * sizeof(cplx<T>);
-        auto pick_tb = [&](int n) {
-            int tb = std::max(1, std::min(64, ZY_EPT * NTHR / n));
-            while (tb > 1 && (size_t)tb * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tb;
-            return std::min(tb, g.nxc);
+        // z / periodic-y FFT tiles: a block loads tb lines and transforms them
+        // in sub-batches of tbf lines.  In FP32, tiles narrower than 128-byte
+        // rows move to 512 threads and at least 16 lines (measured faster); in
+        // FP64 the 256-thread one-sub-batch tile measured best.
+        auto pick_fft = [&](int n, int &tb, int &tbf, int &nthr) {
+            const int rowlines = sizeof(T) == 4 ? (int)(128 / sizeof(cplx<T>)) : 1;
+            nthr = (ZY_EPT * NTHR / n) < rowlines ? 2 * NTHR : NTHR;
+            tbf = std::max(1, std::min(64, ZY_EPT * nthr / n));
+            while (tbf > 1 && (size_t)tbf * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tbf;
+            tbf = std::min(tbf, g.nxc);
+            tb = std::min(g.nxc, std::max(tbf, rowlines));
+            while (tb > tbf && (size_t)tb * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tb;
         };
-        // a block may load more lines than one FFT sub-batch (tb >= tbf); wider
-        // row segments measured no faster here, so the tile is one sub-batch
-        auto tile_lines = [&](int tbf, int n) {
-            (void)n;
-            return tbf;
-        };
-        tbzf = pick_tb(nz);
-        tbz = tile_lines(tbzf, nz);
+        pick_fft(nz, tbz, tbzf, nthrz);
         ldsz = (size_t)tbz * (nz + 1) * sizeof(cplx<T>);
         if (!g.wall) {
-            tbyf = pick_tb(ny);
-            tby = tile_lines(tbyf, ny);
+            pick_fft(ny, tby, tbyf, nthry);
             ldsy = (size_t)tby * (ny + 1) * sizeof(cplx<T>);
         }
         if (g.wall) {
-            // segmented tridiagonal kernels: a block of 256 threads owns TL lines
-            // (128-byte rows where the segment length allows), each thread one
-            // segment of at most RMAX rows in registers; DNS_THOMAS / DNS_CN =
+            // segmented tridiagonal kernels: a block of 256 or 512 threads owns TL
+            // lines (128-byte rows where the segment length allows), each thread
+            // one segment of at most RMAX rows in registers; DNS_THOMAS / DNS_CN =
             // sequential select the one-thread-per-line kernels with an HBM d'
             // scratch.
             auto pick_tile = [&](int elem_bytes, int rmax, auto need, int &tl, int &nthr, size_t &lds) {
-                // widest 128-byte rows first; 256 threads; segments within rmax rows
-                for (int t = std::max(1, 128 / elem_bytes); t >= 1; t /= 2) {
-                    const int nt = 256;
-                    const int rows = (ny + nt / t - 1) / (nt / t);
-                    if (rows <= rmax && need(t, nt) <= (size_t)maxlds) {
-                        tl = t;
-                        nthr = nt;
-                        lds = need(t, nt);
-                        return;
+                // 256 threads first (measured faster than 512), then the widest rows up to
+                // 128 bytes whose segments fit rmax rows
+                for (int nt : {256, 512})
+                    for (int t = std::max(1, 128 / elem_bytes); t >= 1; t /= 2) {
+                        const int rows = (ny + nt / t - 1) / (nt / t);
+                        if (rows <= rmax && need(t, nt) <= (size_t)maxlds) {
+                            tl = t;
+                            nthr = nt;
+                            lds = need(t, nt);
+                            return;
+                        }
                     }
-                }
                 tl = 0;
             };
             const char *env = getenv("DNS_THOMAS");
@@ -563,7 +565,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.end(FAM_X);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
-        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
+        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
                            twz, tbz, tbzf, g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
@@ -586,13 +588,13 @@ template <typename T> struct FVImpl : FVBase {
             }
         } else {
             double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
-            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,
+            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
                                st, A, twy, lam1, lam2, lam3, scale, tby, tbyf, g, fdy);
         }
         timer.end(FAM_Y);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
-        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
+        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
                            twz, tbz, tbzf, g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
@@ -816,9 +818,9 @@ template <typename T> struct FVImpl : FVBase {
             xa.lpb = lpb;
             hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(g.ny * g.nz, lpb)), dim3(NTHR), ldsx, st, xa, g,
                                fdx);
-            hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(NTHR), ldsz, st, A,
+            hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
                                twz, tbz, tbzf, g, fdz);
-            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(NTHR), ldsy,
+            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
                                st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, g, fdy);
             hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR), sizeof(double) * nbins, st, A, g, nbins,
                                1.0 / N2, red);

@@ -522,7 +522,7 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
 // consecutive kx at fixed j (coalesced tb·16-byte row segments).
 // ------------------------------------------------------------------------
 template <typename T, int EPT, bool INV>
-__global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, int tbf, FVGeom g, FFTDesc d)
+__global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, int tbf, FVGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -555,7 +555,7 @@ __global__ void __launch_bounds__(256) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
 // forward y transform is applied (spectrum sampling).
 // ------------------------------------------------------------------------
 template <typename T, int EPT, bool DIVIDE>
-__global__ void __launch_bounds__(256) k_fourier_y(cplx<T> *A, const cplx<T> *tw, const double *lam1,
+__global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw, const double *lam1,
                                                    const double *lam2, const double *lam3, double scale, int tb,
                                                    int tbf, FVGeom g, FFTDesc d)
 {
@@ -686,7 +686,7 @@ __global__ void __launch_bounds__(256) k_thomas_y(cplx<T> *__restrict__ A, doubl
 // CU.  Same pivots and mean-mode rule as k_thomas_y.
 // grid (ceil(nxc / TL), nz); LDS 2 ny + 5 blockDim doubles.
 template <typename T, int RMAX>
-__global__ void __launch_bounds__(256) k_thomas_tile(cplx<T> *__restrict__ A, const double *__restrict__ ip,
+__global__ void __launch_bounds__(512) k_thomas_tile(cplx<T> *__restrict__ A, const double *__restrict__ ip,
                                                      const double *__restrict__ ay, const double *__restrict__ cy,
                                                      const double *__restrict__ lam1, const double *__restrict__ lam3,
                                                      double scale, int TL, FVGeom g)
@@ -939,7 +939,7 @@ __global__ void __launch_bounds__(256) k_cn(CNArgs<T> a, FVGeom g)
 // precede the first barrier.
 // grid (ceil(nx / TL), nz); LDS 6 ny + 3 blockDim doubles.
 template <typename T, int COMP, int RMAX>
-__global__ void __launch_bounds__(256) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
+__global__ void __launch_bounds__(512) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
 {
     extern __shared__ unsigned char smem_raw[];
     constexpr int CH = 8;
