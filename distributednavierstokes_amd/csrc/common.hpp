@@ -97,6 +97,39 @@ __device__ inline dd dd_add1(dd a, double b)
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
+// with its own L2.  This bijection of linear block ids gives every XCD one
+// contiguous range of the domain, so the y/z stencil neighbours a block
+// reads were mostly fetched into the same L2 by the blocks before it.
+__device__ inline int xcd_block(int bid, int nb)
+{
+    const int X = 8;
+    const int q = nb / X, r = nb - q * X;
+    const int xcd = bid % X, slot = bid / X;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// 3D block coordinates after the XCD remap of a (gx, gy, gz) grid.
+__device__ inline void xcd_block3(int &bx, int &by, int &bz)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nb = gx * gy * gridDim.z;
+    const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+    bx = lb % gx;
+    by = (lb / gx) % gy;
+    bz = lb / (gx * gy);
+}
+// 2D block coordinates after the XCD remap of a (gx, gy) grid: blocks that
+// are consecutive in (x fastest) order land on the same XCD, so tiles that
+// share 128-byte lines or neighbour rows share its L2.
+__device__ inline void xcd_block2(int &bx, int &by)
+{
+    const int gx = gridDim.x;
+    const int lb = xcd_block(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+    bx = lb % gx;
+    by = lb / gx;
+}
+
 // Block-strided loop over [0, total) that keeps U independent loads in
 // flight per thread: ld(idx) for U indices into registers, then st(idx, v)
 // for them.  A plain loop whose body loads from HBM and stores to LDS waits

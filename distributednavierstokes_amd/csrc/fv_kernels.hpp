@@ -37,28 +37,6 @@ struct FVGeom {
 __device__ inline int IX(int i, int j, int k, const FVGeom &g) { return i + g.nx * (j + g.ny * k); }
 __device__ inline int CX(int i, int j, int k, const FVGeom &g) { return i + g.ldc * (j + g.ny * k); }
 
-// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
-// with its own L2.  This bijection of linear block ids gives every XCD one
-// contiguous range of the domain, so the y/z stencil neighbours a block
-// reads were mostly fetched into the same L2 by the blocks before it.
-__device__ inline int xcd_block(int bid, int nb)
-{
-    const int X = 8;
-    const int q = nb / X, r = nb - q * X;
-    const int xcd = bid % X, slot = bid / X;
-    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-}
-
-// 3D block coordinates after the XCD remap of a (gx, gy, gz) grid.
-__device__ inline void xcd_block3(int &bx, int &by, int &bz)
-{
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int nb = gx * gy * gridDim.z;
-    const int lb = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
-    bx = lb % gx;
-    by = (lb / gx) % gy;
-    bz = lb / (gx * gy);
-}
 __device__ inline int wrapm(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ inline int wrapp(int i, int n) { return i == n - 1 ? 0 : i + 1; }
 
@@ -527,7 +505,9 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
-    const int kx0 = blockIdx.x * tb, j = blockIdx.y;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int kx0 = bxr * tb, j = byr;
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
@@ -562,7 +542,9 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
-    const int kx0 = blockIdx.x * tb, kz = blockIdx.y;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int kx0 = bxr * tb, kz = byr;
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
@@ -697,7 +679,9 @@ __global__ void __launch_bounds__(512) k_thomas_tile(cplx<T> *__restrict__ A, co
     double2 *Xin = Bs + blockDim.x;                      // [S][TL] segment inflows
     double *As = reinterpret_cast<double *>(Xin + blockDim.x);
     double *sa = As + blockDim.x, *sc = sa + ny;
-    const int kx0 = blockIdx.x * TL, kz = blockIdx.y;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int kx0 = bxr * TL, kz = byr;
     const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
     const int l = threadIdx.x % TL, sg = threadIdx.x / TL, kx = kx0 + l;
     const bool valid = kx < g.nxc;
@@ -955,7 +939,9 @@ __global__ void __launch_bounds__(512) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
         cq[j] = j == ny - 1 ? 0.0 : -a.cp[j]; // backward α
     }
     const int l = threadIdx.x % TL, sg = threadIdx.x / TL;
-    const int i = blockIdx.x * TL + l, k = blockIdx.y;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int i = bxr * TL + l, k = byr;
     const bool valid = i < g.nx;
     const int ii = valid ? i : 0; // loads of an idle lane stay in bounds; its results are discarded
     const int ip1 = wrapp(ii, g.nx), kp1 = wrapp(k, g.nz);

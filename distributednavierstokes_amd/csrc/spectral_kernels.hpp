@@ -57,7 +57,9 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
-    const int i0 = blockIdx.x * tb, kz = blockIdx.y, c = blockIdx.z;
+    int bxr, byr, bzr;
+    xcd_block3(bxr, byr, bzr);
+    const int i0 = bxr * tb, kz = byr, c = bzr;
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
@@ -88,7 +90,9 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
-    const int i0 = blockIdx.x * tb, y = blockIdx.y, c = blockIdx.z;
+    int bxr, byr, bzr;
+    xcd_block3(bxr, byr, bzr);
+    const int i0 = bxr * tb, y = byr, c = bzr;
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
@@ -119,7 +123,9 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
-    const int i0 = blockIdx.x * tb, y = blockIdx.y, p = blockIdx.z;
+    int bxr, byr, bzr;
+    xcd_block3(bxr, byr, bzr);
+    const int i0 = bxr * tb, y = byr, p = bzr;
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
@@ -148,7 +154,9 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
-    const int i0 = blockIdx.x * tb, kc = blockIdx.y, p = blockIdx.z;
+    int bxr, byr, bzr;
+    xcd_block3(bxr, byr, bzr);
+    const int i0 = bxr * tb, kc = byr, p = bzr;
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
@@ -196,7 +204,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     T *v = reinterpret_cast<T *>(smem_raw);
     cplx<T> *work = reinterpret_cast<cplx<T> *>(smem_raw + sizeof(T) * (size_t)3 * lpb * n);
     const int nlines = n * n;
-    const int L0 = blockIdx.x * lpb;
+    const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
     const size_t pstride = (size_t)K * n * n; // component stride of P
     if (MODE != XMID_R2C) {
         for (int c = 0; c < 3; ++c) {
@@ -320,7 +328,9 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m, tb = s.tb;
-    const int i0 = blockIdx.x * tb, kc = blockIdx.y;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int i0 = bxr * tb, kc = byr;
     const T inv = T(1.0 / ((double)n * n * n));
     const T kz = T(g.kfac * sp_freq(kc, g));
     cplx<T> acc[MAXO][3];
