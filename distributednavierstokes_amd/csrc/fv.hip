@@ -121,9 +121,14 @@ template <typename T> struct FVImpl : FVBase {
         g.wall = P->ywall ? 1 : 0;
         imexy = P->ydiffusion ? 1 : 0;
         {
-            // DNS_MOMENTUM selects the stencil kernel: slab (default), col, direct, tiled
+            // DNS_MOMENTUM selects the stencil kernel: slab (default), zm, col, direct, tiled
             const char *e = getenv("DNS_MOMENTUM");
-            momentum_kind = !e ? 3 : !strcmp(e, "direct") ? 1 : !strcmp(e, "tiled") ? 2 : !strcmp(e, "col") ? 0 : 3;
+            momentum_kind = !e                       ? 3
+                            : !strcmp(e, "direct") ? 1
+                            : !strcmp(e, "tiled")  ? 2
+                            : !strcmp(e, "col")    ? 0
+                            : !strcmp(e, "zm")     ? 4
+                                                   : 3;
         }
         visc = P->visc;
         for (int c = 0; c < 3; ++c) f[c] = P->bodyforce[c];
@@ -483,6 +488,20 @@ template <typename T> struct FVImpl : FVBase {
         a.rdz2 = (T)(1.0 / (dz * dz));
         timer.begin(FAM_MOM);
         const bool rq = Ac != 0.0;
+        if (momentum_kind == 4) {
+            constexpr int RY = 2, ZC = 16;
+            dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4 * RY), ceil_div(g.nz, ZC));
+            if (write_v) {
+                if (rq) hipLaunchKernelGGL((k_momentum_zm<T, true, true, RY>), grd, blk, 0, st, a, g, ZC);
+                else hipLaunchKernelGGL((k_momentum_zm<T, false, true, RY>), grd, blk, 0, st, a, g, ZC);
+            } else {
+                if (rq) hipLaunchKernelGGL((k_momentum_zm<T, true, false, RY>), grd, blk, 0, st, a, g, ZC);
+                else hipLaunchKernelGGL((k_momentum_zm<T, false, false, RY>), grd, blk, 0, st, a, g, ZC);
+            }
+            timer.end(FAM_MOM);
+            DNS_HIP_CHECK(hipGetLastError());
+            return DNS_OK;
+        }
         if (momentum_kind == 3) {
             // y-slabs whose three-plane u window (3 planes × 3 components) stays within ~2.5 MB of L2
             const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 4);

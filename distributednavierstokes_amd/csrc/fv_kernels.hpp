@@ -198,6 +198,95 @@ __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
                                       uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m, uz0pm);
 }
 
+// z-marching variant: a thread owns RY consecutive rows of one x column and
+// walks ZC consecutive z planes, holding the in-plane stencil columns of
+// planes k and k+1 in registers (and the few values of plane k−1 the stencil
+// needs), so each plane is read once per column instead of three times and
+// the z neighbours never miss in L1.  Grid (nx/64, ny/(4·RY), nz/ZC) of
+// (64, 4) threads, XCD-ordered.
+template <typename T, int RY> struct MomPlane {
+    T xc[RY + 2], xp[RY], xm[RY + 1];   // ux: centre rows j0-1..j0+RY, x+1 rows, x-1 rows j0..j0+RY
+    T yc[RY + 2], yp[RY + 1], ym[RY];   // uy: centre rows j0-1..j0+RY, x+1 rows j0-1.., x-1 rows
+    T zc[RY + 2], zp[RY], zm[RY];       // uz: centre rows j0-1..j0+RY, x+1 rows, x-1 rows
+};
+
+template <typename T, int RY>
+__device__ inline void mom_load_plane(MomPlane<T, RY> &P, const MomentumArgs<T> &a, const FVGeom &g, int i, int ip,
+                                      int im, int j0, int k)
+{
+    const T m1 = T(-1);
+    auto cx = [&](const T *f, int ii, int r) -> T { return r <= g.ny ? ldc(f, ii, r + 1, k, g, m1) : T(0); };
+    auto fy = [&](int ii, int r) -> T { return r <= g.ny ? ldf(a.uy, ii, r + 1, k, g) : T(0); };
+#pragma unroll
+    for (int r = 0; r < RY + 2; ++r) {
+        P.xc[r] = cx(a.ux, i, j0 - 1 + r);
+        P.yc[r] = fy(i, j0 - 1 + r);
+        P.zc[r] = cx(a.uz, i, j0 - 1 + r);
+    }
+#pragma unroll
+    for (int r = 0; r < RY + 1; ++r) {
+        P.xm[r] = cx(a.ux, im, j0 + r);
+        P.yp[r] = fy(ip, j0 - 1 + r);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        P.xp[r] = cx(a.ux, ip, j0 + r);
+        P.ym[r] = fy(im, j0 + r);
+        P.zp[r] = cx(a.uz, ip, j0 + r);
+        P.zm[r] = cx(a.uz, im, j0 + r);
+    }
+}
+
+template <typename T, bool READ_Q, bool WRITE_V, int RY>
+__global__ void __launch_bounds__(256) k_momentum_zm(MomentumArgs<T> a, FVGeom g, int zc)
+{
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int i = bx * 64 + threadIdx.x;
+    const int j0 = (by * 4 + threadIdx.y) * RY;
+    const int k0 = bz * zc, k1 = min(g.nz, k0 + zc);
+    if (i >= g.nx || j0 >= g.ny) return;
+    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
+    // plane k−1: ux, uy, uz at rows j, uz at rows j+1 and at (x+1, j)
+    T pxc[RY], pyc[RY], pzc[RY + 1], pzp[RY];
+    {
+        MomPlane<T, RY> Pm;
+        mom_load_plane(Pm, a, g, i, ip, im, j0, wrapm(k0, g.nz));
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            pxc[r] = Pm.xc[r + 1];
+            pyc[r] = Pm.yc[r + 1];
+            pzc[r] = Pm.zc[r + 1];
+            pzp[r] = Pm.zp[r];
+        }
+        pzc[RY] = Pm.zc[RY + 1];
+    }
+    MomPlane<T, RY> C, N;
+    mom_load_plane(C, a, g, i, ip, im, j0, k0);
+    for (int k = k0; k < k1; ++k) {
+        mom_load_plane(N, a, g, i, ip, im, j0, wrapp(k, g.nz));
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int j = j0 + r;
+            if (j >= g.ny) break;
+            momentum_cell<T, READ_Q, WRITE_V, true>(
+                a, g, IX(i, j, k, g), j, j + 1, C.xc[r + 1], C.xp[r], C.xm[r], C.xc[r + 2], C.xc[r], N.xc[r + 1],
+                pxc[r], C.xm[r + 1], N.xm[r], C.yc[r + 1], C.yp[r + 1], C.ym[r], C.yc[r + 2], C.yc[r], N.yc[r + 1],
+                pyc[r], C.yp[r], N.yc[r], C.zc[r + 1], C.zp[r], C.zm[r], C.zc[r + 2], C.zc[r], N.zc[r + 1], pzc[r],
+                pzp[r], pzc[r + 1]);
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            pxc[r] = C.xc[r + 1];
+            pyc[r] = C.yc[r + 1];
+            pzc[r] = C.zc[r + 1];
+            pzp[r] = C.zp[r];
+        }
+        pzc[RY] = C.zc[RY + 1];
+        C = N;
+    }
+}
+
 // Slab-ordered variant of k_momentum: blocks are ordered x-fastest within
 // y-slabs of `nbys` block rows, then z, then slab, and each XCD takes a
 // contiguous run of that order, so one XCD streams through consecutive z

@@ -2,6 +2,7 @@
 // orchestration, forcing/IC/diagnostic entry points and their C ABI.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -114,7 +115,12 @@ template <typename T> struct SpecImpl : SpecBase {
         tbs = std::min(tbs, g.K);
         ldss = (size_t)tbs * (g.n + 1) * sz;
         const size_t perline = (size_t)4 * g.n * sizeof(T);
-        lpb = (int)std::max<size_t>(1, std::min<size_t>(16, 32768 / perline));
+        {
+            // lines per block of the fused x pass: LDS budget DNS_SP_XLDS bytes (default 32 KB)
+            const char *e = getenv("DNS_SP_XLDS");
+            const size_t budget = e ? (size_t)atol(e) : 32768;
+            lpb = (int)std::max<size_t>(1, std::min<size_t>(16, budget / perline));
+        }
         while (lpb > 1 && lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
         ldsx = perline * lpb;
         if (lds > (size_t)maxlds || ldss > (size_t)maxlds || ldsx + 2048 > (size_t)maxlds || tbs * g.m > SP_MAXO * SP_NTHR) {
