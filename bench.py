@@ -364,13 +364,26 @@ def fv_form(prob, steps, warmup, fixed_dt=None, ic=None):
     sps = steps / r["seconds"]
     units = mmin_units_per_step(prob, cfl=fixed_dt is None)
     b = units * N * esize
-    return {"workload": f"FV {prob.n[0]}x{prob.n[1]}x{prob.n[2]} {prob.bc[1]} y, {prob.ydiffusion}, {prob.precision}, "
-                        + ("fixed dt" if fixed_dt else "CFL 0.5"),
-            "steps_per_s": sps, "ms_per_step": 1e3 / sps, "mcell_updates_per_s": N * sps / 1e6,
-            "units_per_step": units, "bytes_per_step": b, "achieved_gbs": b * sps / 1e9,
-            "frac": b * sps / 1e9 / HBM_PEAK_GBS,
-            "kernels": {FAMILIES[f]: v[0] / steps for f, v in r["fam"].items()},
-            "checks": {"maxdiv": r["maxdiv"], "kinetic_energy": r["ke"]}}
+    out = {"workload": f"FV {prob.n[0]}x{prob.n[1]}x{prob.n[2]} {prob.bc[1]} y, {prob.ydiffusion}, {prob.precision}, "
+                       + ("fixed dt" if fixed_dt else "CFL 0.5"),
+           "steps_per_s": sps, "ms_per_step": 1e3 / sps, "mcell_updates_per_s": N * sps / 1e6,
+           "units_per_step": units, "bytes_per_step": b, "achieved_gbs": b * sps / 1e9,
+           "frac": b * sps / 1e9 / HBM_PEAK_GBS,
+           "kernels": {FAMILIES[f]: v[0] / steps for f, v in r["fam"].items()},
+           "checks": {"maxdiv": r["maxdiv"], "kinetic_energy": r["ke"]}}
+    if prob.precision == "plain":
+        rl = refliteral_units_per_step(prob, cfl=fixed_dt is None)
+        out["refliteral_units_per_step"] = rl
+        out["refliteral_bytes_per_step"] = rl * N * 8
+        out["refliteral_frac"] = rl * N * 8 * sps / 1e9 / HBM_PEAK_GBS
+    return out
+
+
+def refliteral_units_per_step(prob, cfl=True):
+    """Reference-literal byte model of the plain kernel sequence: 54 units per
+    explicit stage, 129 per CN stage (the reference's separate broadcasts,
+    copies/transposes and HBM Thomas arrays), plus the 3 CFL reductions."""
+    return 3 * (129 if prob.implicit else 54) + (3 if cfl else 0)
 
 
 def mmin_units_per_step(prob, cfl=True):
@@ -554,8 +567,13 @@ def main():
     if not args.no_forms:
         forms = {}
         forms["tgv64_fp64"] = fv_form(tgv_problem(64), 50, 5, fixed_dt=1e-3, ic=tgv_ic)
-        for prec in ("fp64", "mixed"):
+        forms["channel_256x192x256_plain"] = fv_form(channel_problem((256, 192, 256), "explicit", "plain"), 10, 2)
+        for prec in ("plain", "fp64", "mixed"):
             forms[f"channel_cn_512x384x512_{prec}"] = fv_form(channel_problem((512, 384, 512), "implicit", prec), 10, 2)
+        cn = {k: forms[f"channel_cn_512x384x512_{k}"]["steps_per_s"] for k in ("plain", "fp64", "mixed")}
+        forms["channel_cn_512x384x512_speedups"] = {"mixed_over_plain_fp64": cn["mixed"] / cn["plain"],
+                                                    "fp64_over_plain_fp64": cn["fp64"] / cn["plain"],
+                                                    "mixed_over_fp64": cn["mixed"] / cn["fp64"]}
         for prec in ("fp64", "mixed"):
             forms[f"hit512_{prec}"] = run_spectral(512, 170, prec, 5, 1)
         forms["cnn_train_16x64cube"] = run_cnn_train(10, 2)

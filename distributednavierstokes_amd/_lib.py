@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libdnsrocm.so")
 
 DNS_OK = 0
-DNS_FP64, DNS_MIXED = 0, 1
+DNS_FP64, DNS_MIXED, DNS_PLAIN_FP64 = 0, 1, 2
 LAYOUT_INTERIOR, LAYOUT_GHOSTED = 0, 1
 FIELD = dict(ux=0, uy=1, uz=2, p=3, qx=4, qy=5, qz=6, delta=7)
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "fv_kernels.hpp"
+#include "plain_kernels.hpp"
 #include "host_util.hpp"
 
 namespace dns {
@@ -93,6 +94,10 @@ template <typename T> struct FVImpl : FVBase {
     int nb_pgrad = 0;
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
     int momentum_kind = 3; // 3 slab-ordered (k_momentum_slab), 0 column-ordered, 1 direct, 2 LDS-tiled z march
+    // reference-structured ("plain") step: extra arrays, allocated on first use
+    bool plain = false;
+    T *pr = nullptr, *pwork = nullptr, *pghost = nullptr, *pyb[3] = {};
+    cplx<T> *pB = nullptr, *pC = nullptr;
 
     ~FVImpl() override
     {
@@ -100,7 +105,8 @@ template <typename T> struct FVImpl : FVBase {
         timer.flush();
         void *ptrs[] = {u[0], u[1], u[2], v[0], v[1], v[2], q[0], q[1], q[2], p, delta, A, Dth, Dcn, dy_t, rdy_t,
                         rdyu_t, dy_d, dyu_d, rdy_d, rdyu_d, lam1, lam2, lam3, ay, cy, iptab, ydc[0], ydc[1], ydc[2],
-                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout, upart};
+                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout, upart, pr, pwork,
+                        pghost, pyb[0], pyb[1], pyb[2], pB, pC};
         for (void *q_ : ptrs)
             if (q_) hipFree(q_);
         if (pinned) hipHostFree(pinned);
@@ -712,8 +718,182 @@ template <typename T> struct FVImpl : FVBase {
         return DNS_OK;
     }
 
+    // ------------------------------------------------------------ plain step
+    // The reference's single-rank kernel sequence (src/timestep.jl:41-63):
+    // momentum!, separate RK broadcasts, exchange_halo! plane copies with a
+    // synchronisation, divergence into r, poisson! with the copies between
+    // the rfft/fft/solve passes, copybox!, per-component gradients and, with
+    // CN, imex_update! with its staged copies and HBM Thomas scratch.
+    int plain_alloc()
+    {
+        if (pr) return DNS_OK;
+        const size_t pen = (size_t)g.ldc * g.ny * g.nz;
+        DNS_TRY(dalloc(&pr, (size_t)N));
+        DNS_TRY(dalloc(&pwork, (size_t)N));
+        DNS_TRY(dalloc(&pghost, (size_t)6 * std::max(g.nx, g.ny) * std::max(g.ny, g.nz)));
+        DNS_TRY(dalloc(&pB, pen));
+        DNS_TRY(dalloc(&pC, pen));
+        if (g.wall && !Dth) DNS_TRY(dalloc(&Dth, pen));
+        if (imexy) {
+            for (int c = 0; c < 3; ++c) DNS_TRY(dalloc(&pyb[c], (size_t)N));
+            if (!Dcn) DNS_TRY(dalloc(&Dcn, (size_t)N));
+        }
+        return DNS_OK;
+    }
+
+    void plain_copy(T *dst, const T *src)
+    {
+        hipLaunchKernelGGL((k_plain_copy<T>), dim3(2048), dim3(NTHR), 0, st, dst, src, (long)N);
+    }
+    void plain_copy_pencil(cplx<T> *dst, const cplx<T> *src)
+    {
+        hipLaunchKernelGGL((k_plain_copy<cplx<T>>), dim3(2048), dim3(NTHR), 0, st, dst, src,
+                           (long)g.ldc * g.ny * g.nz);
+    }
+
+    int plain_halo(T *const *fields, int nf)
+    {
+        for (int c = 0; c < nf; ++c)
+            for (int face = 0; face < 6; ++face)
+                hipLaunchKernelGGL((k_plain_halo_face<T>), dim3(64), dim3(NTHR), 0, st, fields[c], pghost, face, g);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        return DNS_OK;
+    }
+
+    int plain_project(T *psi)
+    {
+        const dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
+        timer.begin(FAM_X);
+        hipLaunchKernelGGL((k_plain_div<T>), grd, blk, 0, st, u[0], u[1], u[2], rdy_t, (T)(1.0 / dx), (T)(1.0 / dz), pr,
+                           g);
+        XPassArgs<T> xa{};
+        xa.ux = pr;
+        xa.A = A;
+        xa.tw = twN;
+        xa.twx = twx;
+        xa.lpb = lpb;
+        const int nlines = g.ny * g.nz;
+        hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xa, g, fdx);
+        timer.end(FAM_X);
+        timer.begin(FAM_Z);
+        plain_copy_pencil(pB, A);
+        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, pB,
+                           twz, tbz, tbzf, g, fdz);
+        plain_copy_pencil(pC, pB);
+        timer.end(FAM_Z);
+        timer.begin(FAM_Y);
+        if (g.wall) {
+            const int lines = g.ldc * g.nz;
+            hipLaunchKernelGGL((k_thomas_y<T>), dim3(ceil_div(lines, NTHR)), dim3(NTHR), 0, st, pC, Dth, iptab, ay, cy,
+                               lam1, lam3, 1.0 / ((double)g.nx * g.nz), g);
+        } else {
+            hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
+                               st, pC, twy, lam1, lam2, lam3, 1.0 / ((double)g.nx * g.ny * g.nz), tby, tbyf, g, fdy);
+        }
+        timer.end(FAM_Y);
+        timer.begin(FAM_Z);
+        plain_copy_pencil(pB, pC);
+        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, pB,
+                           twz, tbz, tbzf, g, fdz);
+        plain_copy_pencil(A, pB);
+        timer.end(FAM_Z);
+        timer.begin(FAM_X);
+        XInvArgs<T> xi{};
+        xi.A = A;
+        xi.out = pwork;
+        xi.tw = twN;
+        xi.twx = twx;
+        xi.lpb = lpb;
+        hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
+        plain_copy(psi, pwork); // copybox! into the field interior
+        timer.end(FAM_X);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_TRY(plain_halo(&psi, 1));
+        timer.begin(FAM_PGRAD);
+        hipLaunchKernelGGL((k_plain_pgrad<T, 0>), grd, blk, 0, st, u[0], psi, rdyu_t, (T)(1.0 / dx), (T)(1.0 / dz), T(1), g);
+        hipLaunchKernelGGL((k_plain_pgrad<T, 1>), grd, blk, 0, st, u[1], psi, rdyu_t, (T)(1.0 / dx), (T)(1.0 / dz), T(1), g);
+        hipLaunchKernelGGL((k_plain_pgrad<T, 2>), grd, blk, 0, st, u[2], psi, rdyu_t, (T)(1.0 / dx), (T)(1.0 / dz), T(1), g);
+        timer.end(FAM_PGRAD);
+        DNS_HIP_CHECK(hipGetLastError());
+        return plain_halo(u, 3);
+    }
+
+    int plain_imex(double B, double alpha, double dt)
+    {
+        const double theta = alpha * dt * visc / 2;
+        const dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
+        timer.begin(FAM_CN);
+        hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(NTHR), (size_t)6 * g.ny * sizeof(double), st, ydc[0], ydc[1], ydc[2],
+                           ydc[3], ydc[4], ydc[5], theta, g.ny, cncoef);
+        for (int c = 0; c < 3; ++c) {
+            const int set = c == 1 ? 1 : 0;
+            const double *ca = ydc[3 * set], *cb = ydc[3 * set + 1], *cc = ydc[3 * set + 2];
+            const double *cp = cncoef + set * 2 * g.ny, *ip = cp + g.ny;
+            plain_copy(pyb[0], u[c]);   // uⁿ staged
+            plain_copy(pyb[1], pyb[0]); // y-local layout (a copy on one rank)
+            if (c == 0)
+                hipLaunchKernelGGL((k_plain_imexv<T, 0>), grd, blk, 0, st, u[c], q[c], p, rdyu_t, B, alpha * dt,
+                                   1.0 / dx, 1.0 / dz, g);
+            else if (c == 1)
+                hipLaunchKernelGGL((k_plain_imexv<T, 1>), grd, blk, 0, st, u[c], q[c], p, rdyu_t, B, alpha * dt,
+                                   1.0 / dx, 1.0 / dz, g);
+            else
+                hipLaunchKernelGGL((k_plain_imexv<T, 2>), grd, blk, 0, st, u[c], q[c], p, rdyu_t, B, alpha * dt,
+                                   1.0 / dx, 1.0 / dz, g);
+            plain_copy(pyb[0], u[c]);   // v staged
+            plain_copy(pyb[2], pyb[0]); // y-local layout
+            hipLaunchKernelGGL((k_plain_imexrhs<T>), grd, blk, 0, st, pyb[2], pyb[1], ca, cb, cc, theta, g);
+            hipLaunchKernelGGL((k_plain_imexsolve<T>), dim3(ceil_div(g.nx, 64), g.nz), dim3(64), 0, st, pyb[0], pyb[2],
+                               Dcn, ca, cp, ip, theta, g);
+            plain_copy(pyb[1], pyb[0]); // back to the x-local layout
+            plain_copy(u[c], pyb[1]);
+        }
+        timer.end(FAM_CN);
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipStreamSynchronize(st)); // imex_update! synchronises (src/ydiffusion.jl:153)
+        return DNS_OK;
+    }
+
+    int step_plain(double dt)
+    {
+        DNS_TRY(plain_alloc());
+        for (int s = 0; s < 3; ++s) {
+            DNS_TRY(momentum_plain(RK_A[s], dt));
+            if (!imexy) {
+                timer.begin(FAM_MOM);
+                for (int c = 0; c < 3; ++c)
+                    hipLaunchKernelGGL((k_plain_axpy<T>), dim3(2048), dim3(NTHR), 0, st, u[c], q[c], (T)RK_B[s], (long)N);
+                timer.end(FAM_MOM);
+                DNS_TRY(plain_halo(u, 3));
+                DNS_TRY(plain_project(p));
+            } else {
+                DNS_TRY(plain_imex(RK_B[s], RK_ALPHA[s], dt));
+                DNS_TRY(plain_halo(u, 3));
+                DNS_TRY(plain_project(delta));
+                hipLaunchKernelGGL((k_plain_axpy<T>), dim3(2048), dim3(NTHR), 0, st, p, delta,
+                                   (T)(1.0 / (RK_ALPHA[s] * dt)), (long)N);
+                DNS_TRY(plain_halo(&p, 1));
+            }
+        }
+        umax_valid = false;
+        DNS_HIP_CHECK(hipGetLastError());
+        return DNS_OK;
+    }
+
+    // momentum_kernel! as one launch writing q only (src/operators.jl:44-178)
+    int momentum_plain(double Ac, double dt)
+    {
+        const int keep = momentum_kind;
+        momentum_kind = 1;
+        const int rc = launch_momentum(u, nullptr, Ac, dt, 0.0, false);
+        momentum_kind = keep;
+        return rc;
+    }
+
     int step(double dt) override
     {
+        if (plain) return step_plain(dt);
         for (int s = 0; s < 3; ++s) {
             if (!imexy) {
                 DNS_TRY(launch_momentum(u, v, RK_A[s], dt, RK_B[s], true));
@@ -898,15 +1078,16 @@ int dns_fv_create(const dns_fv_params *P, dns_fv **out)
         set_error("implicit y-diffusion requires bc[2] == :wall");
         return DNS_EINVAL;
     }
-    if (P->precision != DNS_FP64 && P->precision != DNS_MIXED) {
-        set_error("precision must be DNS_FP64 or DNS_MIXED");
+    if (P->precision != DNS_FP64 && P->precision != DNS_MIXED && P->precision != DNS_PLAIN_FP64) {
+        set_error("precision must be DNS_FP64, DNS_MIXED or DNS_PLAIN_FP64");
         return DNS_EINVAL;
     }
     std::unique_ptr<dns_fv> h(new dns_fv);
     int rc;
-    if (P->precision == DNS_FP64) {
+    if (P->precision == DNS_FP64 || P->precision == DNS_PLAIN_FP64) {
         auto *impl = new FVImpl<double>();
         h->impl.reset(impl);
+        impl->plain = P->precision == DNS_PLAIN_FP64;
         rc = impl->create(P);
     } else {
         auto *impl = new FVImpl<float>();

@@ -88,7 +88,7 @@ class Setup:
         par.visc = prob.visc
         par.bodyforce[:] = list(prob.bodyforce)
         par.ydiffusion = int(prob.implicit)
-        par.precision = _lib.DNS_MIXED if prob.precision == "mixed" else _lib.DNS_FP64
+        par.precision = {"mixed": _lib.DNS_MIXED, "plain": _lib.DNS_PLAIN_FP64}.get(prob.precision, _lib.DNS_FP64)
         h = _lib.ct.c_void_p()
         check(L.dns_fv_create(_lib.ct.byref(par), _lib.ct.byref(h)))
         self.handle = h

@@ -61,8 +61,8 @@ class FVProblem:
             raise DNSError("ydiffusion must be :explicit or :implicit")
         if self.order != 2:
             raise DNSError("only order 2 is implemented so far")
-        if self.precision not in ("fp64", "mixed"):
-            raise DNSError("precision must be fp64 or mixed")
+        if self.precision not in ("fp64", "mixed", "plain"):
+            raise DNSError("precision must be fp64, mixed or plain")
         if min(self.n) < 2:
             raise DNSError("local block smaller than ghost width; use fewer ranks")
         if self.ydiffusion == "implicit" and self.bc[1] != "wall":

@@ -34,6 +34,11 @@ extern "C" {
 
 #define DNS_FP64 0  /* FP64 storage and arithmetic */
 #define DNS_MIXED 1 /* FP32 storage, stencils and FFTs; FP64 Thomas solves and reductions */
+/* FP64 with the reference's single-rank kernel sequence (one kernel per
+ * reference kernel, separate RK broadcasts, materialised copies/transposes,
+ * halo plane copies with synchronisation, HBM Thomas scratch): the baseline
+ * the fused forms are measured against (FV handles only). */
+#define DNS_PLAIN_FP64 2
 
 #define DNS_LAYOUT_INTERIOR 0
 #define DNS_LAYOUT_GHOSTED 1
@@ -72,7 +77,7 @@ typedef struct dns_fv_params {
     double visc;           /* ν */
     double bodyforce[3];   /* f */
     int32_t ydiffusion;    /* 0: :explicit, 1: :implicit (per-stage CN in y) */
-    int32_t precision;     /* DNS_FP64 or DNS_MIXED */
+    int32_t precision;     /* DNS_FP64, DNS_MIXED or DNS_PLAIN_FP64 */
 } dns_fv_params;
 
 /* setup(; ...) -- src/setup.jl:34-113 (with plan_poisson, src/poisson.jl:167-231,

@@ -1,6 +1,7 @@
 """GPU parity of the finite-volume path against the FP64 CPU oracle on the
 same seeded inputs, through the C ABI (via the package's ctypes binding).
 FP64 tolerance: 1e-11 relative (trajectories), 1e-12 (single operators)."""
+import dataclasses
 import math
 
 import numpy as np
@@ -76,6 +77,29 @@ def test_projection_parity_and_divergence(name):
 def test_step_parity(name):
     prob = problems()[name]
     s, o = pair(prob)
+    u = randomfield(prob, seed=9, halo=o.exchange_halo_u)
+    o.project(u, zeros(prob))
+    ug = [a.copy(order="F") for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    qo, qg = vzeros(prob), vzeros(prob)
+    do, dg = (zeros(prob), zeros(prob)) if prob.implicit else (po, pg)
+    for _ in range(3):
+        o.step(u, po, qo, do, 0.004)
+        dns.step_(ug, pg, qg, dg, s, 0.004)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11, c
+    assert relmax(pg, po) < 1e-10
+    assert dns.maxdiv(ug, s) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
+def test_plain_step_parity(name):
+    """The reference-structured kernel sequence (the measurement baseline)
+    against the FP64 oracle, as the fused step."""
+    prob = problems()[name]
+    o = OracleFV(prob)
+    plainprob = dataclasses.replace(prob, precision="plain")
+    s = Setup(plainprob, ROCmBackend())
     u = randomfield(prob, seed=9, halo=o.exchange_halo_u)
     o.project(u, zeros(prob))
     ug = [a.copy(order="F") for a in u]
