@@ -142,6 +142,8 @@ struct CNN {
         timer.st = st;
         const char *env = getenv("DNS_CNN_MFMA");
         mfma_k8 = env && !strcmp(env, "k8");
+        const char *envc = getenv("DNS_CNN_CONV");
+        row_conv = !(envc && !strcmp(envc, "direct"));
         const int kin[4] = {3, 64, 64, 64}, nout[4] = {64, 64, 64, 6};
         size_t off = 0;
         for (int l = 0; l < 4; ++l) {
@@ -215,6 +217,7 @@ struct CNN {
     // the K = 16 MFMA; DNS_CNN_MFMA=k8 selects the 32x32x8 kernel instead.
     bool mfma_k8 = false;
     bool row_wgrad = false; // k_conv_wgrad16 (M % 16 == 0) instead of k_conv_wgrad
+    bool row_conv = false;  // k_conv_rows (M = 32 or 64) instead of k_conv16; DNS_CNN_CONV=direct disables
     template <int EPI> void conv(const ConvArgs &ca, int np, long Vn, hipStream_t s)
     {
         if (mfma_k8) {
@@ -224,6 +227,18 @@ struct CNN {
             } else {
                 dim3 grd((unsigned)((Vn + 127) / 128), (unsigned)((np + 31) / 32));
                 hipLaunchKernelGGL((k_conv<4, 1, EPI>), grd, dim3(256), 0, s, ca, geom(Vn));
+            }
+            return;
+        }
+        if (row_conv && (M == 32 || M == 64) && ca.kin % 16 == 0 && (np == 32 || np == 64)) {
+            const dim3 grd((unsigned)(M / 4), (unsigned)M, (unsigned)(Vn / Vs));
+            const size_t lds = sizeof(unsigned short) * 6 * (size_t)M * (ca.kin + 8);
+            if (M == 64) {
+                if (np == 64) hipLaunchKernelGGL((k_conv_rows<2, 2, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+                else hipLaunchKernelGGL((k_conv_rows<2, 1, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+            } else {
+                if (np == 64) hipLaunchKernelGGL((k_conv_rows<1, 2, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+                else hipLaunchKernelGGL((k_conv_rows<1, 1, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
             }
             return;
         }

@@ -211,6 +211,102 @@ __global__ void __launch_bounds__(256) k_conv16(ConvArgs a, ConvGeom g)
         }
 }
 
+// Row-staged convolution: a block owns 4 consecutive x-rows (y0..y0+3 at
+// fixed z, b) of M = 32·MB voxels, one row per wave; for each of the three
+// z offsets it stages the input rows y0−1..y0+4 of plane z + sign·dz in LDS
+// (voxel stride kin + 8 halves, so the 16-byte fragment reads of one wave
+// spread over the banks), then runs the nine (dx, dy) offsets from LDS with
+// the x wrap applied per lane.  Weights stream from L2; each wave keeps a
+// (32·MB voxels) × (32·NB channels) accumulator, so every fragment feeds MB
+// or NB MFMAs.  Epilogues as k_conv.  Grid (M/4, M, B) blocks of 256 threads;
+// LDS 6·M·(kin + 8) halves.  Needs M = 32·MB and kin % 16 == 0.
+template <int MB, int NB, int EPI>
+__global__ void __launch_bounds__(256) k_conv_rows(ConvArgs a, ConvGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    unsigned short *sa = reinterpret_cast<unsigned short *>(smem_raw);
+    constexpr int M = 32 * MB;
+    const int KS = a.kin + 8; // voxel stride in LDS (halves)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, h = lane >> 5;
+    const int y0 = blockIdx.x * 4, z = blockIdx.y, b = blockIdx.z;
+    const int n0 = 0;
+    f32x16 acc[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[mb][nb][r] = 0.0f;
+    const int kc8 = a.kin / 8;
+    for (int dzi = 0; dzi < 3; ++dzi) {
+        const int zs = (z + a.sign * (dzi - 1) + M) % M;
+        __syncthreads();
+        // stage rows y0-1 .. y0+4 of plane zs
+        for (int e = threadIdx.x; e < 6 * M * kc8; e += blockDim.x) {
+            const int c8 = e % kc8, t = e / kc8, x = t % M, rr = t / M;
+            const int ys = (y0 - 1 + rr + M) % M;
+            const long v = x + (long)M * (ys + (long)M * (zs + (long)M * b));
+            *reinterpret_cast<uint4 *>(sa + (rr * M + x) * KS + 8 * c8) =
+                *reinterpret_cast<const uint4 *>(a.in + v * a.kin + 8 * c8);
+        }
+        __syncthreads();
+        for (int dyi = 0; dyi < 3; ++dyi) {
+            const int rr = wave + 1 + a.sign * (dyi - 1); // staged row of this wave's neighbour
+            for (int dxi = 0; dxi < 3; ++dxi) {
+                const int o = dxi + 3 * dyi + 9 * dzi;
+                const int dx = a.sign * (dxi - 1);
+                const unsigned short *ap[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    const int xs = (32 * mb + row + dx + M) % M;
+                    ap[mb] = sa + (rr * M + xs) * KS + 8 * h;
+                }
+                const unsigned short *bp[NB];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) bp[nb] = a.W + ((size_t)o * a.np + n0 + 32 * nb + row) * a.kin + 8 * h;
+                for (int k0 = 0; k0 < a.kin; k0 += 16) {
+                    bf16x8 av[MB], bv[NB];
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        bv[nb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(bp[nb] + k0));
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb)
+                        av[mb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(ap[mb] + k0));
+#pragma unroll
+                    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                        for (int nb = 0; nb < NB; ++nb)
+                            acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[mb], bv[nb], acc[mb][nb], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const long vrow = (long)M * ((y0 + wave) + (long)M * (z + (long)M * b));
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int col = n0 + 32 * nb + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long vv = vrow + 32 * mb + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (col >= a.nout) continue;
+                const float c = acc[mb][nb][r];
+                if (EPI == 0) {
+                    const float zz = c + a.bias[col];
+                    a.zbuf[vv * a.ldo + col] = f2bf(zz);
+                    a.out[vv * a.ldo + col] = f2bf(gelu_f(zz));
+                } else if (EPI == 1) {
+                    a.yout[vv * a.ldo + col] = c + a.bias[col];
+                } else {
+                    const float zp = bf2f(a.zprev[vv * a.ldo + col]);
+                    a.out[vv * a.ldo + col] = f2bf(c * gelu_d(zp));
+                }
+            }
+        }
+}
+
 // Weight gradient of one offset per block-row: for o = blockIdx.y,
 // dW[o][n][k] += Σ_{v in chunk} dout[v][n] · in[v + d_o][k], tiles of 64
 // voxels staged (transposed) through LDS.  Each block covers a range of

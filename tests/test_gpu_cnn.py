@@ -24,19 +24,21 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("M", [8, 16])
-def test_forward_matches_oracle(M):
-    net, p, x, _ = setup_case(M=M)
+@pytest.mark.parametrize("M,B", [(8, 2), (16, 2), (32, 1)])
+def test_forward_matches_oracle(M, B):
+    """M = 32 takes the row-staged LDS convolution, smaller grids the direct one."""
+    net, p, x, _ = setup_case(B=B, M=M)
     y = net.forward(x)
     yo, _, _ = oc.forward(oc.bf16_round(p), oc.bf16_round(x))
     assert rel(y, yo) < 2e-2
 
 
-@pytest.mark.parametrize("M", [8, 16])
-def test_gradients_match_oracle(M):
-    """M = 8 takes the voxel-tile weight-gradient kernel, M = 16 the
-    row-staged one (x-rows of a multiple of 16)."""
-    net, p, x, t = setup_case(M=M, seed=1)
+@pytest.mark.parametrize("M,B", [(8, 2), (16, 2), (32, 1)])
+def test_gradients_match_oracle(M, B):
+    """M = 8 takes the voxel-tile weight-gradient kernel, M = 16 and 32 the
+    row-staged one (x-rows of a multiple of 16); M = 32 also the row-staged
+    forward and data-gradient convolutions."""
+    net, p, x, t = setup_case(B=B, M=M, seed=1)
     loss = net.loss_grad(x, t)
     g = net.grads()
     lo, go = oc.loss_grad(oc.bf16_round(p), oc.bf16_round(x), t.astype(np.float64))
