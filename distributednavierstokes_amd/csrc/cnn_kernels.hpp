@@ -378,6 +378,18 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(WGradArgs a, ConvGeom g)
 // the (n-tile, k-tile, dx) 32×32 accumulators.  Partials
 // part[blockIdx.x][o][n][k] as k_conv_wgrad.  Needs M % 16 == 0, np and kp
 // in {32, 64}, ldd and kin multiples of 8.  LDS 4·64·(M + 8) halves.
+// Halves q of four 16-byte vectors (8 BF16 channels of 4 voxels) packed
+// into one 8-byte word in voxel order: the 4×8 → 8×4 register transpose of
+// the weight-gradient staging.
+__device__ inline uint2 pack4_bf16(const uint4 (&v)[4], int q)
+{
+    auto half = [&](const uint4 &w) -> unsigned int {
+        const unsigned int word = (q >> 1) == 0 ? w.x : (q >> 1) == 1 ? w.y : (q >> 1) == 2 ? w.z : w.w;
+        return (q & 1) ? (word >> 16) : (word & 0xffffu);
+    };
+    return make_uint2(half(v[0]) | (half(v[1]) << 16), half(v[2]) | (half(v[3]) << 16));
+}
+
 struct WGrad16Args {
     const unsigned short *dout; // [V][ldd]
     const unsigned short *in;   // [V][kin]
@@ -416,24 +428,31 @@ __global__ void __launch_bounds__(256) k_conv_wgrad16(WGrad16Args a, ConvGeom g)
         const int zs = z + dz < 0 ? z + dz + M : (z + dz >= M ? z + dz - M : z + dz);
         const long vrow = r * M, vsrc = (long)M * (ys + (long)M * (zs + (long)M * b));
         __syncthreads();
-        for (int e = threadIdx.x; e < M * nc8; e += blockDim.x) {
-            const int x = e / nc8, c8 = (e - x * nc8) * 8;
-            const uint4 v = c8 + 8 <= a.ldd ? *reinterpret_cast<const uint4 *>(a.dout + (vrow + x) * a.ldd + c8) : zero4;
-            const unsigned short *hv = reinterpret_cast<const unsigned short *>(&v);
+        // groups of 4 voxels × 8 channels are loaded as four 16-byte vectors and
+        // written transposed as eight 8-byte LDS stores
+        for (int e = threadIdx.x; e < (M / 4) * nc8; e += blockDim.x) {
+            const int xg = e / nc8, c8 = (e - xg * nc8) * 8, x = 4 * xg;
+            uint4 v[4];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) sd[(c8 + q) * LD + x] = hv[q];
+            for (int vi = 0; vi < 4; ++vi)
+                v[vi] = c8 + 8 <= a.ldd ? *reinterpret_cast<const uint4 *>(a.dout + (vrow + x + vi) * a.ldd + c8) : zero4;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                *reinterpret_cast<uint2 *>(sd + (c8 + q) * LD + x) = pack4_bf16(v, q);
         }
-        for (int e = threadIdx.x; e < M * kc8; e += blockDim.x) {
-            const int x = e / kc8, c8 = (e - x * kc8) * 8;
-            const uint4 v = c8 + 8 <= a.kin ? *reinterpret_cast<const uint4 *>(a.in + (vsrc + x) * a.kin + c8) : zero4;
-            const unsigned short *hv = reinterpret_cast<const unsigned short *>(&v);
-            const int xm = x + 1 == M ? 0 : x + 1, xp = x == 0 ? M - 1 : x - 1; // dx = −1, +1 destinations
+        for (int e = threadIdx.x; e < 3 * (M / 4) * kc8; e += blockDim.x) {
+            const int dxi = e / ((M / 4) * kc8), f = e - dxi * ((M / 4) * kc8);
+            const int xg = f / kc8, c8 = (f - xg * kc8) * 8, x = 4 * xg;
+            uint4 v[4];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                si[(0 * 64 + c8 + q) * LD + xm] = hv[q];
-                si[(1 * 64 + c8 + q) * LD + x] = hv[q];
-                si[(2 * 64 + c8 + q) * LD + xp] = hv[q];
+            for (int vi = 0; vi < 4; ++vi) {
+                int xs = x + vi + dxi - 1; // si[dx][k][x] = in[(x + dx) mod M][k]
+                xs = xs < 0 ? xs + M : (xs >= M ? xs - M : xs);
+                v[vi] = c8 + 8 <= a.kin ? *reinterpret_cast<const uint4 *>(a.in + (vsrc + xs) * a.kin + c8) : zero4;
             }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                *reinterpret_cast<uint2 *>(si + (dxi * 64 + c8 + q) * LD + x) = pack4_bf16(v, q);
         }
         __syncthreads();
 #pragma unroll
