@@ -114,15 +114,16 @@ template <typename T> struct SpecImpl : SpecBase {
         while (tbs > 1 && (size_t)tbs * (g.n + 1) * sz > (size_t)maxlds) --tbs;
         tbs = std::min(tbs, g.K);
         ldss = (size_t)tbs * (g.n + 1) * sz;
-        const size_t perline = (size_t)4 * g.n * sizeof(T);
+        // per line: three real velocity lines and three half-length complex work lines
+        const size_t perline = (size_t)6 * g.n * sizeof(T);
         {
-            // lines per block of the fused x pass: LDS budget DNS_SP_XLDS bytes (default 32 KB)
+            // lines per block of the fused x pass: LDS budget DNS_SP_XLDS bytes (default 64 KB)
             const char *e = getenv("DNS_SP_XLDS");
-            const size_t budget = e ? (size_t)atol(e) : 32768;
+            const size_t budget = e ? (size_t)atol(e) : 65536;
             lpb = (int)std::max<size_t>(1, std::min<size_t>(16, budget / perline));
         }
-        while (lpb > 1 && lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
-        ldsx = perline * lpb;
+        while (lpb > 1 && 3 * lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
+        ldsx = perline * lpb + (size_t)(g.n / 2) * sz; // + the LDS twiddle table
         if (lds > (size_t)maxlds || ldss > (size_t)maxlds || ldsx + 2048 > (size_t)maxlds || tbs * g.m > SP_MAXO * SP_NTHR) {
             set_error("spectral tiles do not fit the LDS limit");
             return DNS_EUNSUPPORTED;

@@ -186,7 +186,9 @@ enum { XMID_RHS = 0, XMID_C2R = 1, XMID_R2C = 2, XMID_VMAX = 3 };
 //   XMID_C2R : c2r of P[..,0:3] → phys (n, n, n, 3)
 //   XMID_VMAX: c2r of P[..,0:3] → vmax partial only
 //   XMID_R2C : r2c of phys (n, n, n, 3) → P[..,0:3] (kx < K)
-// LDS: v[3][lpb][n] reals followed by work[lpb][n/2] complex.
+// The three components (and each group of three products) are transformed
+// together as 3·lpb lines.  LDS: v[3][lpb][n] reals, work[3·lpb][n/2]
+// complex, then the n/2 twiddles.
 template <typename T> struct XMidArgs {
     cplx<T> *P;
     T *phys;
@@ -206,40 +208,44 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     const int nlines = n * n;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
     const size_t pstride = (size_t)K * n * n; // component stride of P
+    const int nw = 3 * lpb;                   // work lines: three components (or products) × lpb
+    const int maxl = max(1, EPT * (int)blockDim.x / N); // lines one fft_lds call may hold
+    // twiddles staged in LDS behind the work lines (read by every butterfly pass)
+    cplx<T> *twl = work + (size_t)nw * N;
+    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = a.tw[t];
     if (MODE != XMID_R2C) {
-        for (int c = 0; c < 3; ++c) {
-            batched_for<SP_LOAD_U(EPT), cplx<T>>(
-                lpb * N,
-                [&](int idx) {
-                    int b = idx / N, k = idx - b * N, L = L0 + b;
-                    cplx<T> Z = mk<T>(T(0), T(0));
-                    if (L < nlines) {
-                        const cplx<T> *row = a.P + (size_t)L * K + c * pstride;
-                        cplx<T> Xk = k < K ? row[k] : mk<T>(T(0), T(0));
-                        cplx<T> Xn = (N - k) < K ? row[N - k] : mk<T>(T(0), T(0));
-                        if (k == 0) {
-                            Xk.y = T(0);
-                            Xn.y = T(0);
-                        }
-                        cplx<T> Xc = cconj<T>(Xn);
-                        cplx<T> ev = cadd<T>(Xk, Xc);
-                        cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]);
-                        Z = mk<T>(ev.x - od.y, ev.y + od.x);
+        // c2r of the three components in one batched transform: work line c·lpb + b
+        batched_for<SP_LOAD_U(EPT), cplx<T>>(
+            nw * N,
+            [&](int idx) {
+                const int cl = idx / N, k = idx - cl * N, c = cl / lpb, b = cl - c * lpb, L = L0 + b;
+                cplx<T> Z = mk<T>(T(0), T(0));
+                if (L < nlines) {
+                    const cplx<T> *row = a.P + (size_t)L * K + c * pstride;
+                    cplx<T> Xk = k < K ? row[k] : mk<T>(T(0), T(0));
+                    cplx<T> Xn = (N - k) < K ? row[N - k] : mk<T>(T(0), T(0));
+                    if (k == 0) {
+                        Xk.y = T(0);
+                        Xn.y = T(0);
                     }
-                    return Z;
-                },
-                [&](int idx, cplx<T> v) { work[idx] = v; });
-            __syncthreads();
-            fft_lds<T, EPT, true>(work, lpb, N, d, a.tw);
-            for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
-                int b = idx / N, t = idx - b * N;
-                cplx<T> z = work[idx];
-                T *vl = v + ((size_t)c * lpb + b) * n;
-                vl[2 * t] = z.x;
-                vl[2 * t + 1] = z.y;
-            }
-            __syncthreads();
+                    cplx<T> Xc = cconj<T>(Xn);
+                    cplx<T> ev = cadd<T>(Xk, Xc);
+                    cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]);
+                    Z = mk<T>(ev.x - od.y, ev.y + od.x);
+                }
+                return Z;
+            },
+            [&](int idx, cplx<T> z) { work[idx] = z; });
+        __syncthreads();
+        for (int l0 = 0; l0 < nw; l0 += maxl) fft_lds<T, EPT, true>(work + (size_t)l0 * N, min(maxl, nw - l0), N, d, twl);
+        for (int idx = threadIdx.x; idx < nw * N; idx += blockDim.x) {
+            const int cl = idx / N, t = idx - cl * N;
+            const cplx<T> z = work[idx];
+            T *vl = v + (size_t)cl * n; // v[c][b] = line cl
+            vl[2 * t] = z.x;
+            vl[2 * t + 1] = z.y;
         }
+        __syncthreads();
         if (MODE == XMID_RHS || MODE == XMID_VMAX) {
             __shared__ double red[256];
             double mx = 0;
@@ -267,17 +273,19 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
         }
         if (MODE != XMID_RHS) return;
     }
-    // forward r2c of products (RHS) or of physical components (R2C)
-    const int nout = MODE == XMID_RHS ? 6 : 3;
-    for (int p = 0; p < nout; ++p) {
-        const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
-        const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
+    // forward r2c of the products (RHS: two groups of three) or of the three
+    // physical components (R2C), each group one batched transform
+    const int ngroups = MODE == XMID_RHS ? 2 : 1;
+    for (int grp = 0; grp < ngroups; ++grp) {
         batched_for<SP_LOAD_U(EPT), cplx<T>>(
-            lpb * N,
+            nw * N,
             [&](int idx) {
-                int b = idx / N, t = idx - b * N, L = L0 + b;
+                const int cl = idx / N, t = idx - cl * N, pg = cl / lpb, b = cl - pg * lpb, L = L0 + b;
+                const int p = 3 * grp + pg;
                 cplx<T> z = mk<T>(T(0), T(0));
                 if (MODE == XMID_RHS) {
+                    const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
+                    const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
                     const T *va = v + ((size_t)ca * lpb + b) * n, *vb = v + ((size_t)cb * lpb + b) * n;
                     z = mk<T>(va[2 * t] * vb[2 * t], va[2 * t + 1] * vb[2 * t + 1]);
                 } else if (L < nlines) {
@@ -286,14 +294,15 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
                 }
                 return z;
             },
-            [&](int idx, cplx<T> v2) { work[idx] = v2; });
+            [&](int idx, cplx<T> z) { work[idx] = z; });
         __syncthreads();
-        fft_lds<T, EPT, false>(work, lpb, N, d, a.tw);
-        for (int idx = threadIdx.x; idx < lpb * K; idx += blockDim.x) {
-            int b = idx / K, kx = idx - b * K, L = L0 + b;
+        for (int l0 = 0; l0 < nw; l0 += maxl) fft_lds<T, EPT, false>(work + (size_t)l0 * N, min(maxl, nw - l0), N, d, twl);
+        for (int idx = threadIdx.x; idx < nw * K; idx += blockDim.x) {
+            const int cl = idx / K, kx = idx - cl * K, pg = cl / lpb, b = cl - pg * lpb, L = L0 + b;
             if (L >= nlines) continue;
-            cplx<T> Zk = work[b * N + (kx == N ? 0 : kx)];
-            cplx<T> Zc = cconj<T>(work[b * N + (kx == 0 ? 0 : N - kx)]);
+            const int p = 3 * grp + pg;
+            cplx<T> Zk = work[cl * N + (kx == N ? 0 : kx)];
+            cplx<T> Zc = cconj<T>(work[cl * N + (kx == 0 ? 0 : N - kx)]);
             cplx<T> s = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
             cplx<T> wd = cmul<T>(a.twx[kx], df);
             a.P[kx + (size_t)K * L + p * pstride] = mk<T>(T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x));
