@@ -106,14 +106,14 @@ template <typename T> struct SpecImpl : SpecBase {
         // y/z pencil tiles: FFT sub-batches of tbf lines; a block may load
         // tb >= tbf lines (wider rows measured no faster, so tb = tbf)
         tbf = std::max(1, std::min(64, SP_EPT * SP_NTHR / g.n));
-        while (tbf > 1 && (size_t)tbf * (g.n + 1) * sz > (size_t)maxlds) --tbf;
+        while (tbf > 1 && ((size_t)tbf * (g.n + 1) + g.n) * sz > (size_t)maxlds) --tbf;
         tbf = std::min(tbf, g.K);
         tb = tbf;
-        lds = (size_t)tb * (g.n + 1) * sz;
+        lds = ((size_t)tb * (g.n + 1) + g.n) * sz; // tile + LDS twiddle table
         tbs = std::max(1, std::min({64, SP_SEPT * SP_NTHR / g.n, SP_MAXO * SP_NTHR / g.m}));
-        while (tbs > 1 && (size_t)tbs * (g.n + 1) * sz > (size_t)maxlds) --tbs;
+        while (tbs > 1 && ((size_t)tbs * (g.n + 1) + g.n) * sz > (size_t)maxlds) --tbs;
         tbs = std::min(tbs, g.K);
-        ldss = (size_t)tbs * (g.n + 1) * sz;
+        ldss = ((size_t)tbs * (g.n + 1) + g.n) * sz;
         // per line: three real velocity lines and three half-length complex work lines
         const size_t perline = (size_t)6 * g.n * sizeof(T);
         {

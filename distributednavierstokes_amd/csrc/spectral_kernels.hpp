@@ -48,8 +48,18 @@ __device__ inline int sp_compact(int f, const SpecGeom &g)
 
 // ------------------------------------------------------------ y / z passes
 
+// Copies the n twiddles of a length-n transform into LDS at dst (every
+// butterfly pass reads them); the first barrier after the tile load orders
+// the copy before the transform.
+template <typename T>
+__device__ inline const cplx<T> *stage_twiddles(cplx<T> *dst, const cplx<T> *tw, int n)
+{
+    for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[t];
+    return dst;
+}
+
 // Backward y pass: B1[i, y, kz, c] = Σ_ky uh[i, ky, kz, c] e^{+i ky y}.
-// grid (ceil(K/tb), m, 3); LDS tb·(n+1).
+// grid (ceil(K/tb), m, 3); LDS tb·(n+1) + n (the twiddles).
 template <typename T, int EPT>
 __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb, int tbf,
                                                   SpecGeom g, FFTDesc d)
@@ -57,6 +67,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kz = byr, c = bzr;
@@ -74,7 +85,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb, i = i0 + b;
         if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
@@ -90,6 +101,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, c = bzr;
@@ -107,7 +119,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb, i = i0 + b;
         if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
@@ -123,6 +135,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, p = bzr;
@@ -137,7 +150,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int kc = idx / tb, b = idx - kc * tb, i = i0 + b;
         if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
@@ -154,6 +167,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kc = byr, p = bzr;
@@ -168,7 +182,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     const T inv = T(1.0 / ((double)n * n * n));
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int jc = idx / tb, b = idx - jc * tb, i = i0 + b;
@@ -211,8 +225,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     const int nw = 3 * lpb;                   // work lines: three components (or products) × lpb
     const int maxl = max(1, EPT * (int)blockDim.x / N); // lines one fft_lds call may hold
     // twiddles staged in LDS behind the work lines (read by every butterfly pass)
-    cplx<T> *twl = work + (size_t)nw * N;
-    for (int t = threadIdx.x; t < N; t += blockDim.x) twl[t] = a.tw[t];
+    const cplx<T> *twl = stage_twiddles<T>(work + (size_t)nw * N, a.tw, N);
     if (MODE != XMID_R2C) {
         // c2r of the three components in one batched transform: work line c·lpb + b
         batched_for<SP_LOAD_U(EPT), cplx<T>>(
@@ -337,6 +350,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m, tb = s.tb;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int i0 = bxr * tb, kc = byr;
@@ -359,7 +373,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
                 buf[b * ls + t] = v;
             });
         __syncthreads();
-        fft_lds<T, EPT, false>(buf, tb, ls, d, s.tw);
+        fft_lds<T, EPT, false>(buf, tb, ls, d, twl);
 #pragma unroll
         for (int o = 0; o < MAXO; ++o) {
             int oi = threadIdx.x + o * blockDim.x;
