@@ -200,6 +200,16 @@ __device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns,
     __syncthreads();
 }
 
+// Copies the n twiddles of a length-n transform into LDS at dst (every
+// butterfly pass reads them); the first barrier after the tile load orders
+// the copy before the transform.
+template <typename T>
+__device__ inline const cplx<T> *stage_twiddles(cplx<T> *dst, const cplx<T> *tw, int n)
+{
+    for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[t];
+    return dst;
+}
+
 // Transforms nlines lines of d.n points held in LDS at buf[line*ls + t].
 // tw[t] = exp(-2πi t / n) for t in [0, n).  Unnormalised in both directions.
 // Must be called by every thread of the block (contains barriers).

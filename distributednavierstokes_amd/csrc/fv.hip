@@ -284,7 +284,7 @@ template <typename T> struct FVImpl : FVBase {
         hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (maxlds <= 0) maxlds = 65536;
         lpb = std::max(1, std::min(64, X_EPT * NTHR / std::max(1, Nh)));
-        ldsx = (size_t)lpb * Nh * sizeof(cplx<T>);
+        ldsx = (size_t)(lpb + 1) * Nh * sizeof(cplx<T>); // lines + LDS twiddle table
         // z / periodic-y FFT tiles: a block loads tb lines and transforms them
         // in sub-batches of tbf lines.  In FP32, tiles narrower than 128-byte
         // rows move to 512 threads and at least 16 lines (measured faster); in
@@ -293,16 +293,16 @@ template <typename T> struct FVImpl : FVBase {
             const int rowlines = sizeof(T) == 4 ? (int)(128 / sizeof(cplx<T>)) : 1;
             nthr = (ZY_EPT * NTHR / n) < rowlines ? 2 * NTHR : NTHR;
             tbf = std::max(1, std::min(64, ZY_EPT * nthr / n));
-            while (tbf > 1 && (size_t)tbf * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tbf;
+            while (tbf > 1 && ((size_t)tbf * (n + 1) + n) * sizeof(cplx<T>) > (size_t)maxlds) --tbf;
             tbf = std::min(tbf, g.nxc);
             tb = std::min(g.nxc, std::max(tbf, rowlines));
-            while (tb > tbf && (size_t)tb * (n + 1) * sizeof(cplx<T>) > (size_t)maxlds) --tb;
+            while (tb > tbf && ((size_t)tb * (n + 1) + n) * sizeof(cplx<T>) > (size_t)maxlds) --tb;
         };
         pick_fft(nz, tbz, tbzf, nthrz);
-        ldsz = (size_t)tbz * (nz + 1) * sizeof(cplx<T>);
+        ldsz = ((size_t)tbz * (nz + 1) + nz) * sizeof(cplx<T>); // tile + LDS twiddle table
         if (!g.wall) {
             pick_fft(ny, tby, tbyf, nthry);
-            ldsy = (size_t)tby * (ny + 1) * sizeof(cplx<T>);
+            ldsy = ((size_t)tby * (ny + 1) + ny) * sizeof(cplx<T>);
         }
         if (g.wall) {
             // segmented tridiagonal kernels: a block of 256 or 512 threads owns TL

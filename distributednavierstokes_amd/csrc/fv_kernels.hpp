@@ -486,6 +486,7 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)lpb * N, a.tw, N);
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         lpb * N,
         [&](int idx) {
@@ -514,7 +515,7 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
         },
         [&](int idx, cplx<T> v) { buf[idx] = v; });
     __syncthreads();
-    fft_lds<T, EPT, false>(buf, lpb, N, d, a.tw);
+    fft_lds<T, EPT, false>(buf, lpb, N, d, twl);
     const int nxc = N + 1;
     for (int idx = threadIdx.x; idx < lpb * nxc; idx += blockDim.x) {
         int b = idx / nxc, kx = idx - b * nxc;
@@ -551,6 +552,7 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
     const int N = d.n, lpb = a.lpb;
     const int nlines = g.ny * g.nz;
     const int L0 = xcd_block(blockIdx.x, gridDim.x) * lpb;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)lpb * N, a.tw, N);
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         lpb * N,
         [&](int idx) {
@@ -574,7 +576,7 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
         },
         [&](int idx, cplx<T> v) { buf[idx] = v; });
     __syncthreads();
-    fft_lds<T, EPT, true>(buf, lpb, N, d, a.tw);
+    fft_lds<T, EPT, true>(buf, lpb, N, d, twl);
     for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
         int b = idx / N, t = idx - b * N;
         int L = L0 + b;
@@ -594,6 +596,7 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int kx0 = bxr * tb, j = byr;
@@ -608,7 +611,7 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, INV>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, INV>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb;
         int kx = kx0 + b;
@@ -631,6 +634,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
+    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int kx0 = bxr * tb, kz = byr;
@@ -645,7 +649,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     if (DIVIDE) {
         for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
             int t = idx / tb, b = idx - t * tb;
@@ -657,7 +661,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = mk<T>(T((double)v.x * f), T((double)v.y * f));
         }
         __syncthreads();
-        for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, tw);
+        for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     }
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = idx / tb, b = idx - t * tb;

@@ -48,16 +48,6 @@ __device__ inline int sp_compact(int f, const SpecGeom &g)
 
 // ------------------------------------------------------------ y / z passes
 
-// Copies the n twiddles of a length-n transform into LDS at dst (every
-// butterfly pass reads them); the first barrier after the tile load orders
-// the copy before the transform.
-template <typename T>
-__device__ inline const cplx<T> *stage_twiddles(cplx<T> *dst, const cplx<T> *tw, int n)
-{
-    for (int t = threadIdx.x; t < n; t += blockDim.x) dst[t] = tw[t];
-    return dst;
-}
-
 // Backward y pass: B1[i, y, kz, c] = Σ_ky uh[i, ky, kz, c] e^{+i ky y}.
 // grid (ceil(K/tb), m, 3); LDS tb·(n+1) + n (the twiddles).
 template <typename T, int EPT>
