@@ -152,4 +152,34 @@ __device__ __forceinline__ void batched_for(int total, LD ld, ST st)
     }
 }
 
+// Buffer-resource access: a wave-uniform base and byte range, a per-lane
+// 32-bit byte offset (one VGPR for every row a thread reads) and a
+// wave-uniform byte offset per access (an SGPR).  The hardware range check
+// makes reads at or beyond `bytes` return 0 and drops such writes.
+// Word 3 of the descriptor (0x00020000) is the raw 32-bit data format of
+// the gfx9 family.
+using bufres = __amdgpu_buffer_rsrc_t;
+__device__ inline bufres make_bufres(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ inline T bload(bufres r, int voff, int soff);
+template <> __device__ inline float bload<float>(bufres r, int voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <> __device__ inline double bload<double>(bufres r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline void bstore(bufres r, int voff, int soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+__device__ inline void bstore(bufres r, int voff, int soff, double v)
+{
+    using u2 = __attribute__((ext_vector_type(2))) unsigned int;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+}
+
 }  // namespace dns

@@ -76,6 +76,7 @@ template <typename T> struct FVImpl : FVBase {
     int thomas_tl = 0, thomas_nthr = 256; // lines and threads per block of the LDS Thomas kernel (0: sequential)
     size_t ldst = 0;
     int cn_tl = 0, cn_nthr = 256; // x-lines and threads per block of the LDS CN kernel (0: sequential)
+    int cn_seg = 0;               // rows per segment of the equal-segment CN kernel (0: not used)
     size_t ldscn = 0;
     double *Dcn = nullptr;
     T *dy_t = nullptr, *rdy_t = nullptr, *rdyu_t = nullptr;
@@ -348,7 +349,28 @@ template <typename T> struct FVImpl : FVBase {
                           },
                           cn_tl, cn_nthr, ldscn);
                 if (envc && !strcmp(envc, "sequential")) cn_tl = 0;
+                if (const char *et = getenv("DNS_CN_TILE")) { // "TL:threads" override for tile sweeps
+                    int tl = 0, nt = 0;
+                    if (sscanf(et, "%d:%d", &tl, &nt) == 2 && tl > 0 && (nt == 256 || nt == 512) && nt % tl == 0 &&
+                        (ny + nt / tl - 1) / (nt / tl) <= CN_RMAX) {
+                        cn_tl = tl;
+                        cn_nthr = nt;
+                        ldscn = (size_t)3 * nt * sizeof(double) + (size_t)6 * ny * sizeof(double);
+                    }
+                }
                 if (!cn_tl) DNS_TRY(dalloc(&Dcn, (size_t)N));
+                // equal segments of R rows (ny = S·R) use k_cn_seg; DNS_CN=tile keeps k_cn_tile
+                cn_seg = 0;
+                if (cn_tl && cn_nthr == 256 && !(envc && !strcmp(envc, "tile"))) {
+                    const int S = cn_nthr / cn_tl, R = ny / S;
+                    if (ny % S == 0 && (R == 8 || R == 16 || R == 24 || R == 32) &&
+                        ldscn + (size_t)ny * sizeof(double) <= (size_t)maxlds)
+                        cn_seg = R;
+                }
+                if (cn_tl && !cn_seg && (ny + cn_nthr / cn_tl - 1) / (cn_nthr / cn_tl) > CN_RMAX) {
+                    set_error("CN tile %d:%d leaves more than %d rows per thread", cn_tl, cn_nthr, CN_RMAX);
+                    return DNS_EUNSUPPORTED;
+                }
             }
         }
         if (ldsz > (size_t)maxlds || ldsy > (size_t)maxlds) {
@@ -660,6 +682,15 @@ template <typename T> struct FVImpl : FVBase {
         return DNS_OK;
     }
 
+    // equal-segment CN kernel for component c with R rows per thread
+    template <int R> void cn_seg_launch(int c, dim3 grd, dim3 blk, const CNArgs<T> &a)
+    {
+        const size_t lds = ldscn + (size_t)g.ny * sizeof(double);
+        if (c == 0) hipLaunchKernelGGL((k_cn_seg<T, 0, R>), grd, blk, lds, st, a, g, cn_tl);
+        else if (c == 1) hipLaunchKernelGGL((k_cn_seg<T, 1, R>), grd, blk, lds, st, a, g, cn_tl);
+        else hipLaunchKernelGGL((k_cn_seg<T, 2, R>), grd, blk, lds, st, a, g, cn_tl);
+    }
+
     int launch_cn(double B, double alpha, double dt)
     {
         const double theta = alpha * dt * visc / 2;
@@ -684,7 +715,15 @@ template <typename T> struct FVImpl : FVBase {
             a.theta = theta;
             a.rdx = 1.0 / dx;
             a.rdz = 1.0 / dz;
-            if (cn_tl) {
+            if (cn_tl && cn_seg) {
+                const dim3 grd(ceil_div(g.nx, cn_tl), g.nz), blk(cn_nthr);
+                switch (cn_seg) {
+                    case 8: cn_seg_launch<8>(c, grd, blk, a); break;
+                    case 16: cn_seg_launch<16>(c, grd, blk, a); break;
+                    case 24: cn_seg_launch<24>(c, grd, blk, a); break;
+                    default: cn_seg_launch<32>(c, grd, blk, a); break;
+                }
+            } else if (cn_tl) {
                 dim3 grd(ceil_div(g.nx, cn_tl), g.nz);
                 const dim3 blk(cn_nthr);
                 if (ceil_div(g.ny, cn_nthr / cn_tl) <= CN_RMAX / 2) {

@@ -1152,6 +1152,146 @@ __global__ void __launch_bounds__(512) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
     }
 }
 
+// k_cn_tile for the common case of equal segments: every line's ny rows
+// split into S = blockDim/TL segments of exactly R rows (ny = S·R), so no
+// row needs a bounds test.  All loads of a chunk of CH rows are issued
+// unconditionally (the row above/below a line end is clamped and its
+// coefficient is zero), which lets them all be in flight together; the
+// segment's rows stay in registers through both sweeps as in k_cn_tile.
+// Loads go through buffer resources (one offset register per array for all
+// rows) and the register budget is two blocks per CU (256 VGPRs: R = 24
+// FP64 rows without spills; three blocks per CU spill and measured slower).
+// grid (ceil(nx / TL), nz); LDS 7 ny + 3 blockDim doubles.
+template <typename T, int COMP, int R>
+__global__ void __launch_bounds__(256, 2) k_cn_seg(CNArgs<T> a, FVGeom g, int TL)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int CH = R % 8 == 0 ? 8 : R;
+    const int ny = g.ny, S = blockDim.x / TL;
+    double *As = reinterpret_cast<double *>(smem_raw), *Bs = As + blockDim.x, *Xin = Bs + blockDim.x;
+    double *ca = Xin + blockDim.x, *cb = ca + ny, *cc = cb + ny, *cf = cc + ny, *ci = cf + ny, *cq = ci + ny;
+    double *cr = cq + ny; // 1/Δyu of the normal rows
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        if (COMP == 1) cr[j] = a.rdyu[j + 1];
+        ca[j] = j == 0 ? 0.0 : a.a[j]; // Dy rows with the couplings past the line ends cut
+        cb[j] = a.b[j];
+        cc[j] = j == ny - 1 ? 0.0 : a.c[j];
+        cf[j] = j == 0 ? 0.0 : a.theta * a.a[j] * a.ip[j];
+        ci[j] = a.ip[j];
+        cq[j] = j == ny - 1 ? 0.0 : -a.cp[j];
+    }
+    const int l = threadIdx.x % TL, sg = threadIdx.x / TL;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int i = bxr * TL + l, k = byr;
+    const bool valid = i < g.nx;
+    const int ii = valid ? i : 0;
+    const int j0 = sg * R;
+    // one z plane per buffer range: rows past the top wall read 0
+    const size_t plane = (size_t)g.nx * ny;
+    const uint32_t pbytes = (uint32_t)(plane * sizeof(T));
+    const int rowb = g.nx * (int)sizeof(T);
+    const bufres ru = make_bufres(a.u + plane * k, pbytes), rq = make_bufres(a.q + plane * k, pbytes);
+    const bufres rp = make_bufres(a.p + plane * k, pbytes);
+    // pressure neighbour: x+1 (same plane), y+1 (next row) or z+1 (next plane)
+    const bufres rpn = COMP == 2 ? make_bufres(a.p + plane * wrapp(k, g.nz), pbytes) : rp;
+    const int vo = (ii + g.nx * j0) * (int)sizeof(T);
+    const int von = COMP == 0 ? (wrapp(ii, g.nx) + g.nx * j0) * (int)sizeof(T) : vo;
+    const int jact = (COMP == 1 && g.wall) ? ny - 1 : ny;
+    const double rdn = COMP == 0 ? a.rdx : a.rdz;
+    double x[R];
+    double um = (double)bload<T>(ru, sg > 0 ? vo - rowb : vo, 0); // coefficient ca[0] = 0 cuts row −1
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+        T ur[CH + 1], qr[CH], pr[CH], pn[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int so = rowb * (c0 + q);
+            ur[q] = bload<T>(ru, vo, so);
+            qr[q] = bload<T>(rq, vo, so);
+            pr[q] = bload<T>(rp, vo, so);
+            pn[q] = bload<T>(rpn, von, COMP == 1 ? so + rowb : so);
+        }
+        ur[CH] = bload<T>(ru, vo, rowb * (c0 + CH));
+        if (c0 == 0) __syncthreads(); // the coefficient rows are staged before first use
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int j = j0 + c0 + q;
+            const double rd = COMP == 1 ? cr[j] : rdn;
+            const double dpdc = ((double)pn[q] - (double)pr[q]) * rd;
+            const double v = j < jact ? a.B * (double)qr[q] - a.adt * dpdc : 0.0;
+            const double u0 = (double)ur[q];
+            double lap = cb[j] * u0;
+            lap += ca[j] * um;
+            lap += cc[j] * (double)ur[q + 1];
+            x[c0 + q] = u0 + v + a.theta * lap;
+            um = u0;
+        }
+    }
+    // forward sweep, local to the segment
+    {
+        double P = 1.0, y = 0.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = j0 + q;
+            y = cf[j] * y + ci[j] * x[q];
+            x[q] = y;
+            P *= cf[j];
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double xi = 0.0;
+        for (int q = 0; q < S; ++q) {
+            Xin[q * TL + l] = xi;
+            xi = As[q * TL + l] * xi + Bs[q * TL + l];
+        }
+    }
+    __syncthreads();
+    {
+        const double xi = Xin[threadIdx.x];
+        double P = 1.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            P *= cf[j0 + q];
+            x[q] += P * xi;
+        }
+    }
+    __syncthreads();
+    // back substitution, local to the segment
+    {
+        double P = 1.0, y = 0.0;
+#pragma unroll
+        for (int q = R - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            y = cq[j] * y + x[q];
+            x[q] = y;
+            P *= cq[j];
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double xi = 0.0;
+        for (int q = S - 1; q >= 0; --q) {
+            Xin[q * TL + l] = xi;
+            xi = As[q * TL + l] * xi + Bs[q * TL + l];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const double xi = Xin[threadIdx.x];
+    double P = 1.0;
+#pragma unroll
+    for (int q = R - 1; q >= 0; --q) {
+        P *= cq[j0 + q];
+        bstore(ru, vo, rowb * q, T(x[q] + P * xi));
+    }
+}
+
 // Pivots of I − θ tridiag(a, b, c) for the tangential (set 0) and normal
 // (set 1) rows: out[set][0][j] = c'_j, out[set][1][j] = 1/β_j.
 __global__ void __launch_bounds__(256) k_cn_coeffs(const double *__restrict__ at, const double *__restrict__ bt,

@@ -92,6 +92,30 @@ def test_step_parity(name):
     assert dns.maxdiv(ug, s) < 1e-12
 
 
+@pytest.mark.parametrize("n", [(32, 128, 8), (16, 256, 4), (16, 24, 4)])
+@pytest.mark.parametrize("variant", [{}, {"DNS_CN": "tile"}, {"DNS_CN": "sequential"}])
+def test_cn_variants_step_parity(n, variant, monkeypatch):
+    """Every Crank–Nicolson y-diffusion kernel (equal segments, general
+    segments, one thread per line) against the oracle, on line lengths that
+    split into equal register segments (128, 256 rows) and one that does not."""
+    for key, val in variant.items():
+        monkeypatch.setenv(key, val)
+    prob = channelproblem(n=n, visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit")
+    s, o = pair(prob)
+    u = randomfield(prob, seed=21, halo=o.exchange_halo_u)
+    o.project(u, zeros(prob))
+    ug = [a.copy(order="F") for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    qo, qg = vzeros(prob), vzeros(prob)
+    do, dg = zeros(prob), zeros(prob)
+    for _ in range(2):
+        o.step(u, po, qo, do, 0.004)
+        dns.step_(ug, pg, qg, dg, s, 0.004)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11, c
+    assert relmax(pg, po) < 1e-10
+
+
 @pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
 def test_plain_step_parity(name):
     """The reference-structured kernel sequence (the measurement baseline)
@@ -203,9 +227,10 @@ def test_channel_cfl_trajectory_parity():
         assert relmax(ug[c], u[c]) < 1e-10
 
 
-def test_mixed_precision_error_is_small():
-    prob = channelproblem(n=(32, 24, 16), visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit")
-    prob32 = channelproblem(n=(32, 24, 16), visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit",
+@pytest.mark.parametrize("n", [(32, 24, 16), (32, 128, 8)])
+def test_mixed_precision_error_is_small(n):
+    prob = channelproblem(n=n, visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit")
+    prob32 = channelproblem(n=n, visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit",
                             precision="mixed")
     s64, o = pair(prob)
     s32 = Setup(prob32, ROCmBackend())
