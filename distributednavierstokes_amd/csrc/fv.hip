@@ -94,7 +94,7 @@ template <typename T> struct FVImpl : FVBase {
     int N = 0;
     int nb_pgrad = 0;
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
-    int momentum_kind = 3; // 3 slab-ordered (k_momentum_slab), 0 column-ordered, 1 direct, 2 LDS-tiled z march
+    int momentum_kind = 5; // 5 LDS z-march (k_momentum_lz), 3 slab-ordered, 0 column-ordered, 1 direct, 2 LDS-tiled, 4 z-march in registers
     // reference-structured ("plain") step: extra arrays, allocated on first use
     bool plain = false;
     T *pr = nullptr, *pwork = nullptr, *pghost = nullptr, *pyb[3] = {};
@@ -128,14 +128,17 @@ template <typename T> struct FVImpl : FVBase {
         g.wall = P->ywall ? 1 : 0;
         imexy = P->ydiffusion ? 1 : 0;
         {
-            // DNS_MOMENTUM selects the stencil kernel: slab (default), zm, col, direct, tiled
+            // DNS_MOMENTUM selects the stencil kernel: lz (default), slab, zm, col, direct, tiled
             const char *e = getenv("DNS_MOMENTUM");
-            momentum_kind = !e                       ? 3
+            momentum_kind = !e                       ? 5
                             : !strcmp(e, "direct") ? 1
                             : !strcmp(e, "tiled")  ? 2
                             : !strcmp(e, "col")    ? 0
                             : !strcmp(e, "zm")     ? 4
-                                                   : 3;
+                            : !strcmp(e, "slab")   ? 3
+                                                   : 5;
+            // the LDS z-march loads whole rows as 16-byte vectors
+            if (momentum_kind == 5 && nx % (16 / (int)sizeof(T)) != 0) momentum_kind = 3;
         }
         visc = P->visc;
         for (int c = 0; c < 3; ++c) f[c] = P->bodyforce[c];
@@ -516,6 +519,22 @@ template <typename T> struct FVImpl : FVBase {
         a.rdz2 = (T)(1.0 / (dz * dz));
         timer.begin(FAM_MOM);
         const bool rq = Ac != 0.0;
+        if (momentum_kind == 5) {
+            // z chunks: about four resident-block waves of blocks over the chip
+            const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 8);
+            const int zc = std::max(2, std::min(32, (int)((long)g.nz * nbx * nby / 4096)));
+            dim3 grd(nbx, nby, ceil_div(g.nz, zc));
+            if (write_v) {
+                if (rq) hipLaunchKernelGGL((k_momentum_lz<T, true, true>), grd, dim3(256), 0, st, a, g, zc);
+                else hipLaunchKernelGGL((k_momentum_lz<T, false, true>), grd, dim3(256), 0, st, a, g, zc);
+            } else {
+                if (rq) hipLaunchKernelGGL((k_momentum_lz<T, true, false>), grd, dim3(256), 0, st, a, g, zc);
+                else hipLaunchKernelGGL((k_momentum_lz<T, false, false>), grd, dim3(256), 0, st, a, g, zc);
+            }
+            timer.end(FAM_MOM);
+            DNS_HIP_CHECK(hipGetLastError());
+            return DNS_OK;
+        }
         if (momentum_kind == 4) {
             constexpr int RY = 2, ZC = 16;
             dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4 * RY), ceil_div(g.nz, ZC));

@@ -13,6 +13,8 @@
 // row is the top wall face and always holds zero.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 // Loads kept in flight per thread by the HBM→LDS staging loops (capped by the
@@ -96,14 +98,27 @@ template <typename T> __device__ inline void st_stream(T v, T *p, bool nt)
     else *p = v;
 }
 
-template <typename T, bool READ_Q, bool WRITE_V, bool NT = false>
-__device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, int c, int j, int jj, T ux000,
-                                     T uxp00, T uxm00, T ux0p0, T ux0m0, T ux00p, T ux00m, T uxmp0, T uxm0p, T uy000,
-                                     T uyp00, T uym00, T uy0p0, T uy0m0, T uy00p, T uy00m, T uypm0, T uy0mp, T uz000,
-                                     T uzp00, T uzm00, T uz0p0, T uz0m0, T uz00p, T uz00m, T uzp0m, T uz0pm)
+// Metrics of row j (ghosted index jj = j + 1): Δy at jj and jj+1, their
+// reciprocals, 1/Δyu at jj and jj−1.
+template <typename T> struct RowMetrics {
+    T Dyj, Dyp, rDyj, rDyp, rDyuj, rDyum;
+};
+template <typename T> __device__ inline RowMetrics<T> row_metrics(const MomentumArgs<T> &a, int jj)
 {
-    const T Dyj = a.dy[jj], Dyp = a.dy[jj + 1];
-    const T rDyj = a.rdy[jj], rDyp = a.rdy[jj + 1], rDyuj = a.rdyu[jj], rDyum = a.rdyu[jj - 1];
+    return {a.dy[jj], a.dy[jj + 1], a.rdy[jj], a.rdy[jj + 1], a.rdyu[jj], a.rdyu[jj - 1]};
+}
+
+// The update itself with the row metrics and the old q values (qo*, read
+// only when READ_Q) supplied by the caller.
+template <typename T, bool READ_Q, bool WRITE_V, bool NT>
+__device__ inline void momentum_core(const MomentumArgs<T> &a, const FVGeom &g, int c, int j, const RowMetrics<T> &m,
+                                     T qox, T qoy, T qoz, T ux000, T uxp00, T uxm00, T ux0p0, T ux0m0, T ux00p,
+                                     T ux00m, T uxmp0, T uxm0p, T uy000, T uyp00, T uym00, T uy0p0, T uy0m0, T uy00p,
+                                     T uy00m, T uypm0, T uy0mp, T uz000, T uzp00, T uzm00, T uz0p0, T uz0m0, T uz00p,
+                                     T uz00m, T uzp0m, T uz0pm)
+{
+    const T Dyj = m.Dyj, Dyp = m.Dyp;
+    const T rDyj = m.rDyj, rDyp = m.rDyp, rDyuj = m.rDyuj, rDyum = m.rDyum;
     const T h = T(0.5);
 
     // x momentum at the right x-face of the cell
@@ -118,7 +133,7 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
                  a.epsy * ((ux0p0 - ux000) * rDyuj - (ux000 - ux0m0) * rDyum) * rDyj +
                  (ux00p - T(2) * ux000 + ux00m) * a.rdz2;
         T q = a.dt * (-conv + a.nu * diff + a.fx);
-        if (READ_Q) q += a.A * ld_stream(a.qx + c, NT);
+        if (READ_Q) q += a.A * qox;
         st_stream(q, a.qx + c, NT);
         if (WRITE_V) st_stream(ux000 + a.B * q, a.vx + c, NT);
     }
@@ -138,7 +153,7 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
                      a.epsy * ((uy0p0 - uy000) * rDyp - (uy000 - uy0m0) * rDyj) * rDyuj +
                      (uy00p - T(2) * uy000 + uy00m) * a.rdz2;
             T q = a.dt * (-conv + a.nu * diff + a.fy);
-            if (READ_Q) q += a.A * ld_stream(a.qy + c, NT);
+            if (READ_Q) q += a.A * qoy;
             st_stream(q, a.qy + c, NT);
             if (WRITE_V) st_stream(uy000 + a.B * q, a.vy + c, NT);
         } else if (WRITE_V) {
@@ -157,10 +172,25 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
                  a.epsy * ((uz0p0 - uz000) * rDyuj - (uz000 - uz0m0) * rDyum) * rDyj +
                  (uz00p - T(2) * uz000 + uz00m) * a.rdz2;
         T q = a.dt * (-conv + a.nu * diff + a.fz);
-        if (READ_Q) q += a.A * ld_stream(a.qz + c, NT);
+        if (READ_Q) q += a.A * qoz;
         st_stream(q, a.qz + c, NT);
         if (WRITE_V) st_stream(uz000 + a.B * q, a.vz + c, NT);
     }
+}
+
+template <typename T, bool READ_Q, bool WRITE_V, bool NT = false>
+__device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, int c, int j, int jj, T ux000,
+                                     T uxp00, T uxm00, T ux0p0, T ux0m0, T ux00p, T ux00m, T uxmp0, T uxm0p, T uy000,
+                                     T uyp00, T uym00, T uy0p0, T uy0m0, T uy00p, T uy00m, T uypm0, T uy0mp, T uz000,
+                                     T uzp00, T uzm00, T uz0p0, T uz0m0, T uz00p, T uz00m, T uzp0m, T uz0pm)
+{
+    const T qox = READ_Q ? ld_stream(a.qx + c, NT) : T(0);
+    const T qoy = READ_Q && !(g.wall && j == g.ny - 1) ? ld_stream(a.qy + c, NT) : T(0);
+    const T qoz = READ_Q ? ld_stream(a.qz + c, NT) : T(0);
+    momentum_core<T, READ_Q, WRITE_V, NT>(a, g, c, j, row_metrics(a, jj), qox, qoy, qoz, ux000, uxp00, uxm00, ux0p0,
+                                          ux0m0, ux00p, ux00m, uxmp0, uxm0p, uy000, uyp00, uym00, uy0p0, uy0m0, uy00p,
+                                          uy00m, uypm0, uy0mp, uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m,
+                                          uz0pm);
 }
 
 template <typename T, bool READ_Q, bool WRITE_V>
@@ -391,6 +421,195 @@ __global__ void __launch_bounds__(256) k_momentum_col(MomentumArgs<T> a, FVGeom 
                                           uym[r], uyc[r + 2], uyc[r], uykp[r + 1], uykm[r], uyp[r], uykp[r],
                                           uzc[r + 1], uzp[r], uzm[r], uzc[r + 2], uzc[r], uzkp[r], uzkm[r], uzpkm[r],
                                           uzkm[r + 1]);
+    }
+}
+
+// LDS z-march with staged plane loads: a block owns a 64 × 8 (x, y) tile
+// and marches over `zc` planes, keeping planes k−1, k, k+1 of the three
+// components (with a one-cell halo) in LDS.  The 26 neighbours come from
+// LDS, so a cell costs ~3 vector-memory instructions for u instead of 27.
+// Plane loads are 16-byte vector loads of whole tile rows (plus one scalar
+// load per halo column); each thread's rows, ghost-row sources and signs
+// are fixed for the whole march, so a plane costs a few adds per load.  The
+// loads of plane k+2 are issued before plane k is computed and land in LDS
+// after it (one plane in flight per thread), and the old q values are read
+// before them.  Rows outside the wall follow the ldc/ldf ghost rules.
+// Grid (ceil(nx/64), ceil(ny/8), ceil(nz/zc)) of 256 threads, XCD-ordered;
+// LDS 3 planes × 3 components × 10 rows × (64 + 2V) + metric rows.
+template <typename T> struct LZ {
+    static constexpr int TX = 64, TY = 8, SY = TY + 2;
+    static constexpr int V = 16 / sizeof(T);          // elements per 16-byte load
+    static constexpr int P = TX + 2 * V;              // LDS row pitch; interior at V.., halos at V−1 and V+w
+    static constexpr int LR = TX / V;                 // vector lanes per tile row
+    static constexpr int NROW = 3 * SY;               // tile rows per plane (three components)
+    static constexpr int RP = 256 / LR;               // tile rows one pass of 256 threads covers
+    static constexpr int NI = (NROW + RP - 1) / RP;   // vector items per thread per plane
+    static constexpr int PL = NROW * P;               // one plane slot
+};
+
+template <typename T, bool READ_Q, bool WRITE_V>
+__global__ void __launch_bounds__(256) k_momentum_lz(MomentumArgs<T> a, FVGeom g, int zc)
+{
+    using L = LZ<T>;
+    using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    __shared__ __align__(16) T sm[3 * L::PL];
+    __shared__ T met[6][L::TY];
+    int bx, by, bz;
+    xcd_block3(bx, by, bz);
+    const int x0 = bx * L::TX, y0 = by * L::TY, k0 = bz * zc;
+    const int kend = min(k0 + zc, g.nz);
+    const int w = min(L::TX, g.nx - x0); // tile width (a multiple of V)
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)g.nx * g.ny;
+    const T *fld[3] = {a.ux, a.uy, a.uz};
+
+    // this thread's vector items: tile row r = r0 + s·RP, lane v (fixed)
+    const int v = tid % L::LR, r0 = tid / L::LR;
+    const bool vlane = v * L::V < w;
+    int voff[L::NI];   // element offset of the source row within a plane (−1: no item)
+    T vsgn[L::NI];     // ghost sign (0 clears the row)
+    int vdst[L::NI];   // LDS offset within a slot
+    const T *vsrc[L::NI];
+    auto ghost = [&](int f, int j, int &js, T &sg) {
+        // source row and sign of tile row j of component f (ldc / ldf rules)
+        sg = T(1);
+        js = j;
+        if (j < 0) {
+            if (g.wall) {
+                js = 0;
+                sg = f == 1 ? T(0) : T(-1);
+            } else js = g.ny - 1;
+        } else if (j >= g.ny) {
+            if (j > g.ny) {
+                js = g.ny - 1;
+                sg = T(0); // past the first ghost row: never read
+            } else if (g.wall) {
+                js = f == 1 ? g.ny - 2 : g.ny - 1;
+                sg = T(-1);
+            } else js = 0;
+        }
+    };
+#pragma unroll
+    for (int s2 = 0; s2 < L::NI; ++s2) {
+        const int r = r0 + s2 * L::RP;
+        voff[s2] = -1;
+        vsgn[s2] = T(0);
+        vdst[s2] = 0;
+        vsrc[s2] = a.ux;
+        if (r < L::NROW && vlane) {
+            const int f = r / L::SY, ly = r - f * L::SY;
+            int js;
+            T sg;
+            ghost(f, y0 - 1 + ly, js, sg);
+            voff[s2] = js * g.nx + x0 + v * L::V;
+            vsgn[s2] = sg;
+            vdst[s2] = r * L::P + L::V + v * L::V;
+            vsrc[s2] = fld[f];
+        }
+    }
+    // halo columns: threads 0 .. 2·NROW−1, one (row, side) each
+    const bool hlane = tid < 2 * L::NROW;
+    int hoff = 0, hdst = 0;
+    T hsgn = T(0);
+    const T *hsrc = a.ux;
+    if (hlane) {
+        const int r = tid >> 1, side = tid & 1;
+        const int f = r / L::SY, ly = r - f * L::SY;
+        int js;
+        ghost(f, y0 - 1 + ly, js, hsgn);
+        const int i = side ? (x0 + w == g.nx ? 0 : x0 + w) : (x0 == 0 ? g.nx - 1 : x0 - 1);
+        hoff = js * g.nx + i;
+        hdst = r * L::P + (side ? L::V + w : L::V - 1);
+        hsrc = fld[f];
+    }
+    // row metrics of the tile rows
+    if (tid < L::TY) {
+        const int j = min(y0 + tid, g.ny - 1), jj = j + 1;
+        met[0][tid] = a.dy[jj];
+        met[1][tid] = a.dy[jj + 1];
+        met[2][tid] = a.rdy[jj];
+        met[3][tid] = a.rdy[jj + 1];
+        met[4][tid] = a.rdyu[jj];
+        met[5][tid] = a.rdyu[jj - 1];
+    }
+
+    VT vreg[L::NI];
+    T hreg = T(0);
+    auto issue = [&](int k) {
+        const int kw = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
+        const size_t pk = plane * kw;
+#pragma unroll
+        for (int s2 = 0; s2 < L::NI; ++s2)
+            if (voff[s2] >= 0) vreg[s2] = *reinterpret_cast<const VT *>(vsrc[s2] + pk + voff[s2]);
+        if (hlane) hreg = hsrc[pk + hoff];
+    };
+    auto land = [&](int slot) {
+        T *dst = sm + slot * L::PL;
+#pragma unroll
+        for (int s2 = 0; s2 < L::NI; ++s2) {
+            if (voff[s2] >= 0) {
+                VT x = vreg[s2];
+                T *e = reinterpret_cast<T *>(&x);
+#pragma unroll
+                for (int q = 0; q < L::V; ++q) e[q] = vsgn[s2] == T(0) ? T(0) : vsgn[s2] * e[q];
+                *reinterpret_cast<VT *>(dst + vdst[s2]) = x;
+            }
+        }
+        if (hlane) dst[hdst] = hsgn == T(0) ? T(0) : hsgn * hreg;
+    };
+
+    issue(k0 - 1);
+    land(0);
+    issue(k0);
+    land(1);
+    issue(k0 + 1);
+    land(2);
+    __syncthreads();
+    const int tx = tid & 63, ty = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int sM = 0, s0 = 1, sP = 2;
+    for (int k = k0; k < kend; ++k) {
+        const int i = x0 + tx;
+        // old q values first, then the next plane's loads, then the update
+        T qo[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int j = y0 + ty + 4 * h;
+            const size_t c = (size_t)i + (size_t)g.nx * j + plane * k;
+            qo[h][0] = qo[h][1] = qo[h][2] = T(0);
+            if (READ_Q && tx < w && j < g.ny) {
+                qo[h][0] = ld_stream(a.qx + c, true);
+                if (!(g.wall && j == g.ny - 1)) qo[h][1] = ld_stream(a.qy + c, true);
+                qo[h][2] = ld_stream(a.qz + c, true);
+            }
+        }
+        const bool more = k + 1 < kend;
+        if (more) issue(k + 2);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ly = ty + 4 * h, j = y0 + ly;
+            if (tx < w && j < g.ny) {
+                const int cc = (ly + 1) * L::P + L::V + tx;
+                const T *X0 = sm + s0 * L::PL, *Y0 = X0 + L::SY * L::P, *Z0 = Y0 + L::SY * L::P;
+                const T *XP = sm + sP * L::PL, *YP = XP + L::SY * L::P, *ZP = YP + L::SY * L::P;
+                const T *XM = sm + sM * L::PL, *YM = XM + L::SY * L::P, *ZM = YM + L::SY * L::P;
+                const RowMetrics<T> m{met[0][ly], met[1][ly], met[2][ly], met[3][ly], met[4][ly], met[5][ly]};
+                const int P = L::P;
+                momentum_core<T, READ_Q, WRITE_V, true>(
+                    a, g, IX(i, j, k, g), j, m, qo[h][0], qo[h][1], qo[h][2], X0[cc], X0[cc + 1], X0[cc - 1],
+                    X0[cc + P], X0[cc - P], XP[cc], XM[cc], X0[cc - 1 + P], XP[cc - 1], Y0[cc], Y0[cc + 1],
+                    Y0[cc - 1], Y0[cc + P], Y0[cc - P], YP[cc], YM[cc], Y0[cc + 1 - P], YP[cc - P], Z0[cc],
+                    Z0[cc + 1], Z0[cc - 1], Z0[cc + P], Z0[cc - P], ZP[cc], ZM[cc], ZM[cc + 1], ZM[cc + P]);
+            }
+        }
+        __syncthreads(); // every wave is done with slot sM
+        if (more) {
+            land(sM);
+            const int t = sM;
+            sM = s0;
+            s0 = sP;
+            sP = t;
+        }
+        __syncthreads();
     }
 }
 

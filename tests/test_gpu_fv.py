@@ -57,6 +57,29 @@ def test_momentum_parity(name, A):
         assert relmax(qg[c], qo[c]) < 1e-12, c
 
 
+@pytest.mark.parametrize("kind", ["slab", "lz", "direct"])
+@pytest.mark.parametrize("shape,wall", [((16, 12, 8), True), ((96, 20, 6), True), ((128, 16, 4), False),
+                                        ((64, 9, 3), False)])
+@pytest.mark.parametrize("A", [0.0, -5 / 9])
+def test_momentum_kernels_parity(kind, shape, wall, A, monkeypatch):
+    """Every momentum stencil kernel against the oracle, on tiles that are
+    partial in x and y, with walls and periodic y."""
+    monkeypatch.setenv("DNS_MOMENTUM", kind)
+    if wall:
+        prob = channelproblem(n=shape, visc=0.02, bodyforce=(1.0, 0.0, 0.0))
+    else:
+        prob = FVProblem(n=shape, visc=0.01, bodyforce=(0.3, 0.1, -0.2))
+    s, o = pair(prob)
+    u = randomfield(prob, seed=15, halo=o.exchange_halo_u)
+    q0 = randomfield(prob, seed=16)
+    qg = [a.copy(order="F") for a in q0]
+    qo = [a.copy(order="F") for a in q0]
+    o.momentum(qo, u, A, 0.01)
+    dns.momentum_(qg, u, A, 0.01, s)
+    for c in range(3):
+        assert relmax(qg[c], qo[c]) < 1e-12, c
+
+
 @pytest.mark.parametrize("name", ["periodic", "channel", "case_wall", "case_periodic"])
 def test_projection_parity_and_divergence(name):
     prob = problems()[name]
