@@ -520,9 +520,11 @@ template <typename T> struct FVImpl : FVBase {
         timer.begin(FAM_MOM);
         const bool rq = Ac != 0.0;
         if (momentum_kind == 5) {
-            // z chunks: about four resident-block waves of blocks over the chip
+            // z chunks of up to 32 planes, about 1024 blocks (measured: longer
+            // marches win until the grid stops filling the chip)
             const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 8);
-            const int zc = std::max(2, std::min(32, (int)((long)g.nz * nbx * nby / 4096)));
+            int zc = std::max(2, std::min(32, (int)((long)g.nz * nbx * nby / 1024)));
+            if (const char *ez = getenv("DNS_MOM_ZC")) zc = std::max(1, atoi(ez)); // sweeps
             dim3 grd(nbx, nby, ceil_div(g.nz, zc));
             if (write_v) {
                 if (rq) hipLaunchKernelGGL((k_momentum_lz<T, true, true>), grd, dim3(256), 0, st, a, g, zc);

@@ -431,9 +431,9 @@ __global__ void __launch_bounds__(256) k_momentum_col(MomentumArgs<T> a, FVGeom 
 // Plane loads are 16-byte vector loads of whole tile rows (plus one scalar
 // load per halo column); each thread's rows, ghost-row sources and signs
 // are fixed for the whole march, so a plane costs a few adds per load.  The
-// loads of plane k+2 are issued before plane k is computed and land in LDS
-// after it (one plane in flight per thread), and the old q values are read
-// before them.  Rows outside the wall follow the ldc/ldf ghost rules.
+// loads of plane k+3 are issued before plane k is computed, and plane k+2
+// (issued one plane earlier) lands in LDS after it: two planes in flight per
+// thread.  The old q values are read before the plane loads.  Rows outside the wall follow the ldc/ldf ghost rules.
 // Grid (ceil(nx/64), ceil(ny/8), ceil(nz/zc)) of 256 threads, XCD-ordered;
 // LDS 3 planes × 3 components × 10 rows × (64 + 2V) + metric rows.
 template <typename T> struct LZ {
@@ -533,43 +533,47 @@ __global__ void __launch_bounds__(256) k_momentum_lz(MomentumArgs<T> a, FVGeom g
         met[5][tid] = a.rdyu[jj - 1];
     }
 
-    VT vreg[L::NI];
-    T hreg = T(0);
-    auto issue = [&](int k) {
+    // two register sets alternate: while plane k is computed, planes k+2
+    // (issued one plane earlier) and k+3 are in flight
+    VT vA[L::NI], vB[L::NI];
+    T hA = T(0), hB = T(0);
+    auto issue = [&](int k, VT (&vr)[L::NI], T &hr) {
         const int kw = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
         const size_t pk = plane * kw;
 #pragma unroll
         for (int s2 = 0; s2 < L::NI; ++s2)
-            if (voff[s2] >= 0) vreg[s2] = *reinterpret_cast<const VT *>(vsrc[s2] + pk + voff[s2]);
-        if (hlane) hreg = hsrc[pk + hoff];
+            if (voff[s2] >= 0) vr[s2] = *reinterpret_cast<const VT *>(vsrc[s2] + pk + voff[s2]);
+        if (hlane) hr = hsrc[pk + hoff];
     };
-    auto land = [&](int slot) {
+    auto land = [&](int slot, VT (&vr)[L::NI], T hr) {
         T *dst = sm + slot * L::PL;
 #pragma unroll
         for (int s2 = 0; s2 < L::NI; ++s2) {
             if (voff[s2] >= 0) {
-                VT x = vreg[s2];
+                VT x = vr[s2];
                 T *e = reinterpret_cast<T *>(&x);
 #pragma unroll
                 for (int q = 0; q < L::V; ++q) e[q] = vsgn[s2] == T(0) ? T(0) : vsgn[s2] * e[q];
                 *reinterpret_cast<VT *>(dst + vdst[s2]) = x;
             }
         }
-        if (hlane) dst[hdst] = hsgn == T(0) ? T(0) : hsgn * hreg;
+        if (hlane) dst[hdst] = hsgn == T(0) ? T(0) : hsgn * hr;
     };
 
-    issue(k0 - 1);
-    land(0);
-    issue(k0);
-    land(1);
-    issue(k0 + 1);
-    land(2);
+    issue(k0 - 1, vA, hA);
+    land(0, vA, hA);
+    issue(k0, vA, hA);
+    land(1, vA, hA);
+    issue(k0 + 1, vA, hA);
+    land(2, vA, hA);
+    if (k0 + 1 < kend) issue(k0 + 2, vA, hA);
     __syncthreads();
     const int tx = tid & 63, ty = __builtin_amdgcn_readfirstlane(tid >> 6);
     int sM = 0, s0 = 1, sP = 2;
-    for (int k = k0; k < kend; ++k) {
+    // one plane: old q values, the loads of plane k+3, the update of plane k,
+    // then plane k+2 (in cur) lands in the slot plane k−1 leaves
+    auto plane_step = [&](int k, VT (&cur)[L::NI], T &curh, VT (&nxt)[L::NI], T &nxth) {
         const int i = x0 + tx;
-        // old q values first, then the next plane's loads, then the update
         T qo[2][3];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -582,8 +586,7 @@ __global__ void __launch_bounds__(256) k_momentum_lz(MomentumArgs<T> a, FVGeom g
                 qo[h][2] = ld_stream(a.qz + c, true);
             }
         }
-        const bool more = k + 1 < kend;
-        if (more) issue(k + 2);
+        if (k + 2 < kend) issue(k + 3, nxt, nxth);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int ly = ty + 4 * h, j = y0 + ly;
@@ -602,14 +605,18 @@ __global__ void __launch_bounds__(256) k_momentum_lz(MomentumArgs<T> a, FVGeom g
             }
         }
         __syncthreads(); // every wave is done with slot sM
-        if (more) {
-            land(sM);
+        if (k + 1 < kend) {
+            land(sM, cur, curh);
             const int t = sM;
             sM = s0;
             s0 = sP;
             sP = t;
         }
         __syncthreads();
+    };
+    for (int k = k0; k < kend; k += 2) {
+        plane_step(k, vA, hA, vB, hB);
+        if (k + 1 < kend) plane_step(k + 1, vB, hB, vA, hA);
     }
 }
 
