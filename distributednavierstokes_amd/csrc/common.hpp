@@ -172,6 +172,24 @@ template <> __device__ inline double bload<double>(bufres r, int voff, int soff)
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+template <> __device__ inline float2 bload<float2>(bufres r, int voff, int soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ inline double2 bload<double2>(bufres r, int voff, int soff)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ inline void bstore(bufres r, int voff, int soff, float2 v)
+{
+    using u2 = __attribute__((ext_vector_type(2))) unsigned int;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+}
+__device__ inline void bstore(bufres r, int voff, int soff, double2 v)
+{
+    using u4 = __attribute__((ext_vector_type(4))) unsigned int;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, voff, soff, 0);
+}
 __device__ inline void bstore(bufres r, int voff, int soff, float v)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);

@@ -74,6 +74,7 @@ template <typename T> struct FVImpl : FVBase {
     cplx<T> *A = nullptr;
     double2 *Dth = nullptr;
     int thomas_tl = 0, thomas_nthr = 256; // lines and threads per block of the LDS Thomas kernel (0: sequential)
+    int thomas_seg = 0;                   // rows per segment of the equal-segment Thomas kernel (0: not used)
     size_t ldst = 0;
     int cn_tl = 0, cn_nthr = 256; // x-lines and threads per block of the LDS CN kernel (0: sequential)
     int cn_seg = 0;               // rows per segment of the equal-segment CN kernel (0: not used)
@@ -338,6 +339,12 @@ template <typename T> struct FVImpl : FVBase {
                       thomas_tl, thomas_nthr, ldst);
             if (env && !strcmp(env, "sequential")) thomas_tl = 0;
             if (!thomas_tl) DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
+            // equal segments of R rows (ny = S·R) use k_thomas_seg; DNS_THOMAS=tile keeps k_thomas_tile
+            thomas_seg = 0;
+            if (thomas_tl && thomas_nthr == 256 && !(env && !strcmp(env, "tile"))) {
+                const int S = thomas_nthr / thomas_tl, R = ny / S;
+                if (ny % S == 0 && (R == 4 || R == 6 || R == 8 || R == 12 || R == 16 || R == 24)) thomas_seg = R;
+            }
             if (imexy && (size_t)6 * ny * sizeof(double) > (size_t)maxlds) {
                 set_error("unsupported grid: implicit y-diffusion coefficients need %zu bytes of LDS",
                           (size_t)6 * ny * sizeof(double));
@@ -520,10 +527,14 @@ template <typename T> struct FVImpl : FVBase {
         timer.begin(FAM_MOM);
         const bool rq = Ac != 0.0;
         if (momentum_kind == 5) {
-            // z chunks of up to 32 planes, about 1024 blocks (measured: longer
-            // marches win until the grid stops filling the chip)
+            // z chunks: the longest power-of-two march up to 32 planes that still
+            // gives one full round of resident blocks (3 per CU in FP64, 6 in FP32)
+            // over 256 CUs (measured: longer marches win until the grid stops
+            // filling the chip; a partial last round costs a whole round)
             const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 8);
-            int zc = std::max(2, std::min(32, (int)((long)g.nz * nbx * nby / 1024)));
+            const long round = 256L * (sizeof(T) == 8 ? 3 : 6);
+            int zc = 32;
+            while (zc > 2 && (long)nbx * nby * ceil_div(g.nz, zc) < round) zc /= 2;
             if (const char *ez = getenv("DNS_MOM_ZC")) zc = std::max(1, atoi(ez)); // sweeps
             dim3 grd(nbx, nby, ceil_div(g.nz, zc));
             if (write_v) {
@@ -640,7 +651,24 @@ template <typename T> struct FVImpl : FVBase {
         timer.begin(FAM_Y);
         if (g.wall) {
             double scale = 1.0 / ((double)g.nx * g.nz);
-            if (thomas_tl) {
+            if (thomas_tl && thomas_seg) {
+                const dim3 grd(ceil_div(g.nxc, thomas_tl), g.nz), blk(thomas_nthr);
+#define DNS_THOMAS_SEG(R)                                                                                         \
+    case R:                                                                                                       \
+        hipLaunchKernelGGL((k_thomas_seg<T, R>), grd, blk, ldst, st, A, iptab, ay, cy, lam1, lam3, scale, thomas_tl, \
+                           g);                                                                                    \
+        break;
+                switch (thomas_seg) {
+                    DNS_THOMAS_SEG(4)
+                    DNS_THOMAS_SEG(6)
+                    DNS_THOMAS_SEG(8)
+                    DNS_THOMAS_SEG(12)
+                    DNS_THOMAS_SEG(16)
+                    DNS_THOMAS_SEG(24)
+                    default: break;
+                }
+#undef DNS_THOMAS_SEG
+            } else if (thomas_tl) {
                 const dim3 grd(ceil_div(g.nxc, thomas_tl), g.nz), blk(thomas_nthr);
                 const int rows = ceil_div(g.ny, thomas_nthr / thomas_tl);
                 if (rows <= THOMAS_RMAX / 2)

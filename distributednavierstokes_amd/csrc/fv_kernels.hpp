@@ -723,15 +723,23 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
                 int j = L % g.ny, k = L / g.ny;
                 int i0 = 2 * t, i1 = 2 * t + 1;
                 if (MODE == 0) {
-                    int km = wrapm(k, g.nz), jj = j + 1;
-                    T rdyj = a.rdy[jj];
-                    T x0 = a.ux[IX(i0, j, k, g)], x1 = a.ux[IX(i1, j, k, g)], xm = a.ux[IX(wrapm(i0, g.nx), j, k, g)];
-                    T y0 = ldf(a.uy, i0, jj, k, g) - ldf(a.uy, i0, jj - 1, k, g);
-                    T y1 = ldf(a.uy, i1, jj, k, g) - ldf(a.uy, i1, jj - 1, k, g);
-                    T z0 = a.uz[IX(i0, j, k, g)] - a.uz[IX(i0, j, km, g)];
-                    T z1 = a.uz[IX(i1, j, k, g)] - a.uz[IX(i1, j, km, g)];
-                    z.x = (x0 - xm) * a.rdx + y0 * rdyj + z0 * a.rdz;
-                    z.y = (x1 - x0) * a.rdx + y1 * rdyj + z1 * a.rdz;
+                    // cells i0, i1 = i0 + 1 as one pair load per row (i0 even)
+                    const int km = wrapm(k, g.nz), jj = j + 1;
+                    const T rdyj = a.rdy[jj];
+                    const int c0 = IX(i0, j, k, g);
+                    const cplx<T> X = *reinterpret_cast<const cplx<T> *>(a.ux + c0);
+                    const T xm = a.ux[IX(wrapm(i0, g.nx), j, k, g)];
+                    const cplx<T> Y0 = *reinterpret_cast<const cplx<T> *>(a.uy + c0);
+                    // face below: j − 1, the wall (0) or the periodic image ny − 1
+                    cplx<T> Ym = mk<T>(T(0), T(0));
+                    if (j > 0) Ym = *reinterpret_cast<const cplx<T> *>(a.uy + c0 - g.nx);
+                    else if (!g.wall) Ym = *reinterpret_cast<const cplx<T> *>(a.uy + IX(i0, g.ny - 1, k, g));
+                    const cplx<T> Z0 = *reinterpret_cast<const cplx<T> *>(a.uz + c0);
+                    const cplx<T> Zm = *reinterpret_cast<const cplx<T> *>(a.uz + IX(i0, j, km, g));
+                    const T y0 = Y0.x - Ym.x, y1 = Y0.y - Ym.y;
+                    const T z0 = Z0.x - Zm.x, z1 = Z0.y - Zm.y;
+                    z.x = (X.x - xm) * a.rdx + y0 * rdyj + z0 * a.rdz;
+                    z.y = (X.y - X.x) * a.rdx + y1 * rdyj + z1 * a.rdz;
                 } else {
                     z.x = a.ux[IX(i0, j, k, g)];
                     z.y = a.ux[IX(i1, j, k, g)];
@@ -1105,6 +1113,130 @@ __global__ void __launch_bounds__(512) k_thomas_tile(cplx<T> *__restrict__ A, co
             P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
             if (valid) Al[(size_t)g.ldc * j] = mk<T>(T(x[q].x + P * xi.x), T(x[q].y + P * xi.y));
         }
+    }
+}
+
+// k_thomas_tile for equal segments (ny = S·R): no row needs a bounds test,
+// and the rows of r and of the pivot table are read through buffer
+// resources (one offset register per array for all R rows, a scalar offset
+// per row), all issued before the first use.  Register budget: two blocks
+// of 256 threads per CU.  Same recurrences, pivots and mean-mode rule.
+// grid (ceil(nxc / TL), nz); LDS 2 ny + 5 blockDim doubles.
+template <typename T, int R>
+__global__ void __launch_bounds__(256, 2) k_thomas_seg(cplx<T> *__restrict__ A, const double *__restrict__ ip,
+                                                       const double *__restrict__ ay, const double *__restrict__ cy,
+                                                       const double *__restrict__ lam1, const double *__restrict__ lam3,
+                                                       double scale, int TL, FVGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int ny = g.ny, S = blockDim.x / TL;
+    double2 *Bs = reinterpret_cast<double2 *>(smem_raw); // [S][TL] segment offsets
+    double2 *Xin = Bs + blockDim.x;                      // [S][TL] segment inflows
+    double *As = reinterpret_cast<double *>(Xin + blockDim.x);
+    double *sa = As + blockDim.x, *sc = sa + ny;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int kx0 = bxr * TL, kz = byr;
+    const int kzs = kz <= g.nz - kz ? kz : g.nz - kz;
+    const int l = threadIdx.x % TL, sg = threadIdx.x / TL, kx = kx0 + l;
+    const bool valid = kx < g.nxc;
+    const int kxl = valid ? kx : 0;
+    const int j0 = sg * R;
+    // one kz plane of A and one |kz| plane of the pivot table per buffer range
+    const bufres ra = make_bufres(A + (size_t)g.ldc * ny * kz, (uint32_t)((size_t)g.ldc * ny * sizeof(cplx<T>)));
+    const bufres rw = make_bufres(ip + (size_t)g.nxc * ny * kzs, (uint32_t)((size_t)g.nxc * ny * sizeof(double)));
+    const int arow = g.ldc * (int)sizeof(cplx<T>), wrow = g.nxc * (int)sizeof(double);
+    const int va = (kxl + g.ldc * j0) * (int)sizeof(cplx<T>), vw = (kxl + g.nxc * j0) * (int)sizeof(double);
+    double2 x[R];
+    double w[R];
+    {
+        cplx<T> ar[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            ar[q] = bload<cplx<T>>(ra, va, arow * q);
+            w[q] = bload<double>(rw, vw, wrow * q);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = make_double2((double)ar[q].x * scale, (double)ar[q].y * scale);
+    }
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        sa[j] = ay[j];
+        sc[j] = cy[j];
+    }
+    const bool mean = (lam1[kxl] + lam3[kz]) == 0.0;
+    __syncthreads();
+    // forward sweep d'_j = −a_j ip_j d'_{j−1} + ip_j r_j (row 0: no coupling; 0 for the mean mode)
+    {
+        double P = 1.0;
+        double2 y = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = j0 + q;
+            const double al = j == 0 ? 0.0 : -sa[j] * w[q];
+            const double be = (j == 0 && mean) ? 0.0 : w[q];
+            y = make_double2(al * y.x + be * x[q].x, al * y.y + be * x[q].y);
+            x[q] = y;
+            P *= al;
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double2 xi = make_double2(0.0, 0.0);
+        for (int q = 0; q < S; ++q) {
+            Xin[q * TL + l] = xi;
+            const double a = As[q * TL + l];
+            const double2 b = Bs[q * TL + l];
+            xi = make_double2(a * xi.x + b.x, a * xi.y + b.y);
+        }
+    }
+    __syncthreads();
+    {
+        const double2 xi = Xin[threadIdx.x];
+        double P = 1.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = j0 + q;
+            P *= j == 0 ? 0.0 : -sa[j] * w[q];
+            x[q] = make_double2(x[q].x + P * xi.x, x[q].y + P * xi.y);
+        }
+    }
+    __syncthreads(); // the segment maps of the forward chain are read; reuse As/Bs
+    // back substitution x_j = d'_j − c'_j x_{j+1}, c'_j = c_j ip_j (0 on the last row and row 0 of the mean mode)
+    {
+        double P = 1.0;
+        double2 y = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = R - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            const double al = (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
+            y = make_double2(al * y.x + x[q].x, al * y.y + x[q].y);
+            x[q] = y;
+            P *= al;
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double2 xi = make_double2(0.0, 0.0);
+        for (int q = S - 1; q >= 0; --q) {
+            Xin[q * TL + l] = xi;
+            const double a = As[q * TL + l];
+            const double2 b = Bs[q * TL + l];
+            xi = make_double2(a * xi.x + b.x, a * xi.y + b.y);
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const double2 xi = Xin[threadIdx.x];
+    double P = 1.0;
+#pragma unroll
+    for (int q = R - 1; q >= 0; --q) {
+        const int j = j0 + q;
+        P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
+        bstore(ra, va, arow * q, mk<T>(T(x[q].x + P * xi.x), T(x[q].y + P * xi.y)));
     }
 }
 

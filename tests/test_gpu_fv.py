@@ -80,6 +80,27 @@ def test_momentum_kernels_parity(kind, shape, wall, A, monkeypatch):
         assert relmax(qg[c], qo[c]) < 1e-12, c
 
 
+@pytest.mark.parametrize("n", [(32, 128, 8), (16, 256, 4), (16, 24, 4)])
+@pytest.mark.parametrize("variant", ["", "tile", "sequential"])
+def test_thomas_variants_projection_parity(n, variant, monkeypatch):
+    """Every wall-normal Thomas kernel of the pressure solve (equal segments,
+    general segments, one thread per line) against the oracle."""
+    if variant:
+        monkeypatch.setenv("DNS_THOMAS", variant)
+    prob = channelproblem(n=n, visc=0.02)
+    s, o = pair(prob)
+    u = randomfield(prob, seed=17, halo=o.exchange_halo_u)
+    ug = [a.copy(order="F") for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    o.project(u, po)
+    dns.project_(ug, pg, s)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-12
+    assert relmax(pg, po) < 1e-11
+    # the divergence left is rounding of the same size as the oracle's own
+    assert dns.maxdiv(ug, s) < max(1e-12, 4 * o.maxdiv(u))
+
+
 @pytest.mark.parametrize("name", ["periodic", "channel", "case_wall", "case_periodic"])
 def test_projection_parity_and_divergence(name):
     prob = problems()[name]
