@@ -152,6 +152,26 @@ __device__ __forceinline__ void batched_for(int total, LD ld, ST st)
     }
 }
 
+// Division by a runtime-invariant divisor d ≥ 1 as a multiply-high, add and
+// shift (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, round-up variant): with s = ceil(log2 d) and
+// m = floor(2^32 (2^s − d) / d) + 1, n / d = (mulhi(n, m) + n) >> s for
+// 0 ≤ n < 2^31 (the sum stays below 2^32).  A 32-bit integer division by a
+// runtime value costs some twenty instructions on the GPU.
+struct FastDiv {
+    uint32_t m;
+    int s;
+};
+inline FastDiv make_fastdiv(int d)
+{
+    FastDiv f{1u, 0};
+    if (d <= 1) return f;
+    while ((1u << f.s) < (uint32_t)d) ++f.s;
+    f.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << f.s) - (uint64_t)d)) / (uint64_t)d + 1);
+    return f;
+}
+__device__ inline int fdiv(int n, FastDiv f) { return (int)((__umulhi((uint32_t)n, f.m) + (uint32_t)n) >> f.s); }
+
 // Buffer-resource access: a wave-uniform base and byte range, a per-lane
 // 32-bit byte offset (one VGPR for every row a thread reads) and a
 // wave-uniform byte offset per access (an SGPR).  The hardware range check

@@ -27,6 +27,8 @@ struct FFTDesc {
     int n;
     int npass;
     int radix[FFT_MAXPASS];
+    FastDiv dm[FFT_MAXPASS];  // division by m = n / R of each pass
+    FastDiv dns[FFT_MAXPASS]; // division by the sub-transform size Ns of each pass
 };
 
 // Factorises n into the supported radices (largest first: 8, 4, 2, then 3, 5).
@@ -57,6 +59,12 @@ inline bool make_fft_desc(int n, FFTDesc &d)
         m /= 5;
     }
     if (d.npass > FFT_MAXPASS) return false;
+    int ns = 1;
+    for (int p = 0; p < d.npass; ++p) {
+        d.dm[p] = make_fastdiv(n / d.radix[p]);
+        d.dns[p] = make_fastdiv(ns);
+        ns *= d.radix[p];
+    }
     return m == 1;
 }
 
@@ -156,8 +164,8 @@ template <typename T, bool INV> struct Dft<T, 5, INV> {
 // --------------------------------------------------------------- one pass
 
 template <typename T, int R, int EPT, bool INV>
-__device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns, const cplx<T> *__restrict__ tw,
-                                int tid, int nthr)
+__device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns, FastDiv dm, FastDiv dns,
+                                const cplx<T> *__restrict__ tw, int tid, int nthr)
 {
     constexpr int BPT = (EPT + R - 1) / R;
     const int m = n / R;
@@ -167,7 +175,7 @@ __device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns,
     for (int s = 0; s < BPT; ++s) {
         int t = tid + s * nthr;
         if (t < total) {
-            int line = t / m;
+            int line = fdiv(t, dm);
             int j = t - line * m;
             const cplx<T> *src = buf + line * ls + j;
 #pragma unroll
@@ -180,9 +188,9 @@ __device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns,
     for (int s = 0; s < BPT; ++s) {
         int t = tid + s * nthr;
         if (t < total) {
-            int line = t / m;
+            int line = fdiv(t, dm);
             int j = t - line * m;
-            int jd = j / Ns;
+            int jd = fdiv(j, dns);
             int k = j - jd * Ns;
             if (Ns > 1) {
 #pragma unroll
@@ -221,11 +229,11 @@ __device__ inline void fft_lds(cplx<T> *buf, int nlines, int ls, const FFTDesc &
     for (int p = 0; p < d.npass; ++p) {
         int R = d.radix[p];
         switch (R) {
-            case 8: fft_pass<T, 8, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
-            case 4: fft_pass<T, 4, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
-            case 2: fft_pass<T, 2, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
-            case 3: fft_pass<T, 3, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
-            case 5: fft_pass<T, 5, EPT, INV>(buf, nlines, ls, d.n, Ns, tw, tid, nthr); break;
+            case 8: fft_pass<T, 8, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
+            case 4: fft_pass<T, 4, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
+            case 2: fft_pass<T, 2, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
+            case 3: fft_pass<T, 3, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
+            case 5: fft_pass<T, 5, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
             default: break;
         }
         Ns *= R;
