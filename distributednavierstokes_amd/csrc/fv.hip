@@ -638,6 +638,9 @@ template <typename T> struct FVImpl : FVBase {
         xa.tw = twN;
         xa.twx = twx;
         xa.lpb = lpb;
+        xa.fN = make_fastdiv(g.nx / 2);
+        xa.fnxc = make_fastdiv(g.nxc);
+        xa.fny = make_fastdiv(g.ny);
         const int nlines = g.ny * g.nz;
         timer.begin(FAM_X);
         hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 0>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xa, g, fdx);
@@ -645,7 +648,7 @@ template <typename T> struct FVImpl : FVBase {
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
         hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
-                           twz, tbz, tbzf, g, fdz);
+                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Y);
@@ -685,13 +688,13 @@ template <typename T> struct FVImpl : FVBase {
         } else {
             double scale = 1.0 / ((double)g.nx * g.ny * g.nz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
-                               st, A, twy, lam1, lam2, lam3, scale, tby, tbyf, g, fdy);
+                               st, A, twy, lam1, lam2, lam3, scale, tby, tbyf, make_fastdiv(tby), g, fdy);
         }
         timer.end(FAM_Y);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
         hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
-                           twz, tbz, tbzf, g, fdz);
+                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         XInvArgs<T> xi;
@@ -700,6 +703,7 @@ template <typename T> struct FVImpl : FVBase {
         xi.tw = twN;
         xi.twx = twx;
         xi.lpb = lpb;
+        xi.fN = make_fastdiv(g.nx / 2);
         timer.begin(FAM_X);
         hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
         timer.end(FAM_X);
@@ -861,13 +865,16 @@ template <typename T> struct FVImpl : FVBase {
         xa.tw = twN;
         xa.twx = twx;
         xa.lpb = lpb;
+        xa.fN = make_fastdiv(g.nx / 2);
+        xa.fnxc = make_fastdiv(g.nxc);
+        xa.fny = make_fastdiv(g.ny);
         const int nlines = g.ny * g.nz;
         hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xa, g, fdx);
         timer.end(FAM_X);
         timer.begin(FAM_Z);
         plain_copy_pencil(pB, A);
         hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, pB,
-                           twz, tbz, tbzf, g, fdz);
+                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
         plain_copy_pencil(pC, pB);
         timer.end(FAM_Z);
         timer.begin(FAM_Y);
@@ -877,13 +884,13 @@ template <typename T> struct FVImpl : FVBase {
                                lam1, lam3, 1.0 / ((double)g.nx * g.nz), g);
         } else {
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
-                               st, pC, twy, lam1, lam2, lam3, 1.0 / ((double)g.nx * g.ny * g.nz), tby, tbyf, g, fdy);
+                               st, pC, twy, lam1, lam2, lam3, 1.0 / ((double)g.nx * g.ny * g.nz), tby, tbyf, make_fastdiv(tby), g, fdy);
         }
         timer.end(FAM_Y);
         timer.begin(FAM_Z);
         plain_copy_pencil(pB, pC);
         hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, pB,
-                           twz, tbz, tbzf, g, fdz);
+                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
         plain_copy_pencil(A, pB);
         timer.end(FAM_Z);
         timer.begin(FAM_X);
@@ -893,6 +900,7 @@ template <typename T> struct FVImpl : FVBase {
         xi.tw = twN;
         xi.twx = twx;
         xi.lpb = lpb;
+        xi.fN = make_fastdiv(g.nx / 2);
         hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
         plain_copy(psi, pwork); // copybox! into the field interior
         timer.end(FAM_X);
@@ -1103,12 +1111,15 @@ template <typename T> struct FVImpl : FVBase {
             xa.tw = twN;
             xa.twx = twx;
             xa.lpb = lpb;
+        xa.fN = make_fastdiv(g.nx / 2);
+        xa.fnxc = make_fastdiv(g.nxc);
+        xa.fny = make_fastdiv(g.ny);
             hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(g.ny * g.nz, lpb)), dim3(NTHR), ldsx, st, xa, g,
                                fdx);
             hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
-                               twz, tbz, tbzf, g, fdz);
+                               twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
-                               st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, g, fdy);
+                               st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, make_fastdiv(tby), g, fdy);
             hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR), sizeof(double) * nbins, st, A, g, nbins,
                                1.0 / N2, red);
             hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nbins, NTHR)), dim3(NTHR), 0, st, red, nb, nbins, dout);

@@ -702,6 +702,7 @@ template <typename T> struct XPassArgs {
     const cplx<T> *tw;  // length nx/2
     const cplx<T> *twx; // length nx/2+1
     int lpb;
+    FastDiv fN, fnxc, fny; // division by nx/2, nx/2+1 and ny
 };
 
 template <typename T, int EPT, int MODE>
@@ -716,11 +717,11 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         lpb * N,
         [&](int idx) {
-            int b = idx / N, t = idx - b * N;
+            int b = fdiv(idx, a.fN), t = idx - b * N;
             int L = L0 + b;
             cplx<T> z = mk<T>(T(0), T(0));
             if (L < nlines) {
-                int j = L % g.ny, k = L / g.ny;
+                int k = fdiv(L, a.fny), j = L - k * g.ny;
                 int i0 = 2 * t, i1 = 2 * t + 1;
                 if (MODE == 0) {
                     // cells i0, i1 = i0 + 1 as one pair load per row (i0 even)
@@ -752,7 +753,7 @@ __global__ void __launch_bounds__(256) k_rfft_x(XPassArgs<T> a, FVGeom g, FFTDes
     fft_lds<T, EPT, false>(buf, lpb, N, d, twl);
     const int nxc = N + 1;
     for (int idx = threadIdx.x; idx < lpb * nxc; idx += blockDim.x) {
-        int b = idx / nxc, kx = idx - b * nxc;
+        int b = fdiv(idx, a.fnxc), kx = idx - b * nxc;
         int L = L0 + b;
         if (L >= nlines) continue;
         cplx<T> Zk = buf[b * N + (kx == N ? 0 : kx)];
@@ -776,6 +777,7 @@ template <typename T> struct XInvArgs {
     const cplx<T> *tw;  // length nx/2
     const cplx<T> *twx; // length nx/2+1
     int lpb;
+    FastDiv fN; // division by nx/2
 };
 
 template <typename T, int EPT>
@@ -790,7 +792,7 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         lpb * N,
         [&](int idx) {
-            int b = idx / N, k = idx - b * N;
+            int b = fdiv(idx, a.fN), k = idx - b * N;
             int L = L0 + b;
             cplx<T> Z = mk<T>(T(0), T(0));
             if (L < nlines) {
@@ -812,7 +814,7 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
     __syncthreads();
     fft_lds<T, EPT, true>(buf, lpb, N, d, twl);
     for (int idx = threadIdx.x; idx < lpb * N; idx += blockDim.x) {
-        int b = idx / N, t = idx - b * N;
+        int b = fdiv(idx, a.fN), t = idx - b * N;
         int L = L0 + b;
         if (L >= nlines) continue;
         cplx<T> z = buf[idx];
@@ -825,7 +827,8 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
 // consecutive kx at fixed j (coalesced tb·16-byte row segments).
 // ------------------------------------------------------------------------
 template <typename T, int EPT, bool INV>
-__global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, int tbf, FVGeom g, FFTDesc d)
+__global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, FVGeom g,
+                                                  FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -837,17 +840,17 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            int t = idx / tb, kx = kx0 + (idx - t * tb);
+            int t = fdiv(idx, ftb), kx = kx0 + (idx - t * tb);
             return kx < g.nxc ? A[CX(kx, j, t, g)] : mk<T>(T(0), T(0));
         },
         [&](int idx, cplx<T> v) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             buf[b * ls + t] = v;
         });
     __syncthreads();
     for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, INV>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb;
+        int t = fdiv(idx, ftb), b = idx - t * tb;
         int kx = kx0 + b;
         if (kx < g.nxc) A[CX(kx, j, t, g)] = buf[b * ls + t];
     }
@@ -863,7 +866,7 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
 template <typename T, int EPT, bool DIVIDE>
 __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw, const double *lam1,
                                                    const double *lam2, const double *lam3, double scale, int tb,
-                                                   int tbf, FVGeom g, FFTDesc d)
+                                                   int tbf, FastDiv ftb, FVGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -875,18 +878,18 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
     batched_for<FV_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            int t = idx / tb, kx = kx0 + (idx - t * tb);
+            int t = fdiv(idx, ftb), kx = kx0 + (idx - t * tb);
             return kx < g.nxc ? A[CX(kx, t, kz, g)] : mk<T>(T(0), T(0));
         },
         [&](int idx, cplx<T> v) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             buf[b * ls + t] = v;
         });
     __syncthreads();
     for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     if (DIVIDE) {
         for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             int kx = kx0 + b;
             if (kx >= g.nxc) continue;
             double lam = lam1[kx] + lam2[t] + lam3[kz];
@@ -898,7 +901,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
         for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     }
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb;
+        int t = fdiv(idx, ftb), b = idx - t * tb;
         int kx = kx0 + b;
         if (kx < g.nxc) A[CX(kx, t, kz, g)] = buf[b * ls + t];
     }

@@ -203,11 +203,11 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         timer.begin(SFAM_Y);
         hipLaunchKernelGGL((k_sp_yback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st,
-                           src ? src : uh, Q, twn, tb, tbf, g, fdn);
+                           src ? src : uh, Q, twn, tb, tbf, make_fastdiv(tb), g, fdn);
         timer.end(SFAM_Y);
         timer.begin(SFAM_Z);
         hipLaunchKernelGGL((k_sp_zback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, 3), dim3(SP_NTHR), lds, st, Q, P,
-                           twn, tb, tbf, g, fdn);
+                           twn, tb, tbf, make_fastdiv(tb), g, fdn);
         timer.end(SFAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
@@ -224,6 +224,10 @@ template <typename T> struct SpecImpl : SpecBase {
         a.tw = twN;
         a.twx = twx;
         a.lpb = lpb;
+        a.fN = make_fastdiv(g.n / 2);
+        a.fn = make_fastdiv(g.n);
+        a.fK = make_fastdiv(g.K);
+        a.flpb = make_fastdiv(lpb);
         timer.begin(SFAM_X);
         hipLaunchKernelGGL((k_sp_xmid<T, SP_XEPT, MODE>), dim3(ceil_div((long)g.n * g.n, lpb)), dim3(SP_NTHR), ldsx,
                            st, a, g, fdh);
@@ -236,7 +240,7 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         timer.begin(SFAM_Z);
         hipLaunchKernelGGL((k_sp_zfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, ncomp), dim3(SP_NTHR), lds, st, P, Q,
-                           twn, tb, tbf, g, fdn);
+                           twn, tb, tbf, make_fastdiv(tb), g, fdn);
         timer.end(SFAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
@@ -252,6 +256,7 @@ template <typename T> struct SpecImpl : SpecBase {
         s.extra = extra;
         s.tw = twn;
         s.tb = tbs;
+        s.ftb = make_fastdiv(tbs);
         s.first = stage == 0;
         s.a = (T)W_A[stage];
         s.b = (T)W_B[stage];
@@ -295,6 +300,7 @@ template <typename T> struct SpecImpl : SpecBase {
         sa.extra = nullptr;
         sa.tw = twn;
         sa.tb = tbs;
+        sa.ftb = make_fastdiv(tbs);
         sa.first = 1;
         sa.a = sa.b = sa.nudcdt = sa.dt = T(0);
         hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, 2>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR), ldss,
@@ -329,7 +335,7 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(xpass<XMID_R2C>(red));
         DNS_TRY(zfwd(3));
         hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, uh, twn,
-                           tb, tbf, g, fdn);
+                           tb, tbf, make_fastdiv(tb), g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }
@@ -341,7 +347,7 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(xpass<XMID_R2C>(red, src));
         DNS_TRY(zfwd(3));
         hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, dst, twn,
-                           tb, tbf, g, fdn);
+                           tb, tbf, make_fastdiv(tb), g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }

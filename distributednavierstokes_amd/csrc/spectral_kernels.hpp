@@ -51,7 +51,7 @@ __device__ inline int sp_compact(int f, const SpecGeom &g)
 // Backward y pass: B1[i, y, kz, c] = Σ_ky uh[i, ky, kz, c] e^{+i ky y}.
 // grid (ceil(K/tb), m, 3); LDS tb·(n+1) + n (the twiddles).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb, int tbf,
+__global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb, int tbf, FastDiv ftb,
                                                   SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -64,20 +64,20 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
             int jc = sp_compact(t, g);
             cplx<T> v = mk<T>(T(0), T(0));
             if (i < K && jc >= 0) v = uh[i + (size_t)K * (jc + (size_t)m * (kz + (size_t)m * c))];
             return v;
         },
         [&](int idx, cplx<T> v) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             buf[b * ls + t] = v;
         });
     __syncthreads();
     for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
     }
 }
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
 // Backward z pass: P[i, y, z, c] = Σ_kz B1[i, y, kz, c] e^{+i kz z}.
 // grid (ceil(K/tb), n, 3).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, int tbf, SpecGeom g,
+__global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, SpecGeom g,
                                                   FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -98,20 +98,20 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
             int kc = sp_compact(t, g);
             cplx<T> v = mk<T>(T(0), T(0));
             if (i < K && kc >= 0) v = B1[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * c))];
             return v;
         },
         [&](int idx, cplx<T> v) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             buf[b * ls + t] = v;
         });
     __syncthreads();
     for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = idx / tb, b = idx - t * tb, i = i0 + b;
+        int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
     }
 }
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
 // Forward z pass: Q[i, y, kc, p] = Σ_z P[i, y, z, p] e^{−i kz z} at the
 // retained kz.  grid (ceil(K/tb), n, ncomp).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, int tbf, SpecGeom g,
+__global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, SpecGeom g,
                                                  FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -132,17 +132,17 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
             return i < K ? P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * p))] : mk<T>(T(0), T(0));
         },
         [&](int idx, cplx<T> v) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             buf[b * ls + t] = v;
         });
     __syncthreads();
     for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
-        int kc = idx / tb, b = idx - kc * tb, i = i0 + b;
+        int kc = fdiv(idx, ftb), b = idx - kc * tb, i = i0 + b;
         if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
     }
 }
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
 // src/spectral/fft.jl:190-193): out[i, jc, kc, p] (K, m, m, ncomp).
 // grid (ceil(K/tb), m, ncomp).
 template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, int tbf, SpecGeom g,
+__global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, SpecGeom g,
                                                  FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -164,18 +164,18 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            int t = idx / tb, b = idx - t * tb, i = i0 + b;
+            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
             return i < K ? Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
         },
         [&](int idx, cplx<T> v) {
-            int t = idx / tb, b = idx - t * tb;
+            int t = fdiv(idx, ftb), b = idx - t * tb;
             buf[b * ls + t] = v;
         });
     __syncthreads();
     for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
     const T inv = T(1.0 / ((double)n * n * n));
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
-        int jc = idx / tb, b = idx - jc * tb, i = i0 + b;
+        int jc = fdiv(idx, ftb), b = idx - jc * tb, i = i0 + b;
         if (i < K) out[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))] = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
     }
 }
@@ -200,6 +200,7 @@ template <typename T> struct XMidArgs {
     const cplx<T> *tw;  // length n/2
     const cplx<T> *twx; // length n/2 + 1
     int lpb;
+    FastDiv fN, fn, fK, flpb; // division by n/2, n, K and lpb
 };
 
 template <typename T, int EPT, int MODE>
@@ -221,7 +222,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
         batched_for<SP_LOAD_U(EPT), cplx<T>>(
             nw * N,
             [&](int idx) {
-                const int cl = idx / N, k = idx - cl * N, c = cl / lpb, b = cl - c * lpb, L = L0 + b;
+                const int cl = fdiv(idx, a.fN), k = idx - cl * N, c = fdiv(cl, a.flpb), b = cl - c * lpb, L = L0 + b;
                 cplx<T> Z = mk<T>(T(0), T(0));
                 if (L < nlines) {
                     const cplx<T> *row = a.P + (size_t)L * K + c * pstride;
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
         __syncthreads();
         for (int l0 = 0; l0 < nw; l0 += maxl) fft_lds<T, EPT, true>(work + (size_t)l0 * N, min(maxl, nw - l0), N, d, twl);
         for (int idx = threadIdx.x; idx < nw * N; idx += blockDim.x) {
-            const int cl = idx / N, t = idx - cl * N;
+            const int cl = fdiv(idx, a.fN), t = idx - cl * N;
             const cplx<T> z = work[idx];
             T *vl = v + (size_t)cl * n; // v[c][b] = line cl
             vl[2 * t] = z.x;
@@ -253,7 +254,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
             __shared__ double red[256];
             double mx = 0;
             for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
-                int b = idx / n;
+                int b = fdiv(idx, a.fn);
                 if (L0 + b >= nlines) continue;
                 double s = fabs((double)v[idx]) + fabs((double)v[(size_t)lpb * n + idx]) +
                            fabs((double)v[(size_t)2 * lpb * n + idx]);
@@ -270,7 +271,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
         if (MODE == XMID_C2R) {
             for (int c = 0; c < 3; ++c)
                 for (int idx = threadIdx.x; idx < lpb * n; idx += blockDim.x) {
-                    int b = idx / n, x = idx - b * n, L = L0 + b;
+                    int b = fdiv(idx, a.fn), x = idx - b * n, L = L0 + b;
                     if (L < nlines) a.phys[x + (size_t)n * L + (size_t)c * n * nlines] = v[((size_t)c * lpb + b) * n + x];
                 }
         }
@@ -283,7 +284,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
         batched_for<SP_LOAD_U(EPT), cplx<T>>(
             nw * N,
             [&](int idx) {
-                const int cl = idx / N, t = idx - cl * N, pg = cl / lpb, b = cl - pg * lpb, L = L0 + b;
+                const int cl = fdiv(idx, a.fN), t = idx - cl * N, pg = fdiv(cl, a.flpb), b = cl - pg * lpb, L = L0 + b;
                 const int p = 3 * grp + pg;
                 cplx<T> z = mk<T>(T(0), T(0));
                 if (MODE == XMID_RHS) {
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
         __syncthreads();
         for (int l0 = 0; l0 < nw; l0 += maxl) fft_lds<T, EPT, false>(work + (size_t)l0 * N, min(maxl, nw - l0), N, d, twl);
         for (int idx = threadIdx.x; idx < nw * K; idx += blockDim.x) {
-            const int cl = idx / K, kx = idx - cl * K, pg = cl / lpb, b = cl - pg * lpb, L = L0 + b;
+            const int cl = fdiv(idx, a.fK), kx = idx - cl * K, pg = fdiv(cl, a.flpb), b = cl - pg * lpb, L = L0 + b;
             if (L >= nlines) continue;
             const int p = 3 * grp + pg;
             cplx<T> Zk = work[cl * N + (kx == N ? 0 : kx)];
@@ -330,6 +331,7 @@ template <typename T> struct StageArgs {
     const cplx<T> *extra; // (K, m, m, 6) stress added to σ̂ before the divergence, or null
     const cplx<T> *tw;
     int tb;
+    FastDiv ftb; // division by tb
     int first;   // stage 1: the stage start is uh itself
     T a, b, nudcdt, dt;
 };
@@ -340,6 +342,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = n + 1, K = g.K, m = g.m, tb = s.tb;
+    const FastDiv ftb = s.ftb;
     const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
     int bxr, byr;
     xcd_block2(bxr, byr);
@@ -355,11 +358,11 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
         batched_for<SP_LOAD_U(EPT), cplx<T>>(
             tb * n,
             [&](int idx) {
-                int t = idx / tb, i = i0 + (idx - t * tb);
+                int t = fdiv(idx, ftb), i = i0 + (idx - t * tb);
                 return i < K ? s.Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
             },
             [&](int idx, cplx<T> v) {
-                int t = idx / tb, b = idx - t * tb;
+                int t = fdiv(idx, ftb), b = idx - t * tb;
                 buf[b * ls + t] = v;
             });
         __syncthreads();
@@ -368,7 +371,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
         for (int o = 0; o < MAXO; ++o) {
             int oi = threadIdx.x + o * blockDim.x;
             if (oi < tb * m) {
-                int jc = oi / tb, b = oi - jc * tb, i = i0 + b;
+                int jc = fdiv(oi, ftb), b = oi - jc * tb, i = i0 + b;
                 if (i < K && MODE == 2) {
                     s.du[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))] =
                         cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
@@ -405,7 +408,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
     for (int o = 0; o < MAXO; ++o) {
         int oi = threadIdx.x + o * blockDim.x;
         if (oi >= tb * m) continue;
-        int jc = oi / tb, b = oi - jc * tb, i = i0 + b;
+        int jc = fdiv(oi, ftb), b = oi - jc * tb, i = i0 + b;
         if (i >= K) continue;
         const size_t base = i + (size_t)K * (jc + (size_t)m * kc), cs = (size_t)K * m * m;
         if (MODE == 1) {
