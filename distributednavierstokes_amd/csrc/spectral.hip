@@ -117,9 +117,11 @@ template <typename T> struct SpecImpl : SpecBase {
         // per line: three real velocity lines and three half-length complex work lines
         const size_t perline = (size_t)6 * g.n * sizeof(T);
         {
-            // lines per block of the fused x pass: LDS budget DNS_SP_XLDS bytes (default 64 KB)
+            // lines per block of the fused x pass: LDS budget DNS_SP_XLDS bytes
+            // (default 32 KB: one line per block in FP64, two in FP32, measured
+            // fastest at n = 512)
             const char *e = getenv("DNS_SP_XLDS");
-            const size_t budget = e ? (size_t)atol(e) : 65536;
+            const size_t budget = e ? (size_t)atol(e) : 32768;
             lpb = (int)std::max<size_t>(1, std::min<size_t>(16, budget / perline));
         }
         while (lpb > 1 && 3 * lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
