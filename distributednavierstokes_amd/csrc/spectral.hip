@@ -18,7 +18,7 @@ static const double W_A[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
 static const double W_B[3] = {1.0 / 4.0, 0.0, 0.0};
 static const double W_DC[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
 
-constexpr int SP_EPT = 16, SP_XEPT = 8, SP_SEPT = 8, SP_MAXO = 6, SP_NTHR = 256, SP_REDB = 1024;
+constexpr int SP_XEPT = 8, SP_SEPT = 8, SP_MAXO = 6, SP_NTHR = 256, SP_REDB = 1024;
 
 enum { SFAM_Y = 0, SFAM_Z = 1, SFAM_X = 2, SFAM_STAGE = 3, SFAM_FORCE = 4 };
 
@@ -46,6 +46,9 @@ struct SpecBase {
 };
 
 template <typename T> struct SpecImpl : SpecBase {
+    // elements per thread of the y / z pencil FFT tiles (FP64: 8, fewer
+    // registers; FP32: 16), measured at n = 512
+    static constexpr int SP_EPT = sizeof(T) == 8 ? 8 : 16;
     SpecGeom g{};
     double visc = 0, l = 0;
     hipStream_t st = nullptr;
@@ -125,7 +128,7 @@ template <typename T> struct SpecImpl : SpecBase {
             lpb = (int)std::max<size_t>(1, std::min<size_t>(16, budget / perline));
         }
         while (lpb > 1 && 3 * lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
-        ldsx = perline * lpb + (size_t)(g.n / 2) * sz; // + the LDS twiddle table
+        ldsx = perline * lpb + (size_t)(sizeof(T) == 4 ? g.n + 1 : g.n / 2) * sz; // + the LDS twiddle tables
         if (lds > (size_t)maxlds || ldss > (size_t)maxlds || ldsx + 2048 > (size_t)maxlds || tbs * g.m > SP_MAXO * SP_NTHR) {
             set_error("spectral tiles do not fit the LDS limit");
             return DNS_EUNSUPPORTED;

@@ -192,7 +192,8 @@ enum { XMID_RHS = 0, XMID_C2R = 1, XMID_R2C = 2, XMID_VMAX = 3 };
 //   XMID_R2C : r2c of phys (n, n, n, 3) → P[..,0:3] (kx < K)
 // The three components (and each group of three products) are transformed
 // together as 3·lpb lines.  LDS: v[3][lpb][n] reals, work[3·lpb][n/2]
-// complex, then the n/2 twiddles.
+// complex, then the n/2 transform twiddles and (FP32) the n/2+1
+// real-transform twiddles.
 template <typename T> struct XMidArgs {
     cplx<T> *P;
     T *phys;
@@ -217,6 +218,9 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     const int maxl = max(1, EPT * (int)blockDim.x / N); // lines one fft_lds call may hold
     // twiddles staged in LDS behind the work lines (read by every butterfly pass)
     const cplx<T> *twl = stage_twiddles<T>(work + (size_t)nw * N, a.tw, N);
+    // real-transform twiddles: an LDS copy in FP32 (measured faster there),
+    // read from L1/L2 in FP64
+    const cplx<T> *twxl = sizeof(T) == 4 ? stage_twiddles<T>(work + (size_t)(nw + 1) * N, a.twx, N + 1) : a.twx;
     if (MODE != XMID_R2C) {
         // c2r of the three components in one batched transform: work line c·lpb + b
         batched_for<SP_LOAD_U(EPT), cplx<T>>(
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
                     }
                     cplx<T> Xc = cconj<T>(Xn);
                     cplx<T> ev = cadd<T>(Xk, Xc);
-                    cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]);
+                    cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), a.twx[k]); // (the LDS copy is not yet complete)
                     Z = mk<T>(ev.x - od.y, ev.y + od.x);
                 }
                 return Z;
@@ -308,7 +312,7 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
             cplx<T> Zk = work[cl * N + (kx == N ? 0 : kx)];
             cplx<T> Zc = cconj<T>(work[cl * N + (kx == 0 ? 0 : N - kx)]);
             cplx<T> s = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
-            cplx<T> wd = cmul<T>(a.twx[kx], df);
+            cplx<T> wd = cmul<T>(twxl[kx], df);
             a.P[kx + (size_t)K * L + p * pstride] = mk<T>(T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x));
         }
         __syncthreads();
