@@ -488,6 +488,23 @@ def run_cpu_baseline(prob, steps=2, warmup=1, cfl=0.5):
     return steps / (time.perf_counter() - t0)
 
 
+def measured_traffic(prob, family):
+    """DRAM bytes per launch of `family` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated), or None when
+    no profile of this exact workload and kernel family is committed."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{family}_traffic_channel_{prob.n[0]}x{prob.n[1]}x{prob.n[2]}_{prob.precision}.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (list(rec.get("n", [])) != list(prob.n) or rec.get("ydiffusion") != prob.ydiffusion
+            or rec.get("precision") != prob.precision or rec.get("family") != family):
+        return None
+    return rec["bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -596,7 +613,9 @@ def main():
                    "n": list(prob.n), "ydiffusion": prob.ydiffusion, "precision": prob.precision},
         "mcell_updates_per_s": N * sps / 1e6,
         "roofline": {"bound": "hbm", "kernel": FAMILIES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(prob, FAMILIES[dom]),
+                     "algorithmic_bytes": fam_bytes},
         "step_roofline": {"model": "minimal-pass", "units_per_step": units, "bytes_per_step": step_bytes,
                           "achieved_gbs": step_bytes * sps / 1e9, "frac": step_bytes * sps / 1e9 / HBM_PEAK_GBS},
         "kernels": fam_summary,
