@@ -289,6 +289,9 @@ template <typename T> struct FVImpl : FVBase {
         hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (maxlds <= 0) maxlds = 65536;
         lpb = std::max(1, std::min(64, X_EPT * NTHR / std::max(1, Nh)));
+        if (sizeof(T) == 8) lpb = std::min(lpb, 4); // FP64: four lines per block measured fastest (C2, C3)
+        if (const char *el = getenv("DNS_X_LPB")) // sweeps: fewer lines per x-pass block
+            lpb = std::max(1, std::min(lpb, atoi(el)));
         ldsx = (size_t)(lpb + 1) * Nh * sizeof(cplx<T>); // lines + LDS twiddle table
         // z / periodic-y FFT tiles: a block loads tb lines and transforms them
         // in sub-batches of tbf lines.  In FP32, tiles narrower than 128-byte
