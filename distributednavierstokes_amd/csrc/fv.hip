@@ -94,6 +94,7 @@ template <typename T> struct FVImpl : FVBase {
     double *pinned = nullptr;
     int N = 0;
     int nb_pgrad = 0;
+    bool pgrad_vec = false; // k_pgrad_vec (16-byte accesses) instead of k_pgrad
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
     int momentum_kind = 5; // 5 LDS z-march (k_momentum_lz), 3 slab-ordered, 0 column-ordered, 1 direct, 2 LDS-tiled, 4 z-march in registers
     // reference-structured ("plain") step: extra arrays, allocated on first use
@@ -397,7 +398,13 @@ template <typename T> struct FVImpl : FVBase {
 
         redcap = std::max<size_t>({(size_t)RED_BLOCKS * 4, (size_t)16 * ny * CH_NCHUNK + 16});
         DNS_TRY(dalloc(&red, redcap));
-        nb_pgrad = ceil_div(nx, 64) * ceil_div(ny, 4) * nz;
+        // vector pressure-gradient kernel unless nx is not a multiple of its
+        // vector width (DNS_PGRAD=scalar keeps the one-cell-per-thread kernel)
+        {
+            const char *ep = getenv("DNS_PGRAD");
+            pgrad_vec = nx % (16 / (int)sizeof(T)) == 0 && !(ep && !strcmp(ep, "scalar"));
+        }
+        nb_pgrad = ceil_div(pgrad_vec ? nx / (16 / (int)sizeof(T)) : nx, 64) * ceil_div(ny, 4) * nz;
         DNS_TRY(dalloc(&upart, (size_t)3 * nb_pgrad));
         DNS_TRY(dalloc(&redout, (size_t)8 * ny + 64));
         DNS_HIP_CHECK(hipHostMalloc((void **)&pinned, sizeof(double) * ((size_t)8 * ny + 64)));
@@ -728,10 +735,16 @@ template <typename T> struct FVImpl : FVBase {
         a.rdz = (T)(1.0 / dz);
         a.s = T(1);
         a.inv_adt = (T)inv_adt;
-        dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
         timer.begin(FAM_PGRAD);
-        if (pupdate) hipLaunchKernelGGL((k_pgrad<T, true, true>), grd, blk, 0, st, a, g, upart);
-        else hipLaunchKernelGGL((k_pgrad<T, false, true>), grd, blk, 0, st, a, g, upart);
+        if (pgrad_vec) {
+            const dim3 blk(64, 4, 1), grd(ceil_div(g.nx / (16 / (int)sizeof(T)), 64), ceil_div(g.ny, 4), g.nz);
+            if (pupdate) hipLaunchKernelGGL((k_pgrad_vec<T, true, true>), grd, blk, 0, st, a, g, upart);
+            else hipLaunchKernelGGL((k_pgrad_vec<T, false, true>), grd, blk, 0, st, a, g, upart);
+        } else {
+            const dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4), g.nz);
+            if (pupdate) hipLaunchKernelGGL((k_pgrad<T, true, true>), grd, blk, 0, st, a, g, upart);
+            else hipLaunchKernelGGL((k_pgrad<T, false, true>), grd, blk, 0, st, a, g, upart);
+        }
         timer.end(FAM_PGRAD);
         DNS_HIP_CHECK(hipGetLastError());
         umax_valid = true;

@@ -136,6 +136,30 @@ def test_step_parity(name):
     assert dns.maxdiv(ug, s) < 1e-12
 
 
+@pytest.mark.parametrize("pgrad", ["", "scalar"])
+@pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
+def test_pgrad_kernels_step_parity(name, pgrad, monkeypatch):
+    """The vector (16-byte) and one-cell pressure-gradient kernels inside the
+    step, including the CFL bound their partial maxima feed."""
+    if pgrad:
+        monkeypatch.setenv("DNS_PGRAD", pgrad)
+    prob = problems()[name]
+    s, o = pair(prob)
+    u = randomfield(prob, seed=19, halo=o.exchange_halo_u)
+    o.project(u, zeros(prob))
+    ug = [a.copy(order="F") for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    qo, qg = vzeros(prob), vzeros(prob)
+    do, dg = (zeros(prob), zeros(prob)) if prob.implicit else (po, pg)
+    for _ in range(2):
+        o.step(u, po, qo, do, 0.004)
+        dns.step_(ug, pg, qg, dg, s, 0.004)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11, c
+    assert relmax(pg, po) < 1e-10
+    assert math.isclose(dns.cfl_timestep(ug, s, 0.5), o.cfl_timestep(ug, 0.5), rel_tol=1e-14)
+
+
 @pytest.mark.parametrize("n", [(32, 128, 8), (16, 256, 4), (16, 24, 4)])
 @pytest.mark.parametrize("variant", [{}, {"DNS_CN": "tile"}, {"DNS_CN": "sequential"}])
 def test_cn_variants_step_parity(n, variant, monkeypatch):
