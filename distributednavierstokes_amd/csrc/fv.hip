@@ -96,7 +96,7 @@ template <typename T> struct FVImpl : FVBase {
     int nb_pgrad = 0;
     bool pgrad_vec = false; // k_pgrad_vec (16-byte accesses) instead of k_pgrad
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
-    int momentum_kind = 5; // 5 LDS z-march (k_momentum_lz), 3 slab-ordered, 0 column-ordered, 1 direct, 2 LDS-tiled, 4 z-march in registers
+    int momentum_kind = 5; // 5 LDS z-march (k_momentum_lz), 3 slab-ordered (k_momentum_slab), 1 direct (k_momentum)
     // reference-structured ("plain") step: extra arrays, allocated on first use
     bool plain = false;
     T *pr = nullptr, *pwork = nullptr, *pghost = nullptr, *pyb[3] = {};
@@ -130,13 +130,10 @@ template <typename T> struct FVImpl : FVBase {
         g.wall = P->ywall ? 1 : 0;
         imexy = P->ydiffusion ? 1 : 0;
         {
-            // DNS_MOMENTUM selects the stencil kernel: lz (default), slab, zm, col, direct, tiled
+            // DNS_MOMENTUM selects the stencil kernel: lz (default), slab, direct
             const char *e = getenv("DNS_MOMENTUM");
             momentum_kind = !e                       ? 5
                             : !strcmp(e, "direct") ? 1
-                            : !strcmp(e, "tiled")  ? 2
-                            : !strcmp(e, "col")    ? 0
-                            : !strcmp(e, "zm")     ? 4
                             : !strcmp(e, "slab")   ? 3
                                                    : 5;
             // the LDS z-march loads whole rows as 16-byte vectors
@@ -558,20 +555,6 @@ template <typename T> struct FVImpl : FVBase {
             DNS_HIP_CHECK(hipGetLastError());
             return DNS_OK;
         }
-        if (momentum_kind == 4) {
-            constexpr int RY = 2, ZC = 16;
-            dim3 blk(64, 4, 1), grd(ceil_div(g.nx, 64), ceil_div(g.ny, 4 * RY), ceil_div(g.nz, ZC));
-            if (write_v) {
-                if (rq) hipLaunchKernelGGL((k_momentum_zm<T, true, true, RY>), grd, blk, 0, st, a, g, ZC);
-                else hipLaunchKernelGGL((k_momentum_zm<T, false, true, RY>), grd, blk, 0, st, a, g, ZC);
-            } else {
-                if (rq) hipLaunchKernelGGL((k_momentum_zm<T, true, false, RY>), grd, blk, 0, st, a, g, ZC);
-                else hipLaunchKernelGGL((k_momentum_zm<T, false, false, RY>), grd, blk, 0, st, a, g, ZC);
-            }
-            timer.end(FAM_MOM);
-            DNS_HIP_CHECK(hipGetLastError());
-            return DNS_OK;
-        }
         if (momentum_kind == 3) {
             // y-slabs whose three-plane u window (3 planes × 3 components) stays within ~2.5 MB of L2
             const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 4);
@@ -585,35 +568,6 @@ template <typename T> struct FVImpl : FVBase {
             } else {
                 if (rq) hipLaunchKernelGGL((k_momentum_slab<T, true, false>), grd, blk, 0, st, a, g, nbx, nbys);
                 else hipLaunchKernelGGL((k_momentum_slab<T, false, false>), grd, blk, 0, st, a, g, nbx, nbys);
-            }
-            timer.end(FAM_MOM);
-            DNS_HIP_CHECK(hipGetLastError());
-            return DNS_OK;
-        }
-        if (momentum_kind == 0) {
-            constexpr int RY = sizeof(T) == 8 ? 2 : 4;
-            const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 4 * RY);
-            dim3 blk(64, 4, 1), grd((unsigned)nbx * nby * g.nz);
-            if (write_v) {
-                if (rq) hipLaunchKernelGGL((k_momentum_col<T, true, true, RY>), grd, blk, 0, st, a, g, nbx);
-                else hipLaunchKernelGGL((k_momentum_col<T, false, true, RY>), grd, blk, 0, st, a, g, nbx);
-            } else {
-                if (rq) hipLaunchKernelGGL((k_momentum_col<T, true, false, RY>), grd, blk, 0, st, a, g, nbx);
-                else hipLaunchKernelGGL((k_momentum_col<T, false, false, RY>), grd, blk, 0, st, a, g, nbx);
-            }
-            timer.end(FAM_MOM);
-            DNS_HIP_CHECK(hipGetLastError());
-            return DNS_OK;
-        }
-        if (momentum_kind == 2) {
-            const int zc = std::min(g.nz, 32);
-            dim3 tgrd(ceil_div(g.nx, 64), ceil_div(g.ny, 8), ceil_div(g.nz, zc));
-            if (write_v) {
-                if (rq) hipLaunchKernelGGL((k_momentum_tiled<T, true, true>), tgrd, dim3(256), 0, st, a, g, zc);
-                else hipLaunchKernelGGL((k_momentum_tiled<T, false, true>), tgrd, dim3(256), 0, st, a, g, zc);
-            } else {
-                if (rq) hipLaunchKernelGGL((k_momentum_tiled<T, true, false>), tgrd, dim3(256), 0, st, a, g, zc);
-                else hipLaunchKernelGGL((k_momentum_tiled<T, false, false>), tgrd, dim3(256), 0, st, a, g, zc);
             }
             timer.end(FAM_MOM);
             DNS_HIP_CHECK(hipGetLastError());

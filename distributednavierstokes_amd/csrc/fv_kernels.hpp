@@ -228,95 +228,6 @@ __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
                                       uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m, uz0pm);
 }
 
-// z-marching variant: a thread owns RY consecutive rows of one x column and
-// walks ZC consecutive z planes, holding the in-plane stencil columns of
-// planes k and k+1 in registers (and the few values of plane k−1 the stencil
-// needs), so each plane is read once per column instead of three times and
-// the z neighbours never miss in L1.  Grid (nx/64, ny/(4·RY), nz/ZC) of
-// (64, 4) threads, XCD-ordered.
-template <typename T, int RY> struct MomPlane {
-    T xc[RY + 2], xp[RY], xm[RY + 1];   // ux: centre rows j0-1..j0+RY, x+1 rows, x-1 rows j0..j0+RY
-    T yc[RY + 2], yp[RY + 1], ym[RY];   // uy: centre rows j0-1..j0+RY, x+1 rows j0-1.., x-1 rows
-    T zc[RY + 2], zp[RY], zm[RY];       // uz: centre rows j0-1..j0+RY, x+1 rows, x-1 rows
-};
-
-template <typename T, int RY>
-__device__ inline void mom_load_plane(MomPlane<T, RY> &P, const MomentumArgs<T> &a, const FVGeom &g, int i, int ip,
-                                      int im, int j0, int k)
-{
-    const T m1 = T(-1);
-    auto cx = [&](const T *f, int ii, int r) -> T { return r <= g.ny ? ldc(f, ii, r + 1, k, g, m1) : T(0); };
-    auto fy = [&](int ii, int r) -> T { return r <= g.ny ? ldf(a.uy, ii, r + 1, k, g) : T(0); };
-#pragma unroll
-    for (int r = 0; r < RY + 2; ++r) {
-        P.xc[r] = cx(a.ux, i, j0 - 1 + r);
-        P.yc[r] = fy(i, j0 - 1 + r);
-        P.zc[r] = cx(a.uz, i, j0 - 1 + r);
-    }
-#pragma unroll
-    for (int r = 0; r < RY + 1; ++r) {
-        P.xm[r] = cx(a.ux, im, j0 + r);
-        P.yp[r] = fy(ip, j0 - 1 + r);
-    }
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-        P.xp[r] = cx(a.ux, ip, j0 + r);
-        P.ym[r] = fy(im, j0 + r);
-        P.zp[r] = cx(a.uz, ip, j0 + r);
-        P.zm[r] = cx(a.uz, im, j0 + r);
-    }
-}
-
-template <typename T, bool READ_Q, bool WRITE_V, int RY>
-__global__ void __launch_bounds__(256) k_momentum_zm(MomentumArgs<T> a, FVGeom g, int zc)
-{
-    int bx, by, bz;
-    xcd_block3(bx, by, bz);
-    const int i = bx * 64 + threadIdx.x;
-    const int j0 = (by * 4 + threadIdx.y) * RY;
-    const int k0 = bz * zc, k1 = min(g.nz, k0 + zc);
-    if (i >= g.nx || j0 >= g.ny) return;
-    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
-    // plane k−1: ux, uy, uz at rows j, uz at rows j+1 and at (x+1, j)
-    T pxc[RY], pyc[RY], pzc[RY + 1], pzp[RY];
-    {
-        MomPlane<T, RY> Pm;
-        mom_load_plane(Pm, a, g, i, ip, im, j0, wrapm(k0, g.nz));
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            pxc[r] = Pm.xc[r + 1];
-            pyc[r] = Pm.yc[r + 1];
-            pzc[r] = Pm.zc[r + 1];
-            pzp[r] = Pm.zp[r];
-        }
-        pzc[RY] = Pm.zc[RY + 1];
-    }
-    MomPlane<T, RY> C, N;
-    mom_load_plane(C, a, g, i, ip, im, j0, k0);
-    for (int k = k0; k < k1; ++k) {
-        mom_load_plane(N, a, g, i, ip, im, j0, wrapp(k, g.nz));
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            const int j = j0 + r;
-            if (j >= g.ny) break;
-            momentum_cell<T, READ_Q, WRITE_V, true>(
-                a, g, IX(i, j, k, g), j, j + 1, C.xc[r + 1], C.xp[r], C.xm[r], C.xc[r + 2], C.xc[r], N.xc[r + 1],
-                pxc[r], C.xm[r + 1], N.xm[r], C.yc[r + 1], C.yp[r + 1], C.ym[r], C.yc[r + 2], C.yc[r], N.yc[r + 1],
-                pyc[r], C.yp[r], N.yc[r], C.zc[r + 1], C.zp[r], C.zm[r], C.zc[r + 2], C.zc[r], N.zc[r + 1], pzc[r],
-                pzp[r], pzc[r + 1]);
-        }
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            pxc[r] = C.xc[r + 1];
-            pyc[r] = C.yc[r + 1];
-            pzc[r] = C.zc[r + 1];
-            pzp[r] = C.zp[r];
-        }
-        pzc[RY] = C.zc[RY + 1];
-        C = N;
-    }
-}
-
 // Slab-ordered variant of k_momentum: blocks are ordered x-fastest within
 // y-slabs of `nbys` block rows, then z, then slab, and each XCD takes a
 // contiguous run of that order, so one XCD streams through consecutive z
@@ -359,69 +270,6 @@ __global__ void __launch_bounds__(256) k_momentum_slab(MomentumArgs<T> a, FVGeom
     momentum_cell<T, READ_Q, WRITE_V, true>(a, g, IX(i, j, k, g), j, jj, ux000, uxp00, uxm00, ux0p0, ux0m0, ux00p,
                                             ux00m, uxmp0, uxm0p, uy000, uyp00, uym00, uy0p0, uy0m0, uy00p, uy00m, uypm0,
                                             uy0mp, uz000, uzp00, uzm00, uz0p0, uz0m0, uz00p, uz00m, uzp0m, uz0pm);
-}
-
-// Column-ordered variant: each thread computes RY consecutive y rows, so
-// the y-neighbour values of one row are the centre values of the next and
-// each column of the stencil is loaded once; the blocks (64 × 4·RY cells)
-// are ordered z-fastest within an (x, y) tile column and each XCD takes a
-// contiguous run of that order, so the blocks resident on one XCD are
-// consecutive z planes of one tile column and their z-neighbour planes hit
-// in its L2.  1D grid of nbx · nby · nz blocks of (64, 4) threads.
-template <typename T, bool READ_Q, bool WRITE_V, int RY>
-__global__ void __launch_bounds__(256) k_momentum_col(MomentumArgs<T> a, FVGeom g, int nbx)
-{
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int k = lb % g.nz, tcol = lb / g.nz, bx = tcol % nbx, by = tcol / nbx;
-    const int i = bx * 64 + threadIdx.x;
-    const int j0 = (by * 4 + threadIdx.y) * RY;
-    if (i >= g.nx || j0 >= g.ny) return;
-    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
-    const int kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
-    const T m1 = T(-1);
-    const T *ux = a.ux, *uy = a.uy, *uz = a.uz;
-    // values at interior row r (r = -1 and r = ny are ghost rows); rows past ny are never used
-    auto cx = [&](const T *f, int ii, int r, int kk) -> T { return r <= g.ny ? ldc(f, ii, r + 1, kk, g, m1) : T(0); };
-    auto fy = [&](int ii, int r, int kk) -> T { return r <= g.ny ? ldf(uy, ii, r + 1, kk, g) : T(0); };
-    T uxc[RY + 2], uxp[RY], uxm[RY + 1], uxkp[RY], uxkm[RY], uxmkp[RY];
-    T uyc[RY + 2], uyp[RY + 1], uym[RY], uykp[RY + 1], uykm[RY];
-    T uzc[RY + 2], uzp[RY], uzm[RY], uzkp[RY], uzkm[RY + 1], uzpkm[RY];
-#pragma unroll
-    for (int r = 0; r < RY + 2; ++r) {
-        uxc[r] = cx(ux, i, j0 - 1 + r, k);
-        uyc[r] = fy(i, j0 - 1 + r, k);
-        uzc[r] = cx(uz, i, j0 - 1 + r, k);
-    }
-#pragma unroll
-    for (int r = 0; r < RY + 1; ++r) {
-        uxm[r] = cx(ux, im, j0 + r, k);
-        uyp[r] = fy(ip, j0 - 1 + r, k);
-        uykp[r] = fy(i, j0 - 1 + r, kp);
-        uzkm[r] = cx(uz, i, j0 + r, km);
-    }
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-        uxp[r] = cx(ux, ip, j0 + r, k);
-        uxkp[r] = cx(ux, i, j0 + r, kp);
-        uxkm[r] = cx(ux, i, j0 + r, km);
-        uxmkp[r] = cx(ux, im, j0 + r, kp);
-        uym[r] = fy(im, j0 + r, k);
-        uykm[r] = fy(i, j0 + r, km);
-        uzp[r] = cx(uz, ip, j0 + r, k);
-        uzm[r] = cx(uz, im, j0 + r, k);
-        uzkp[r] = cx(uz, i, j0 + r, kp);
-        uzpkm[r] = cx(uz, ip, j0 + r, km);
-    }
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-        const int j = j0 + r;
-        if (j >= g.ny) break;
-        momentum_cell<T, READ_Q, WRITE_V>(a, g, IX(i, j, k, g), j, j + 1, uxc[r + 1], uxp[r], uxm[r], uxc[r + 2],
-                                          uxc[r], uxkp[r], uxkm[r], uxm[r + 1], uxmkp[r], uyc[r + 1], uyp[r + 1],
-                                          uym[r], uyc[r + 2], uyc[r], uykp[r + 1], uykm[r], uyp[r], uykp[r],
-                                          uzc[r + 1], uzp[r], uzm[r], uzc[r + 2], uzc[r], uzkp[r], uzkm[r], uzpkm[r],
-                                          uzkm[r + 1]);
-    }
 }
 
 // LDS z-march with staged plane loads: a block owns a 64 × 8 (x, y) tile
@@ -617,72 +465,6 @@ __global__ void __launch_bounds__(256) k_momentum_lz(MomentumArgs<T> a, FVGeom g
     for (int k = k0; k < kend; k += 2) {
         plane_step(k, vA, hA, vB, hB);
         if (k + 1 < kend) plane_step(k + 1, vB, hB, vA, hA);
-    }
-}
-
-// LDS-tiled variant: a block owns a 64 × 8 (x, y) tile and marches over
-// `zc` planes in z, keeping planes k−1, k, k+1 of all three components with
-// a one-cell halo in LDS (each plane is read from HBM once per block and the
-// 26 neighbours come from LDS).  256 threads, each computes rows ty and
-// ty + 4 of the tile.  Halo values follow the same ghost rules as ldc/ldf.
-template <typename T, bool READ_Q, bool WRITE_V>
-__global__ void __launch_bounds__(256) k_momentum_tiled(MomentumArgs<T> a, FVGeom g, int zc)
-{
-    constexpr int TX = 64, TY = 8, SX = TX + 2, SY = TY + 2, PL = SX * SY;
-    __shared__ T sm[3][3][PL]; // [slot][component][ly * SX + lx]
-    int bx, by, bz;
-    xcd_block3(bx, by, bz);
-    const int x0 = bx * TX, y0 = by * TY, k0 = bz * zc;
-    const int kend = min(k0 + zc, g.nz);
-    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    const T m1 = T(-1);
-
-    auto load_plane = [&](int slot, int k) {
-        const int kw = k < 0 ? k + g.nz : (k >= g.nz ? k - g.nz : k);
-        for (int e = tid; e < 3 * PL; e += 256) {
-            const int comp = e / PL, r = e - comp * PL;
-            const int ly = r / SX, lx = r - ly * SX;
-            int i = (x0 + lx - 1) % g.nx;
-            if (i < 0) i += g.nx;
-            const int jj = y0 + ly; // ghosted row of tile row ly − 1
-            T v;
-            if (comp == 0) v = ldc(a.ux, i, jj, kw, g, m1);
-            else if (comp == 1) v = ldf(a.uy, i, jj, kw, g);
-            else v = ldc(a.uz, i, jj, kw, g, m1);
-            sm[slot][comp][r] = v;
-        }
-    };
-
-    load_plane(0, k0 - 1);
-    load_plane(1, k0);
-    load_plane(2, k0 + 1);
-    __syncthreads();
-    int sM = 0, s0 = 1, sP = 2;
-    for (int k = k0; k < kend; ++k) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int i = x0 + tx, j = y0 + ty + 4 * h;
-            if (i < g.nx && j < g.ny) {
-                const int c = (ty + 4 * h + 1) * SX + tx + 1;
-                const T *X0 = sm[s0][0], *Y0 = sm[s0][1], *Z0 = sm[s0][2];
-                const T *XP = sm[sP][0], *YP = sm[sP][1], *ZP = sm[sP][2];
-                const T *XM = sm[sM][0], *YM = sm[sM][1], *ZM = sm[sM][2];
-                momentum_cell<T, READ_Q, WRITE_V>(
-                    a, g, IX(i, j, k, g), j, j + 1,
-                    X0[c], X0[c + 1], X0[c - 1], X0[c + SX], X0[c - SX], XP[c], XM[c], X0[c - 1 + SX], XP[c - 1],
-                    Y0[c], Y0[c + 1], Y0[c - 1], Y0[c + SX], Y0[c - SX], YP[c], YM[c], Y0[c + 1 - SX], YP[c - SX],
-                    Z0[c], Z0[c + 1], Z0[c - 1], Z0[c + SX], Z0[c - SX], ZP[c], ZM[c], ZM[c + 1], ZM[c + SX]);
-            }
-        }
-        __syncthreads();
-        if (k + 1 < kend) {
-            load_plane(sM, k + 2);
-            const int t = sM;
-            sM = s0;
-            s0 = sP;
-            sP = t;
-        }
-        __syncthreads();
     }
 }
 
