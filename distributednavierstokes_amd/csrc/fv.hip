@@ -78,6 +78,7 @@ template <typename T> struct FVImpl : FVBase {
     size_t ldst = 0;
     int cn_tl = 0, cn_nthr = 256; // x-lines and threads per block of the LDS CN kernel (0: sequential)
     int cn_seg = 0;               // rows per segment of the equal-segment CN kernel (0: not used)
+    int cn_nl = 1;                // x-lines per thread of the equal-segment CN kernel
     size_t ldscn = 0;
     double *Dcn = nullptr;
     T *dy_t = nullptr, *rdy_t = nullptr, *rdyu_t = nullptr;
@@ -377,6 +378,18 @@ template <typename T> struct FVImpl : FVBase {
                     if (ny % S == 0 && (R == 8 || R == 16 || R == 24 || R == 32) &&
                         ldscn + (size_t)ny * sizeof(double) <= (size_t)maxlds)
                         cn_seg = R;
+                }
+                // DNS_CN_NL=2: two x-lines per thread (pair accesses, block rows twice
+                // as wide, same segments); one line measured as fast in FP64 and faster
+                // in FP32 (the pair version spills there)
+                cn_nl = 1;
+                if (cn_seg && nx % 2 == 0) {
+                    const char *en = getenv("DNS_CN_NL");
+                    const size_t need = (size_t)6 * cn_nthr * sizeof(double) + (size_t)7 * ny * sizeof(double);
+                    if (en && atoi(en) == 2 && need <= (size_t)maxlds) {
+                        cn_nl = 2;
+                        cn_tl *= 2;
+                    }
                 }
                 if (cn_tl && !cn_seg && (ny + cn_nthr / cn_tl - 1) / (cn_nthr / cn_tl) > CN_RMAX) {
                     set_error("CN tile %d:%d leaves more than %d rows per thread", cn_tl, cn_nthr, CN_RMAX);
@@ -706,12 +719,17 @@ template <typename T> struct FVImpl : FVBase {
     }
 
     // equal-segment CN kernel for component c with R rows per thread
+    template <int R, int NL> void cn_seg_launch_nl(int c, dim3 grd, dim3 blk, const CNArgs<T> &a)
+    {
+        const size_t lds = (size_t)3 * NL * cn_nthr * sizeof(double) + (size_t)7 * g.ny * sizeof(double);
+        if (c == 0) hipLaunchKernelGGL((k_cn_seg<T, 0, R, NL>), grd, blk, lds, st, a, g, cn_tl);
+        else if (c == 1) hipLaunchKernelGGL((k_cn_seg<T, 1, R, NL>), grd, blk, lds, st, a, g, cn_tl);
+        else hipLaunchKernelGGL((k_cn_seg<T, 2, R, NL>), grd, blk, lds, st, a, g, cn_tl);
+    }
     template <int R> void cn_seg_launch(int c, dim3 grd, dim3 blk, const CNArgs<T> &a)
     {
-        const size_t lds = ldscn + (size_t)g.ny * sizeof(double);
-        if (c == 0) hipLaunchKernelGGL((k_cn_seg<T, 0, R>), grd, blk, lds, st, a, g, cn_tl);
-        else if (c == 1) hipLaunchKernelGGL((k_cn_seg<T, 1, R>), grd, blk, lds, st, a, g, cn_tl);
-        else hipLaunchKernelGGL((k_cn_seg<T, 2, R>), grd, blk, lds, st, a, g, cn_tl);
+        if (cn_nl == 2) cn_seg_launch_nl<R, 2>(c, grd, blk, a);
+        else cn_seg_launch_nl<R, 1>(c, grd, blk, a);
     }
 
     int launch_cn(double B, double alpha, double dt)

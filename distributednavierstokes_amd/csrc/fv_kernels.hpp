@@ -1366,23 +1366,45 @@ __global__ void __launch_bounds__(512) k_cn_tile(CNArgs<T> a, FVGeom g, int TL)
 }
 
 // k_cn_tile for the common case of equal segments: every line's ny rows
-// split into S = blockDim/TL segments of exactly R rows (ny = S·R), so no
-// row needs a bounds test.  All loads of a chunk of CH rows are issued
-// unconditionally (the row above/below a line end is clamped and its
-// coefficient is zero), which lets them all be in flight together; the
-// segment's rows stay in registers through both sweeps as in k_cn_tile.
-// Loads go through buffer resources (one offset register per array for all
-// rows) and the register budget is two blocks per CU (256 VGPRs: R = 24
-// FP64 rows without spills; three blocks per CU spill and measured slower).
-// grid (ceil(nx / TL), nz); LDS 7 ny + 3 blockDim doubles.
-template <typename T, int COMP, int R>
+// split into S segments of exactly R rows (ny = S·R), so no row needs a
+// bounds test.  Each thread owns NL neighbouring x-lines (NL = 2: pair
+// accesses of 8 or 16 bytes, a block row of TL lines is TL·sizeof(T) bytes
+// wide) over the R rows of its segment; S = blockDim·NL / TL.  All loads of
+// a chunk of CH rows are issued unconditionally (the row past a line end
+// reads 0 through the buffer range check, the row before it is clamped, and
+// both couplings are cut by zero coefficients), so they are all in flight
+// together; the segment's rows stay in registers through both sweeps as in
+// k_cn_tile.  Loads go through buffer resources (one offset register per
+// array for all rows) and the register budget is two blocks per CU (three
+// spilled and measured slower).
+// grid (ceil(nx / TL), nz); LDS 7 ny + 3 NL blockDim doubles.
+template <typename T, int NL> struct CNVec;
+template <typename T> struct CNVec<T, 1> {
+    using type = T;
+    __device__ static T at(const T &v, int) { return v; }
+    __device__ static T make(const double *x) { return T(x[0]); }
+};
+template <> struct CNVec<float, 2> {
+    using type = float2;
+    __device__ static float at(const float2 &v, int e) { return e == 0 ? v.x : v.y; }
+    __device__ static float2 make(const double *x) { return make_float2((float)x[0], (float)x[1]); }
+};
+template <> struct CNVec<double, 2> {
+    using type = double2;
+    __device__ static double at(const double2 &v, int e) { return e == 0 ? v.x : v.y; }
+    __device__ static double2 make(const double *x) { return make_double2(x[0], x[1]); }
+};
+
+template <typename T, int COMP, int R, int NL>
 __global__ void __launch_bounds__(256, 2) k_cn_seg(CNArgs<T> a, FVGeom g, int TL)
 {
+    using CV = CNVec<T, NL>;
+    using VT = typename CV::type;
     extern __shared__ unsigned char smem_raw[];
     constexpr int CH = R % 8 == 0 ? 8 : R;
-    const int ny = g.ny, S = blockDim.x / TL;
-    double *As = reinterpret_cast<double *>(smem_raw), *Bs = As + blockDim.x, *Xin = Bs + blockDim.x;
-    double *ca = Xin + blockDim.x, *cb = ca + ny, *cc = cb + ny, *cf = cc + ny, *ci = cf + ny, *cq = ci + ny;
+    const int ny = g.ny, TLT = TL / NL, S = blockDim.x / TLT; // threads per block row, segments
+    double *As = reinterpret_cast<double *>(smem_raw), *Bs = As + NL * blockDim.x, *Xin = Bs + NL * blockDim.x;
+    double *ca = Xin + NL * blockDim.x, *cb = ca + ny, *cc = cb + ny, *cf = cc + ny, *ci = cf + ny, *cq = ci + ny;
     double *cr = cq + ny; // 1/Δyu of the normal rows
     for (int j = threadIdx.x; j < ny; j += blockDim.x) {
         if (COMP == 1) cr[j] = a.rdyu[j + 1];
@@ -1393,10 +1415,10 @@ __global__ void __launch_bounds__(256, 2) k_cn_seg(CNArgs<T> a, FVGeom g, int TL
         ci[j] = a.ip[j];
         cq[j] = j == ny - 1 ? 0.0 : -a.cp[j];
     }
-    const int l = threadIdx.x % TL, sg = threadIdx.x / TL;
+    const int l = threadIdx.x % TLT, sg = threadIdx.x / TLT;
     int bxr, byr;
     xcd_block2(bxr, byr);
-    const int i = bxr * TL + l, k = byr;
+    const int i = bxr * TL + NL * l, k = byr; // first of this thread's NL lines
     const bool valid = i < g.nx;
     const int ii = valid ? i : 0;
     const int j0 = sg * R;
@@ -1409,99 +1431,131 @@ __global__ void __launch_bounds__(256, 2) k_cn_seg(CNArgs<T> a, FVGeom g, int TL
     // pressure neighbour: x+1 (same plane), y+1 (next row) or z+1 (next plane)
     const bufres rpn = COMP == 2 ? make_bufres(a.p + plane * wrapp(k, g.nz), pbytes) : rp;
     const int vo = (ii + g.nx * j0) * (int)sizeof(T);
-    const int von = COMP == 0 ? (wrapp(ii, g.nx) + g.nx * j0) * (int)sizeof(T) : vo;
+    // x+1 of the last of the NL lines (the others are in the pair itself)
+    const int von = COMP == 0 ? (wrapp(ii + NL - 1, g.nx) + g.nx * j0) * (int)sizeof(T) : vo;
     const int jact = (COMP == 1 && g.wall) ? ny - 1 : ny;
     const double rdn = COMP == 0 ? a.rdx : a.rdz;
-    double x[R];
-    double um = (double)bload<T>(ru, sg > 0 ? vo - rowb : vo, 0); // coefficient ca[0] = 0 cuts row −1
+    double x[NL][R];
+    double um[NL];
+    {
+        const VT v = bload<VT>(ru, sg > 0 ? vo - rowb : vo, 0); // coefficient ca[0] = 0 cuts row −1
+#pragma unroll
+        for (int e = 0; e < NL; ++e) um[e] = (double)CV::at(v, e);
+    }
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
-        T ur[CH + 1], qr[CH], pr[CH], pn[CH];
+        VT ur[CH + 1], qr[CH], pr[CH], pn[CH];
+        T pxl[CH]; // COMP 0: x+1 of the last line
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
             const int so = rowb * (c0 + q);
-            ur[q] = bload<T>(ru, vo, so);
-            qr[q] = bload<T>(rq, vo, so);
-            pr[q] = bload<T>(rp, vo, so);
-            pn[q] = bload<T>(rpn, von, COMP == 1 ? so + rowb : so);
+            ur[q] = bload<VT>(ru, vo, so);
+            qr[q] = bload<VT>(rq, vo, so);
+            pr[q] = bload<VT>(rp, vo, so);
+            if (COMP == 0) pxl[q] = bload<T>(rpn, von, so);
+            else pn[q] = bload<VT>(rpn, vo, COMP == 1 ? so + rowb : so);
         }
-        ur[CH] = bload<T>(ru, vo, rowb * (c0 + CH));
+        ur[CH] = bload<VT>(ru, vo, rowb * (c0 + CH));
         if (c0 == 0) __syncthreads(); // the coefficient rows are staged before first use
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
             const int j = j0 + c0 + q;
             const double rd = COMP == 1 ? cr[j] : rdn;
-            const double dpdc = ((double)pn[q] - (double)pr[q]) * rd;
-            const double v = j < jact ? a.B * (double)qr[q] - a.adt * dpdc : 0.0;
-            const double u0 = (double)ur[q];
-            double lap = cb[j] * u0;
-            lap += ca[j] * um;
-            lap += cc[j] * (double)ur[q + 1];
-            x[c0 + q] = u0 + v + a.theta * lap;
-            um = u0;
+#pragma unroll
+            for (int e = 0; e < NL; ++e) {
+                const double pc = (double)CV::at(pr[q], e);
+                const double pnb = COMP == 0 ? (e + 1 < NL ? (double)CV::at(pr[q], e + 1) : (double)pxl[q])
+                                             : (double)CV::at(pn[q], e);
+                const double dpdc = (pnb - pc) * rd;
+                const double v = j < jact ? a.B * (double)CV::at(qr[q], e) - a.adt * dpdc : 0.0;
+                const double u0 = (double)CV::at(ur[q], e);
+                double lap = cb[j] * u0;
+                lap += ca[j] * um[e];
+                lap += cc[j] * (double)CV::at(ur[q + 1], e);
+                x[e][c0 + q] = u0 + v + a.theta * lap;
+                um[e] = u0;
+            }
         }
     }
     // forward sweep, local to the segment
-    {
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
         double P = 1.0, y = 0.0;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int j = j0 + q;
-            y = cf[j] * y + ci[j] * x[q];
-            x[q] = y;
+            y = cf[j] * y + ci[j] * x[e][q];
+            x[e][q] = y;
             P *= cf[j];
         }
-        As[threadIdx.x] = P;
-        Bs[threadIdx.x] = y;
+        As[NL * threadIdx.x + e] = P;
+        Bs[NL * threadIdx.x + e] = y;
     }
     __syncthreads();
     if (sg == 0) {
-        double xi = 0.0;
-        for (int q = 0; q < S; ++q) {
-            Xin[q * TL + l] = xi;
-            xi = As[q * TL + l] * xi + Bs[q * TL + l];
+        for (int e = 0; e < NL; ++e) {
+            double xi = 0.0;
+            for (int q = 0; q < S; ++q) {
+                const int t = NL * (q * TLT + l) + e;
+                Xin[t] = xi;
+                xi = As[t] * xi + Bs[t];
+            }
         }
     }
     __syncthreads();
-    {
-        const double xi = Xin[threadIdx.x];
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
+        const double xi = Xin[NL * threadIdx.x + e];
         double P = 1.0;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             P *= cf[j0 + q];
-            x[q] += P * xi;
+            x[e][q] += P * xi;
         }
     }
     __syncthreads();
     // back substitution, local to the segment
-    {
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
         double P = 1.0, y = 0.0;
 #pragma unroll
         for (int q = R - 1; q >= 0; --q) {
             const int j = j0 + q;
-            y = cq[j] * y + x[q];
-            x[q] = y;
+            y = cq[j] * y + x[e][q];
+            x[e][q] = y;
             P *= cq[j];
         }
-        As[threadIdx.x] = P;
-        Bs[threadIdx.x] = y;
+        As[NL * threadIdx.x + e] = P;
+        Bs[NL * threadIdx.x + e] = y;
     }
     __syncthreads();
     if (sg == 0) {
-        double xi = 0.0;
-        for (int q = S - 1; q >= 0; --q) {
-            Xin[q * TL + l] = xi;
-            xi = As[q * TL + l] * xi + Bs[q * TL + l];
+        for (int e = 0; e < NL; ++e) {
+            double xi = 0.0;
+            for (int q = S - 1; q >= 0; --q) {
+                const int t = NL * (q * TLT + l) + e;
+                Xin[t] = xi;
+                xi = As[t] * xi + Bs[t];
+            }
         }
     }
     __syncthreads();
     if (!valid) return;
-    const double xi = Xin[threadIdx.x];
-    double P = 1.0;
+    double xi[NL], P[NL];
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
+        xi[e] = Xin[NL * threadIdx.x + e];
+        P[e] = 1.0;
+    }
 #pragma unroll
     for (int q = R - 1; q >= 0; --q) {
-        P *= cq[j0 + q];
-        bstore(ru, vo, rowb * q, T(x[q] + P * xi));
+        double out[NL];
+#pragma unroll
+        for (int e = 0; e < NL; ++e) {
+            P[e] *= cq[j0 + q];
+            out[e] = x[e][q] + P[e] * xi[e];
+        }
+        bstore(ru, vo, rowb * q, CV::make(out));
     }
 }
 

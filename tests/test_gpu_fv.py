@@ -161,10 +161,11 @@ def test_pgrad_kernels_step_parity(name, pgrad, monkeypatch):
 
 
 @pytest.mark.parametrize("n", [(32, 128, 8), (16, 256, 4), (16, 24, 4)])
-@pytest.mark.parametrize("variant", [{}, {"DNS_CN": "tile"}, {"DNS_CN": "sequential"}])
+@pytest.mark.parametrize("variant", [{}, {"DNS_CN_NL": "2"}, {"DNS_CN": "tile"}, {"DNS_CN": "sequential"}])
 def test_cn_variants_step_parity(n, variant, monkeypatch):
-    """Every Crank–Nicolson y-diffusion kernel (equal segments, general
-    segments, one thread per line) against the oracle, on line lengths that
+    """Every Crank–Nicolson y-diffusion kernel (equal segments with two or one
+    x-lines per thread, general segments, one thread per line) against the
+    oracle, on line lengths that
     split into equal register segments (128, 256 rows) and one that does not."""
     for key, val in variant.items():
         monkeypatch.setenv(key, val)
