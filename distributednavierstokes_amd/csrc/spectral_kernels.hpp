@@ -319,6 +319,119 @@ __global__ void __launch_bounds__(256) k_sp_xmid(XMidArgs<T> a, SpecGeom g, FFTD
     }
 }
 
+// The RHS x pass of k_sp_xmid, persistent over `lpi` consecutive (y, z)
+// lines per block, one line at a time: the raw P rows of line L+1 are loaded
+// into registers before line L is computed and land in LDS after it, so the
+// HBM latency of a line hides behind the previous line's transforms; the
+// twiddle tables are staged once per block and the CFL bound is one block
+// reduction at the end.  Needs 3K ≤ XR_RP · blockDim.
+// LDS: v[3][n] reals, work[3][n/2] and raw[3][K] complex, the n/2 transform
+// and n/2+1 real-transform twiddles.  One vmax partial per block.
+constexpr int XR_RP = 3; // raw P elements per thread per line
+
+template <typename T, int EPT>
+__global__ void __launch_bounds__(256) k_sp_xrhs(XMidArgs<T> a, SpecGeom g, FFTDesc d, int lpi)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int n = g.n, N = n / 2, K = g.K;
+    T *v = reinterpret_cast<T *>(smem_raw);
+    cplx<T> *work = reinterpret_cast<cplx<T> *>(smem_raw + sizeof(T) * (size_t)3 * n);
+    cplx<T> *raw = work + 3 * N;
+    const cplx<T> *twl = stage_twiddles<T>(raw + 3 * K, a.tw, N);
+    // real-transform twiddles: LDS copy in FP32 (read after the first barrier), L1/L2 in FP64
+    const cplx<T> *twxl = sizeof(T) == 4 ? stage_twiddles<T>(raw + 3 * K + N, a.twx, N + 1) : a.twx;
+    const int nlines = n * n;
+    const int Lb = xcd_block(blockIdx.x, gridDim.x) * lpi, Le = min(nlines, Lb + lpi);
+    const size_t pstride = (size_t)K * n * n; // component stride of P
+    const int maxl = max(1, EPT * (int)blockDim.x / N);
+    // this thread's raw elements: e_s = tid + s·blockDim → P offset c·pstride + k
+    size_t roff[XR_RP];
+    bool rok[XR_RP];
+#pragma unroll
+    for (int s = 0; s < XR_RP; ++s) {
+        const int e = threadIdx.x + s * (int)blockDim.x;
+        rok[s] = e < 3 * K;
+        const int c = fdiv(rok[s] ? e : 0, a.fK), k = (rok[s] ? e : 0) - c * K;
+        roff[s] = k + c * pstride;
+    }
+    cplx<T> r0 = mk<T>(T(0), T(0)), r1 = r0, r2 = r0; // prefetched raw elements (XR_RP = 3)
+    static_assert(XR_RP == 3, "three prefetch registers");
+    double mx = 0;
+    if (Lb < Le) {
+        const cplx<T> *src = a.P + (size_t)K * Lb;
+        if (rok[0]) r0 = src[roff[0]];
+        if (rok[1]) r1 = src[roff[1]];
+        if (rok[2]) r2 = src[roff[2]];
+    }
+    for (int L = Lb; L < Le; ++L) {
+        if (rok[0]) raw[threadIdx.x] = r0;
+        if (rok[1]) raw[threadIdx.x + blockDim.x] = r1;
+        if (rok[2]) raw[threadIdx.x + 2 * blockDim.x] = r2;
+        if (L + 1 < Le) {
+            const cplx<T> *src = a.P + (size_t)K * (L + 1);
+            if (rok[0]) r0 = src[roff[0]];
+            if (rok[1]) r1 = src[roff[1]];
+            if (rok[2]) r2 = src[roff[2]];
+        }
+        __syncthreads();
+        // c2r of the three components in one batched transform
+        for (int idx = threadIdx.x; idx < 3 * N; idx += blockDim.x) {
+            const int c = fdiv(idx, a.fN), k = idx - c * N;
+            const cplx<T> *row = raw + c * K;
+            cplx<T> Xk = k < K ? row[k] : mk<T>(T(0), T(0));
+            cplx<T> Xn = (N - k) < K ? row[N - k] : mk<T>(T(0), T(0));
+            if (k == 0) {
+                Xk.y = T(0);
+                Xn.y = T(0);
+            }
+            const cplx<T> Xc = cconj<T>(Xn);
+            const cplx<T> ev = cadd<T>(Xk, Xc);
+            const cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), twxl[k]);
+            work[idx] = mk<T>(ev.x - od.y, ev.y + od.x);
+        }
+        __syncthreads();
+        for (int l0 = 0; l0 < 3; l0 += maxl) fft_lds<T, EPT, true>(work + (size_t)l0 * N, min(maxl, 3 - l0), N, d, twl);
+        for (int idx = threadIdx.x; idx < 3 * N; idx += blockDim.x) {
+            const int c = fdiv(idx, a.fN), t = idx - c * N;
+            const cplx<T> z = work[idx];
+            v[(size_t)c * n + 2 * t] = z.x;
+            v[(size_t)c * n + 2 * t + 1] = z.y;
+        }
+        __syncthreads();
+        for (int x = threadIdx.x; x < n; x += blockDim.x)
+            mx = fmax(mx, fabs((double)v[x]) + fabs((double)v[n + x]) + fabs((double)v[2 * n + x]));
+        // six products, two batched r2c transforms of three
+        for (int grp = 0; grp < 2; ++grp) {
+            for (int idx = threadIdx.x; idx < 3 * N; idx += blockDim.x) {
+                const int pg = fdiv(idx, a.fN), t = idx - pg * N, p = 3 * grp + pg;
+                const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
+                const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
+                const T *va = v + (size_t)ca * n, *vb = v + (size_t)cb * n;
+                work[idx] = mk<T>(va[2 * t] * vb[2 * t], va[2 * t + 1] * vb[2 * t + 1]);
+            }
+            __syncthreads();
+            for (int l0 = 0; l0 < 3; l0 += maxl) fft_lds<T, EPT, false>(work + (size_t)l0 * N, min(maxl, 3 - l0), N, d, twl);
+            for (int idx = threadIdx.x; idx < 3 * K; idx += blockDim.x) {
+                const int pg = fdiv(idx, a.fK), kx = idx - pg * K, p = 3 * grp + pg;
+                const cplx<T> Zk = work[pg * N + (kx == N ? 0 : kx)];
+                const cplx<T> Zc = cconj<T>(work[pg * N + (kx == 0 ? 0 : N - kx)]);
+                const cplx<T> s = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
+                const cplx<T> wd = cmul<T>(twxl[kx], df);
+                a.P[kx + (size_t)K * L + p * pstride] = mk<T>(T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x));
+            }
+            __syncthreads();
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.vpart[blockIdx.x] = red[0];
+}
+
 // ----------------------------------------- y forward + RHS + IF stage
 
 // Forward y pass of the six product spectra with the divergence-form

@@ -54,8 +54,15 @@ def test_rhs_parity(N):
     assert math.isclose(s.scalar(dns._lib.lib().dns_spec_vsummax), vmax, rel_tol=1e-13)
 
 
+@pytest.mark.parametrize("xpass", [{}, {"DNS_SP_XRHS": "1"}, {"DNS_SP_XRHS": "1", "DNS_SP_XLPI": "3"},
+                                   {"DNS_SP_XRHS": "0"}])
 @pytest.mark.parametrize("N", [12, 18, 64])
-def test_step_parity(N):
+def test_step_parity(N, xpass, monkeypatch):
+    """Three IF-RK3 steps and their CFL velocity bound against the oracle,
+    with the persistent RHS x pass (16 or 3 lines per block: a partial last
+    block) and with the one-line-per-block kernel."""
+    for key, val in xpass.items():
+        monkeypatch.setenv(key, val)
     s, o = pair(N, visc=5e-3)
     uh = o.velocityfield(fx, fy, fz)
     ug = uh.copy(order="F")
@@ -129,7 +136,11 @@ def test_energy_dissipation_parity():
     assert st.e > 0 and st.diss > 0 and st.Re_tay > 0
 
 
-def test_mixed_precision_close_to_fp64():
+@pytest.mark.parametrize("xrhs", ["", "0"])
+def test_mixed_precision_close_to_fp64(xrhs, monkeypatch):
+    """FP32 transforms (persistent or one-line RHS x pass) stay within 1e-5 of FP64."""
+    if xrhs:
+        monkeypatch.setenv("DNS_SP_XRHS", xrhs)
     s64 = dns.spectral_setup(n=32, visc=5e-3)
     s32 = dns.spectral_setup(n=32, visc=5e-3, T=np.float32)
     uh = dns.spectral_randomfield_(dns.specvelocity(s64), s64, seed=1)
