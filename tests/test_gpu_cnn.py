@@ -51,6 +51,27 @@ def test_gradients_match_oracle(M, B):
         off += nw + co
 
 
+@pytest.mark.parametrize("env", [{"DNS_CNN_MFMA": "k8"}, {"DNS_CNN_CONV": "direct"}])
+def test_kernel_variants_match_oracle(env, monkeypatch):
+    """The 32x32x8 MFMA kernels (DNS_CNN_MFMA=k8) and the direct convolution
+    at M = 32 (DNS_CNN_CONV=direct): forward and gradients against the oracle."""
+    for key, val in env.items():
+        monkeypatch.setenv(key, val)
+    net, p, x, t = setup_case(B=1, M=32, seed=4)
+    y = net.forward(x)
+    yo, _, _ = oc.forward(oc.bf16_round(p), oc.bf16_round(x))
+    assert rel(y, yo) < 2e-2
+    loss = net.loss_grad(x, t)
+    g = net.grads()
+    lo, go = oc.loss_grad(oc.bf16_round(p), oc.bf16_round(x), t.astype(np.float64))
+    assert abs(loss - lo) < 2e-2 * lo
+    off = 0
+    for ci, co in oc.CH:
+        nw = 27 * co * ci
+        assert rel(g[off:off + nw], go[off:off + nw]) < 5e-2
+        off += nw + co
+
+
 def test_adam_update():
     net, p, x, t = setup_case(seed=2)
     net.loss_grad(x, t)
