@@ -343,7 +343,8 @@ template <typename T> struct FVImpl : FVBase {
             if (!thomas_tl) DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
             // equal segments of R rows (ny = S·R) use k_thomas_seg; DNS_THOMAS=tile keeps k_thomas_tile
             thomas_seg = 0;
-            if (thomas_tl && thomas_nthr == 256 && !(env && !strcmp(env, "tile"))) {
+            if (thomas_tl && thomas_nthr == 256 && !(env && !strcmp(env, "tile")) &&
+                (size_t)g.ldc * ny * sizeof(cplx<T>) < ((size_t)1 << 31)) {
                 const int S = thomas_nthr / thomas_tl, R = ny / S;
                 if (ny % S == 0 && (R == 4 || R == 6 || R == 8 || R == 12 || R == 16 || R == 24)) thomas_seg = R;
             }
@@ -373,7 +374,9 @@ template <typename T> struct FVImpl : FVBase {
                 if (!cn_tl) DNS_TRY(dalloc(&Dcn, (size_t)N));
                 // equal segments of R rows (ny = S·R) use k_cn_seg; DNS_CN=tile keeps k_cn_tile
                 cn_seg = 0;
-                if (cn_tl && cn_nthr == 256 && !(envc && !strcmp(envc, "tile"))) {
+                // (its buffer offsets are 32-bit byte offsets within one z plane)
+                if (cn_tl && cn_nthr == 256 && !(envc && !strcmp(envc, "tile")) &&
+                    (size_t)nx * ny * sizeof(T) < ((size_t)1 << 31)) {
                     const int S = cn_nthr / cn_tl, R = ny / S;
                     if (ny % S == 0 && (R == 8 || R == 16 || R == 24 || R == 32) &&
                         ldscn + (size_t)ny * sizeof(double) <= (size_t)maxlds)
@@ -1156,8 +1159,10 @@ int dns_fv_create(const dns_fv_params *P, dns_fv **out)
         set_error("local block smaller than ghost width; use fewer ranks");
         return DNS_EINVAL;
     }
+    // 32-bit element indices; the complex pencil's x extent is padded to whole
+    // 128-byte lines (at most 16 elements)
     if ((long long)P->n[0] * P->n[1] * P->n[2] >= (1LL << 31) ||
-        (long long)(P->n[0] / 2 + 1) * P->n[1] * P->n[2] >= (1LL << 31)) {
+        (long long)((P->n[0] / 2 + 1 + 15) / 16 * 16) * P->n[1] * P->n[2] >= (1LL << 31)) {
         set_error("grid too large for 32-bit indexing");
         return DNS_EUNSUPPORTED;
     }
