@@ -21,20 +21,29 @@ template <typename T> inline std::vector<cplx<T>> twiddles(int n, int len)
     return tw;
 }
 
+// Device copy of a host table.  hipMemcpy from pageable memory may return
+// once the data is staged, before the DMA lands, and the handles' streams are
+// non-blocking: wait for it before any kernel can read the table.
 template <typename V> inline int dupload(V **dptr, const std::vector<V> &h)
 {
     *dptr = nullptr;
     size_t bytes = std::max<size_t>(1, h.size()) * sizeof(V);
     DNS_HIP_CHECK(hipMalloc((void **)dptr, bytes));
     if (!h.empty()) DNS_HIP_CHECK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+    DNS_HIP_CHECK(hipDeviceSynchronize());
     return DNS_OK;
 }
 
+// Zero-initialised device allocation.  The zero fill runs on the null stream,
+// which has no implicit ordering with the handles' non-blocking streams, so
+// it is waited for here: otherwise a copy or kernel enqueued right after on a
+// handle's stream could run before (and be overwritten by) the fill.
 template <typename V> inline int dalloc(V **dptr, size_t count)
 {
     *dptr = nullptr;
     DNS_HIP_CHECK(hipMalloc((void **)dptr, std::max<size_t>(1, count) * sizeof(V)));
     DNS_HIP_CHECK(hipMemset(*dptr, 0, std::max<size_t>(1, count) * sizeof(V)));
+    DNS_HIP_CHECK(hipDeviceSynchronize());
     return DNS_OK;
 }
 
