@@ -516,6 +516,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the other configurations")
+    ap.add_argument("--spectral-n", type=int, default=0,
+                    help="run only the pseudo-spectral HIT form at this n (kcut = n/3; --precision fp64|mixed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scales the FETCH/WRITE counters)")
     ap.add_argument("--closure-experiment", action="store_true",
@@ -539,6 +541,10 @@ def main():
         res = run_closure_experiment(n=args.dns_n, M=args.les_m, nsnap=args.nsnap, train_steps=args.train_steps,
                                      log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(res), flush=True)
+        return
+    if args.spectral_n:
+        print(json.dumps(run_spectral(args.spectral_n, args.spectral_n // 3, args.precision, args.steps, args.warmup)),
+              flush=True)
         return
     if args.copy_calibration:
         import torch
