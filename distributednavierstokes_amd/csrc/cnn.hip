@@ -315,7 +315,11 @@ struct CNN {
         }
         hipLaunchKernelGGL(k_wgrad_reduce, dim3(256), dim3(256), 0, st, wpart, ns, c.np, c.kpt, c.nout, c.kin,
                            g + c.woff);
-        hipLaunchKernelGGL(k_bias_grad, dim3(bblocks, c.nout), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);
+        if (ldd <= 0 || 256 % ldd != 0 || c.nout > ldd) {
+            set_error("bias gradient: channel stride %d does not divide 256", ldd);
+            return DNS_EINVAL;
+        }
+        hipLaunchKernelGGL(k_bias_grad, dim3(bblocks), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);
         hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, st, bpart, bblocks, c.nout, g + c.boff);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;

@@ -497,21 +497,24 @@ __global__ void k_wgrad_reduce(const float *part, int nsplit, int np, int kp, in
     }
 }
 
-// Bias gradient: per-channel sum of dout over all voxels (per-block partials).
+// Bias gradient: per-channel sum of dout over all voxels (per-block
+// partials part[c · gridDim + block]).  A block reads whole voxel rows: thread
+// t takes channel t mod ldd of voxel rows t div ldd, + 256/ldd, ..., so a
+// wave reads contiguous channel-last memory and dout is read once (ldd
+// divides 256).  1D grid.
 __global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *dout, int ldd, int nout, long V, float *part)
 {
     __shared__ float s[256];
-    const int c = blockIdx.y;
+    const int c = threadIdx.x % ldd, r = threadIdx.x / ldd, rows = blockDim.x / ldd;
     float acc = 0;
-    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x)
-        acc += bf2f(dout[v * ldd + c]);
+    for (long v = blockIdx.x * (long)rows + r; v < V; v += (long)gridDim.x * rows) acc += bf2f(dout[v * ldd + c]);
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
+    if (r == 0 && c < nout) {
+        float t = 0;
+        for (int q = 0; q < rows; ++q) t += s[q * ldd + c];
+        part[(size_t)c * gridDim.x + blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) part[(size_t)c * gridDim.x + blockIdx.x] = s[0];
 }
 
 __global__ void k_bias_reduce(const float *part, int nblocks, int nout, float *grad)
