@@ -104,7 +104,7 @@ struct CNN {
     float *y = nullptr, *t = nullptr;
     float *wpart = nullptr, *bpart = nullptr;
     double *lpart = nullptr, *lval = nullptr, *pinned = nullptr;
-    int nsplit = 64, bblocks = 256;
+    int nsplit = 64, bblocks = 1024;
     static constexpr int kLossBlocks = 512;
     // dataset of filtered-DNS pairs (FP32 planar per sample: ū (3, M³), τ (6, M³))
     std::vector<float *> ds_u, ds_t;
@@ -315,8 +315,8 @@ struct CNN {
         }
         hipLaunchKernelGGL(k_wgrad_reduce, dim3(256), dim3(256), 0, st, wpart, ns, c.np, c.kpt, c.nout, c.kin,
                            g + c.woff);
-        if (ldd <= 0 || 256 % ldd != 0 || c.nout > ldd) {
-            set_error("bias gradient: channel stride %d does not divide 256", ldd);
+        if (ldd <= 0 || ldd % 8 != 0 || 2048 % ldd != 0 || c.nout > ldd) {
+            set_error("bias gradient: channel stride %d is not a multiple of 8 dividing 2048", ldd);
             return DNS_EINVAL;
         }
         hipLaunchKernelGGL(k_bias_grad, dim3(bblocks), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);

@@ -498,22 +498,38 @@ __global__ void k_wgrad_reduce(const float *part, int nsplit, int np, int kp, in
 }
 
 // Bias gradient: per-channel sum of dout over all voxels (per-block
-// partials part[c · gridDim + block]).  A block reads whole voxel rows: thread
-// t takes channel t mod ldd of voxel rows t div ldd, + 256/ldd, ..., so a
-// wave reads contiguous channel-last memory and dout is read once (ldd
-// divides 256).  1D grid.
+// partials part[c · gridDim + block]).  A thread reads 8 channels (16 bytes)
+// of a voxel row, ldd/8 threads cover a row and a block 256·8/ldd rows per
+// pass, so waves read contiguous channel-last memory, dout is read once and
+// each load moves 16 bytes (ldd a multiple of 8 dividing 2048).  1D grid.
 __global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *dout, int ldd, int nout, long V, float *part)
 {
-    __shared__ float s[256];
-    const int c = threadIdx.x % ldd, r = threadIdx.x / ldd, rows = blockDim.x / ldd;
-    float acc = 0;
-    for (long v = blockIdx.x * (long)rows + r; v < V; v += (long)gridDim.x * rows) acc += bf2f(dout[v * ldd + c]);
-    s[threadIdx.x] = acc;
+    __shared__ float s[8][256];
+    const int tpr = ldd / 8, c8 = threadIdx.x % tpr, r = threadIdx.x / tpr, rows = blockDim.x / tpr;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+    for (long v = blockIdx.x * (long)rows + r; v < V; v += (long)gridDim.x * rows) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(dout + v * ldd + 8 * c8);
+        const unsigned int w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            acc[2 * e] += bf2f((unsigned short)(w[e] & 0xffffu));
+            acc[2 * e + 1] += bf2f((unsigned short)(w[e] >> 16));
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e][threadIdx.x] = acc[e];
     __syncthreads();
-    if (r == 0 && c < nout) {
-        float t = 0;
-        for (int q = 0; q < rows; ++q) t += s[q * ldd + c];
-        part[(size_t)c * gridDim.x + blockIdx.x] = t;
+    // thread (row 0, c8) sums its 8 channels over the block's rows
+    if (r == 0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = 8 * c8 + e;
+            if (c >= nout) continue;
+            float t = 0;
+            for (int q = 0; q < rows; ++q) t += s[e][q * tpr + c8];
+            part[(size_t)c * gridDim.x + blockIdx.x] = t;
+        }
     }
 }
 
