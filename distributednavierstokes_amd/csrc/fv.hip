@@ -103,9 +103,22 @@ template <typename T> struct FVImpl : FVBase {
     T *pr = nullptr, *pwork = nullptr, *pghost = nullptr, *pyb[3] = {};
     cplx<T> *pB = nullptr, *pC = nullptr;
 
+    // HIP graphs of one fused step at a fixed Δt: the explicit form swaps the
+    // u / u′ buffers once per step, so there is one graph per buffer parity
+    // (index 0: u in the buffers allocated as u).  A step replays a graph
+    // when the same Δt was used by the previous step and no per-launch timing
+    // is on; DNS_FV_GRAPH=0 disables replay.
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double gdt[2] = {0.0, 0.0};
+    double last_dt = -1.0;
+    T *ubuf0 = nullptr;
+    bool graphs = true;
+
     ~FVImpl() override
     {
         if (st) hipStreamSynchronize(st);
+        for (auto &ge : gexec)
+            if (ge) hipGraphExecDestroy(ge);
         timer.flush();
         void *ptrs[] = {u[0], u[1], u[2], v[0], v[1], v[2], q[0], q[1], q[2], p, delta, A, Dth, Dcn, dy_t, rdy_t,
                         rdyu_t, dy_d, dyu_d, rdy_d, rdyu_d, lam1, lam2, lam3, ay, cy, iptab, ydc[0], ydc[1], ydc[2],
@@ -183,6 +196,8 @@ template <typename T> struct FVImpl : FVBase {
             DNS_TRY(dalloc(&q[c], N));
             if (!imexy) DNS_TRY(dalloc(&v[c], N));
         }
+        ubuf0 = u[0];
+        if (const char *eg = getenv("DNS_FV_GRAPH")) graphs = atoi(eg) != 0;
         DNS_TRY(dalloc(&p, N));
         if (imexy) DNS_TRY(dalloc(&delta, N));
         DNS_TRY(dalloc(&A, (size_t)g.ldc * ny * nz));
@@ -981,6 +996,44 @@ template <typename T> struct FVImpl : FVBase {
     int step(double dt) override
     {
         if (plain) return step_plain(dt);
+        const bool replay = graphs && !timer.on && dt == last_dt;
+        last_dt = dt;
+        if (!replay) return step_launch(dt);
+        const int key = u[0] == ubuf0 ? 0 : 1;
+        if (gexec[key] && gdt[key] == dt) {
+            DNS_HIP_CHECK(hipGraphLaunch(gexec[key], st));
+            // the host side of a step: the explicit form's three u / u′ swaps
+            // (one net swap) and the CFL partials left by the last pgrad
+            if (!imexy)
+                for (int c = 0; c < 3; ++c) std::swap(u[c], v[c]);
+            umax_valid = true;
+            return DNS_OK;
+        }
+        if (gexec[key]) {
+            DNS_HIP_CHECK(hipGraphExecDestroy(gexec[key]));
+            gexec[key] = nullptr;
+        }
+        // capture the launches of one step (its host-side state changes happen
+        // now), then run the graph once for this step
+        hipGraph_t graph = nullptr;
+        DNS_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = step_launch(dt);
+        const hipError_t ec = hipStreamEndCapture(st, &graph);
+        if (rc != DNS_OK) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        DNS_HIP_CHECK(ec);
+        const hipError_t ei = hipGraphInstantiate(&gexec[key], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        DNS_HIP_CHECK(ei);
+        gdt[key] = dt;
+        DNS_HIP_CHECK(hipGraphLaunch(gexec[key], st));
+        return DNS_OK;
+    }
+
+    int step_launch(double dt)
+    {
         for (int s = 0; s < 3; ++s) {
             if (!imexy) {
                 DNS_TRY(launch_momentum(u, v, RK_A[s], dt, RK_B[s], true));

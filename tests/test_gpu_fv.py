@@ -185,6 +185,44 @@ def test_cn_variants_step_parity(n, variant, monkeypatch):
     assert relmax(pg, po) < 1e-10
 
 
+@pytest.mark.parametrize("graph", ["", "0"])
+@pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
+def test_fixed_dt_graph_replay_parity(name, graph, monkeypatch):
+    """Steps with a repeated Δt replay captured HIP graphs (one per u / u′
+    buffer parity); a new Δt recaptures.  Same trajectory as the oracle with
+    replay on and off (DNS_FV_GRAPH=0)."""
+    if graph:
+        monkeypatch.setenv("DNS_FV_GRAPH", graph)
+    prob = problems()[name]
+    s, o = pair(prob)
+    u = randomfield(prob, seed=25, halo=o.exchange_halo_u)
+    o.project(u, zeros(prob))
+    ug = [a.copy(order="F") for a in u]
+    po, pg = zeros(prob), zeros(prob)
+    qo, qg = vzeros(prob), vzeros(prob)
+    do, dg = (zeros(prob), zeros(prob)) if prob.implicit else (po, pg)
+    for dt in (0.004, 0.004, 0.004, 0.003, 0.003, 0.004, 0.004, 0.004):
+        o.step(u, po, qo, do, dt)
+    s.upload_u(ug)
+    s.upload("p", pg)
+    for c, nm in enumerate(("qx", "qy", "qz")):
+        s.upload(nm, qg[c])
+    if prob.implicit:
+        s.upload("delta", dg)
+    L = dns._lib.lib()
+    for dt in (0.004, 0.004, 0.004, 0.003, 0.003, 0.004, 0.004, 0.004):
+        dns._lib.check(L.dns_fv_step(s.handle, dt))
+    # CFL from the partial maxima the last (replayed) pressure-gradient pass left
+    out = np.zeros(1)
+    dns._lib.check(L.dns_fv_cfl_timestep(s.handle, 0.5, dns._lib.dptr(out)))
+    s.download_u(ug)
+    s.download("p", pg)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11, c
+    assert relmax(pg, po) < 1e-10
+    assert math.isclose(float(out[0]), o.cfl_timestep(u, 0.5), rel_tol=1e-13)
+
+
 @pytest.mark.parametrize("name", ["periodic", "channel", "channel_cn"])
 def test_plain_step_parity(name):
     """The reference-structured kernel sequence (the measurement baseline)
