@@ -318,6 +318,24 @@ def test_taylor_green_64_parity():
         assert relmax(ug[c], u[c]) < 1e-11
 
 
+def test_taylor_green_64_full_config_parity():
+    """The 64³ periodic Taylor–Green configuration in full: ν = 1/1600, fixed
+    Δt = 1e-3, 200 Williamson RK3 steps (steps 2–200 replay a captured HIP
+    graph), against the oracle: velocity and kinetic energy."""
+    prob = FVProblem(n=(64, 64, 64), visc=1 / 1600)
+    s, o = pair(prob)
+    u = vzeros(prob)
+    fill_velocity(prob, u, lambda x, y, z: math.sin(x) * math.cos(y) * math.cos(z),
+                  lambda x, y, z: -math.cos(x) * math.sin(y) * math.cos(z))
+    o.exchange_halo_u(u)
+    ug = [a.copy(order="F") for a in u]
+    o.solve(u, (0.0, 0.2), dt=1e-3)
+    dns.solve_(u=ug, setup=s, tlims=(0.0, 0.2), dt=1e-3)
+    for c in range(3):
+        assert relmax(ug[c], u[c]) < 1e-11
+    assert math.isclose(dns.kinetic_energy(ug, s), o.kinetic_energy(u), rel_tol=1e-12)
+
+
 def test_channel_cfl_trajectory_parity():
     """Channel with walls, tanh stretching and CFL-adaptive Δt: 10 steps."""
     prob = channelproblem(n=(32, 24, 16), lims=((0, 4 * math.pi), (-1, 1), (0, 4 * math.pi / 3)),
