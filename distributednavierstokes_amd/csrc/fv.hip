@@ -46,6 +46,7 @@ struct FVBase {
     virtual int project(int psi) = 0;
     virtual int step(double dt) = 0;
     virtual int cfl(double cfl, double *dt) = 0;
+    virtual int advance_cfl(double cfl, int nsteps, double tend, double *t, double *dts) = 0;
     virtual int maxdiv(double *out) = 0;
     virtual int kinetic(double *out) = 0;
     virtual int chstats(double *acc) = 0;
@@ -113,12 +114,24 @@ template <typename T> struct FVImpl : FVBase {
     double last_dt = -1.0;
     T *ubuf0 = nullptr;
     bool graphs = true;
+    // CFL-adaptive explicit steps with Δt computed on the device (advance_cfl):
+    // graphs per buffer parity of [CFL reduction, Δt, step], keyed by (cfl, tend)
+    hipGraphExec_t gcfl[2] = {nullptr, nullptr};
+    double gcfl_key[2][2] = {{0, 0}, {0, 0}};
+    const double *mom_dtp = nullptr; // set while capturing: momentum reads Δt here
+    double *ddt = nullptr, *dtime = nullptr, *dlog = nullptr;
+    int *dstep = nullptr;
+    int nlog = 0;
 
     ~FVImpl() override
     {
         if (st) hipStreamSynchronize(st);
         for (auto &ge : gexec)
             if (ge) hipGraphExecDestroy(ge);
+        for (auto &ge : gcfl)
+            if (ge) hipGraphExecDestroy(ge);
+        for (void *q_ : {(void *)ddt, (void *)dtime, (void *)dlog, (void *)dstep})
+            if (q_) hipFree(q_);
         timer.flush();
         void *ptrs[] = {u[0], u[1], u[2], v[0], v[1], v[2], q[0], q[1], q[2], p, delta, A, Dth, Dcn, dy_t, rdy_t,
                         rdyu_t, dy_d, dyu_d, rdy_d, rdyu_d, lam1, lam2, lam3, ay, cy, iptab, ydc[0], ydc[1], ydc[2],
@@ -552,6 +565,7 @@ template <typename T> struct FVImpl : FVBase {
         a.rdyu = rdyu_t;
         a.A = (T)Ac;
         a.dt = (T)dt;
+        a.dtp = mom_dtp;
         a.nu = (T)visc;
         a.fx = (T)f[0];
         a.fy = (T)f[1];
@@ -1057,6 +1071,106 @@ template <typename T> struct FVImpl : FVBase {
         return DNS_OK;
     }
 
+    // nsteps CFL-adaptive steps (solve! with cfl, src/timestep.jl:92-111): Δt =
+    // min(cfl_timestep, tend − t) each step.  The explicit form computes Δt on
+    // the device from the pressure-gradient partials and replays one captured
+    // graph per buffer parity, so no step waits for the host; other forms (or
+    // per-launch timing) run the host loop.  t is advanced; dts (optional)
+    // receives every Δt.
+    int advance_cfl(double cflv, int nsteps, double tend, double *t_io, double *dts) override
+    {
+        if (nsteps <= 0) return DNS_OK;
+        double t = *t_io;
+        int done = 0;
+        auto host_step = [&]() -> int {
+            double d = 0;
+            DNS_TRY(cfl(cflv, &d));
+            d = std::min(d, tend - t);
+            DNS_TRY(step(d));
+            t += d;
+            if (dts) dts[done] = d;
+            ++done;
+            return DNS_OK;
+        };
+        if (plain || imexy || timer.on || !graphs) {
+            while (done < nsteps) DNS_TRY(host_step());
+            *t_io = t;
+            return DNS_OK;
+        }
+        if (!umax_valid) DNS_TRY(host_step()); // the device Δt needs a pressure pass's partials
+        if (done == nsteps) {
+            *t_io = t;
+            return DNS_OK;
+        }
+        if (!ddt) {
+            DNS_TRY(dalloc(&ddt, 1));
+            DNS_TRY(dalloc(&dtime, 1));
+            DNS_TRY(dalloc(&dstep, 1));
+        }
+        if (nlog < nsteps) {
+            if (dlog) DNS_HIP_CHECK(hipFree(dlog));
+            dlog = nullptr;
+            DNS_TRY(dalloc(&dlog, (size_t)nsteps));
+            nlog = nsteps;
+            for (auto &ge : gcfl)
+                if (ge) {
+                    DNS_HIP_CHECK(hipGraphExecDestroy(ge));
+                    ge = nullptr;
+                }
+        }
+        const int step0 = done;
+        DNS_HIP_CHECK(hipMemcpyAsync(dtime, &t, sizeof(double), hipMemcpyHostToDevice, st));
+        DNS_HIP_CHECK(hipMemcpyAsync(dstep, &step0, sizeof(int), hipMemcpyHostToDevice, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st)); // the host sources are locals
+        const double eps = 2.220446049250313e-16;
+        const double invdy2 = 1 / (dymin * dymin);
+        const double diff = 1 / (2 * visc * (1 / (dx * dx) + invdy2 + 1 / (dz * dz)) + eps);
+        const int G = std::min(512, std::max(1, nb_pgrad / 1024));
+        for (; done < nsteps; ++done) {
+            const int key = u[0] == ubuf0 ? 0 : 1;
+            if (!gcfl[key] || gcfl_key[key][0] != cflv || gcfl_key[key][1] != tend) {
+                if (gcfl[key]) {
+                    DNS_HIP_CHECK(hipGraphExecDestroy(gcfl[key]));
+                    gcfl[key] = nullptr;
+                }
+                hipGraph_t graph = nullptr;
+                DNS_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                hipLaunchKernelGGL(k_max_mid, dim3(G), dim3(NTHR), 0, st, upart, nb_pgrad, 3, red);
+                hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, red, G, 3, redout);
+                hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, st, redout, dx, dymin, dz, diff, cflv, tend, dtime,
+                                   ddt, dlog, dstep);
+                mom_dtp = ddt;
+                const int rc = step_launch(0.0);
+                mom_dtp = nullptr;
+                const hipError_t ec = hipStreamEndCapture(st, &graph);
+                if (rc != DNS_OK) {
+                    if (graph) hipGraphDestroy(graph);
+                    return rc;
+                }
+                DNS_HIP_CHECK(ec);
+                const hipError_t ei = hipGraphInstantiate(&gcfl[key], graph, nullptr, nullptr, 0);
+                hipGraphDestroy(graph);
+                DNS_HIP_CHECK(ei);
+                gcfl_key[key][0] = cflv;
+                gcfl_key[key][1] = tend;
+                // the capture already advanced the host state (buffer swaps)
+                DNS_HIP_CHECK(hipGraphLaunch(gcfl[key], st));
+            } else {
+                DNS_HIP_CHECK(hipGraphLaunch(gcfl[key], st));
+                for (int c = 0; c < 3; ++c) std::swap(u[c], v[c]);
+            }
+            umax_valid = true;
+        }
+        last_dt = -1.0; // the fixed-Δt graphs stay valid, but the next step launches directly
+        if (dts)
+            DNS_HIP_CHECK(hipMemcpyAsync(dts + step0, dlog + step0, sizeof(double) * (nsteps - step0),
+                                         hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, dtime, sizeof(double), hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        *t_io = pinned[0];
+        return DNS_OK;
+    }
+
     int cfl(double cflv, double *dt) override
     {
         if (umax_valid) {
@@ -1292,6 +1406,12 @@ int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt)
     FV_CHECK(h);
     if (!dt) return DNS_EINVAL;
     return h->impl->cfl(cfl, dt);
+}
+int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts)
+{
+    FV_CHECK(h);
+    if (!t || nsteps < 0) return DNS_EINVAL;
+    return h->impl->advance_cfl(cfl, nsteps, tend, t, dts);
 }
 int dns_fv_maxdiv(dns_fv *h, double *out)
 {

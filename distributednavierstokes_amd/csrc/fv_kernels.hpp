@@ -84,6 +84,7 @@ template <typename T> struct MomentumArgs {
     const T *dy, *rdy, *rdyu; // ghosted y metrics and reciprocals
     T A, dt, nu, fx, fy, fz, epsy, B;
     T rdx, rdx2, rdz, rdz2;
+    const double *dtp; // when set, Δt is read here (computed on the device) instead of dt
 };
 
 // The three momentum components at one cell from its 27 stencil values
@@ -120,6 +121,7 @@ __device__ inline void momentum_core(const MomentumArgs<T> &a, const FVGeom &g, 
     const T Dyj = m.Dyj, Dyp = m.Dyp;
     const T rDyj = m.rDyj, rDyp = m.rDyp, rDyuj = m.rDyuj, rDyum = m.rDyum;
     const T h = T(0.5);
+    const T dt = a.dtp ? T(*a.dtp) : a.dt;
 
     // x momentum at the right x-face of the cell
     {
@@ -132,7 +134,7 @@ __device__ inline void momentum_core(const MomentumArgs<T> &a, const FVGeom &g, 
         T diff = (uxp00 - T(2) * ux000 + uxm00) * a.rdx2 +
                  a.epsy * ((ux0p0 - ux000) * rDyuj - (ux000 - ux0m0) * rDyum) * rDyj +
                  (ux00p - T(2) * ux000 + ux00m) * a.rdz2;
-        T q = a.dt * (-conv + a.nu * diff + a.fx);
+        T q = dt * (-conv + a.nu * diff + a.fx);
         if (READ_Q) q += a.A * qox;
         st_stream(q, a.qx + c, NT);
         if (WRITE_V) st_stream(ux000 + a.B * q, a.vx + c, NT);
@@ -152,7 +154,7 @@ __device__ inline void momentum_core(const MomentumArgs<T> &a, const FVGeom &g, 
             T diff = (uyp00 - T(2) * uy000 + uym00) * a.rdx2 +
                      a.epsy * ((uy0p0 - uy000) * rDyp - (uy000 - uy0m0) * rDyj) * rDyuj +
                      (uy00p - T(2) * uy000 + uy00m) * a.rdz2;
-            T q = a.dt * (-conv + a.nu * diff + a.fy);
+            T q = dt * (-conv + a.nu * diff + a.fy);
             if (READ_Q) q += a.A * qoy;
             st_stream(q, a.qy + c, NT);
             if (WRITE_V) st_stream(uy000 + a.B * q, a.vy + c, NT);
@@ -171,7 +173,7 @@ __device__ inline void momentum_core(const MomentumArgs<T> &a, const FVGeom &g, 
         T diff = (uzp00 - T(2) * uz000 + uzm00) * a.rdx2 +
                  a.epsy * ((uz0p0 - uz000) * rDyuj - (uz000 - uz0m0) * rDyum) * rDyj +
                  (uz00p - T(2) * uz000 + uz00m) * a.rdz2;
-        T q = a.dt * (-conv + a.nu * diff + a.fz);
+        T q = dt * (-conv + a.nu * diff + a.fz);
         if (READ_Q) q += a.A * qoz;
         st_stream(q, a.qz + c, NT);
         if (WRITE_V) st_stream(uz000 + a.B * q, a.vz + c, NT);
@@ -1023,6 +1025,23 @@ __global__ void __launch_bounds__(256, 2) k_thomas_seg(cplx<T> *__restrict__ A, 
         P *= (j == ny - 1 || (j == 0 && mean)) ? 0.0 : -sc[j] * w[q];
         bstore(ra, va, arow * q, mk<T>(T(x[q].x + P * xi.x), T(x[q].y + P * xi.y)));
     }
+}
+
+// Δt of a CFL-adaptive step on the device (cfl_timestep, src/timestep.jl:69-80,
+// the same expression as the host's): from the per-component velocity maxima,
+// clamped to the time left before tend; advances the device time and logs Δt.
+// One thread.
+__global__ void k_cfl_dt(const double *umax, double dx, double dymin, double dz, double diff, double cfl,
+                         double tend, double *t, double *dt, double *dt_log, int *step)
+{
+    const double eps = 2.220446049250313e-16;
+    const double conv = fmin(fmin(dx / (umax[0] + eps), dymin / (umax[1] + eps)), dz / (umax[2] + eps));
+    double d = fmin(cfl * conv, cfl * diff);
+    d = fmin(d, tend - *t);
+    *dt = d;
+    *t += d;
+    dt_log[*step] = d;
+    *step += 1;
 }
 
 // ------------------------------------------------------------------------

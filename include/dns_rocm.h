@@ -105,6 +105,13 @@ int dns_fv_step(dns_fv *h, double dt);
 /* cfl_timestep(u, s; cfl) -- src/timestep.jl:69-80. */
 int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt);
 
+/* The CFL branch of solve! (src/timestep.jl:92-111) for nsteps steps:
+ * Δt = min(cfl_timestep(u, s; cfl), tend − t) then step!, with *t advanced
+ * and every Δt stored in dts (may be null).  In the explicit form Δt is
+ * computed on the device and the steps replay captured HIP graphs without
+ * waiting for the host; returns after the last step. */
+int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts);
+
 /* maxdiv(u, s) -- src/fields.jl:38-42 (divergence residual in FP64). */
 int dns_fv_maxdiv(dns_fv *h, double *out);
 

@@ -336,6 +336,60 @@ def test_taylor_green_64_full_config_parity():
     assert math.isclose(dns.kinetic_energy(ug, s), o.kinetic_energy(u), rel_tol=1e-12)
 
 
+@pytest.mark.parametrize("name,prec", [("channel", "fp64"), ("periodic", "fp64"), ("channel", "mixed"),
+                                       ("channel_cn", "fp64")])
+def test_advance_cfl_matches_host_loop(name, prec):
+    """dns_fv_advance_cfl (device Δt and replayed graphs in the explicit form)
+    reproduces the host loop of cfl_timestep + step! bit for bit: same Δt
+    sequence, same fields, and the tend clamp of solve!."""
+    prob = dataclasses.replace(problems()[name], precision=prec)
+    o = OracleFV(problems()[name])
+    u0 = randomfield(prob, seed=27, halo=o.exchange_halo_u)
+    o.project(u0, zeros(prob))
+    L = dns._lib.lib()
+    nsteps, cfl = 7, 0.5
+
+    def fresh():
+        s = Setup(prob, ROCmBackend())
+        s.upload_u([a.copy(order="F") for a in u0])
+        for nm in ("p", "qx", "qy", "qz"):
+            s.upload(nm, zeros(prob))
+        if prob.implicit:
+            s.upload("delta", zeros(prob))
+        return s
+
+    # host loop with the tend clamp on the last step
+    sh = fresh()
+    out = np.zeros(1)
+    t, dts_host = 0.0, []
+    for n in range(nsteps):
+        dns._lib.check(L.dns_fv_cfl_timestep(sh.handle, cfl, dns._lib.dptr(out)))
+        dts_host.append(float(out[0]))
+        t += dts_host[-1]
+    tend = sum(dts_host[:-1]) + 0.5 * dts_host[-1]
+    sh = fresh()
+    t, dts_host = 0.0, []
+    for n in range(nsteps):
+        dns._lib.check(L.dns_fv_cfl_timestep(sh.handle, cfl, dns._lib.dptr(out)))
+        dt = min(float(out[0]), tend - t)
+        dns._lib.check(L.dns_fv_step(sh.handle, dt))
+        dts_host.append(dt)
+        t += dt
+    uh = vzeros(prob)
+    sh.download_u(uh)
+    # device loop
+    sd = fresh()
+    tdev = np.zeros(1)
+    dts = np.zeros(nsteps)
+    dns._lib.check(L.dns_fv_advance_cfl(sd.handle, cfl, nsteps, tend, dns._lib.dptr(tdev), dns._lib.dptr(dts)))
+    ud = vzeros(prob)
+    sd.download_u(ud)
+    assert np.array_equal(dts, np.array(dts_host))
+    assert tdev[0] == t
+    for c in range(3):
+        assert np.array_equal(ud[c], uh[c]), c
+
+
 def test_channel_cfl_trajectory_parity():
     """Channel with walls, tanh stretching and CFL-adaptive Δt: 10 steps."""
     prob = channelproblem(n=(32, 24, 16), lims=((0, 4 * math.pi), (-1, 1), (0, 4 * math.pi / 3)),
