@@ -256,6 +256,16 @@ def cfl_timestep(u, s, cfl):
     return float(out[0])
 
 
+def advance_cfl_(s, cfl, nsteps, tend=math.inf, t=0.0):
+    """``nsteps`` CFL steps of ``solve!`` (src/timestep.jl:92-111) on the
+    device-resident state: Δt = min(cfl_timestep, tend − t) each step,
+    computed on the device in the explicit form.  Returns (t, Δt array)."""
+    tt = np.array([float(t)])
+    dts = np.zeros(max(1, int(nsteps)))
+    check(lib().dns_fv_advance_cfl(s.handle, float(cfl), int(nsteps), float(tend), dptr(tt), dptr(dts)))
+    return float(tt[0]), dts[:int(nsteps)]
+
+
 def maxdiv(u, s):
     """``maxdiv(u, s)`` -- src/fields.jl:38-42 (FP64 residual)."""
     s.upload_u(u)
