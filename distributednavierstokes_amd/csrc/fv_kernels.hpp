@@ -99,6 +99,13 @@ template <typename T> __device__ inline void st_stream(T v, T *p, bool nt)
     else *p = v;
 }
 
+// Device-computed Δt: read once at kernel entry, before any store, so the
+// load is a uniform scalar load and Δt stays in scalar registers.
+template <typename T> __device__ inline void resolve_dt(MomentumArgs<T> &a)
+{
+    if (a.dtp) a.dt = T(*a.dtp);
+}
+
 // Metrics of row j (ghosted index jj = j + 1): Δy at jj and jj+1, their
 // reciprocals, 1/Δyu at jj and jj−1.
 template <typename T> struct RowMetrics {
@@ -121,7 +128,7 @@ __device__ inline void momentum_core(const MomentumArgs<T> &a, const FVGeom &g, 
     const T Dyj = m.Dyj, Dyp = m.Dyp;
     const T rDyj = m.rDyj, rDyp = m.rDyp, rDyuj = m.rDyuj, rDyum = m.rDyum;
     const T h = T(0.5);
-    const T dt = a.dtp ? T(*a.dtp) : a.dt;
+    const T dt = a.dt;
 
     // x momentum at the right x-face of the cell
     {
@@ -198,6 +205,7 @@ __device__ inline void momentum_cell(const MomentumArgs<T> &a, const FVGeom &g, 
 template <typename T, bool READ_Q, bool WRITE_V>
 __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
 {
+    resolve_dt(a);
     int bx, by, bz;
     xcd_block3(bx, by, bz);
     const int i = bx * 64 + threadIdx.x;
@@ -238,6 +246,7 @@ __global__ void __launch_bounds__(256) k_momentum(MomentumArgs<T> a, FVGeom g)
 template <typename T, bool READ_Q, bool WRITE_V>
 __global__ void __launch_bounds__(256) k_momentum_slab(MomentumArgs<T> a, FVGeom g, int nbx, int nbys)
 {
+    resolve_dt(a);
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     int t = lb;
     const int bx = t % nbx;
@@ -300,6 +309,7 @@ template <typename T> struct LZ {
 template <typename T, bool READ_Q, bool WRITE_V>
 __global__ void __launch_bounds__(256) k_momentum_lz(MomentumArgs<T> a, FVGeom g, int zc)
 {
+    resolve_dt(a);
     using L = LZ<T>;
     using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     __shared__ __align__(16) T sm[3 * L::PL];
