@@ -7,7 +7,7 @@ reference (``step!``) are spelled with a trailing underscore (``step_``).
 from .problem import DNSError, FVProblem, SpectralProblem, tanhstretch  # noqa: F401
 from .fv import (  # noqa: F401
     ROCmBackend, setup, scalarfield, vectorfield, velocityfield_, exchange_halo_, project_, momentum_, step_,
-    cfl_timestep, solve_, maxdiv, kinetic_energy, channelstats, channelprofiles, spectrumstats, energyspectrum, rk3,
+    cfl_timestep, advance_cfl_, solve_, maxdiv, kinetic_energy, channelstats, channelprofiles, spectrumstats, energyspectrum, rk3,
 )
 from .fv import VectorField  # noqa: F401
 from .spectral import (  # noqa: F401

@@ -406,6 +406,40 @@ def test_channel_cfl_trajectory_parity():
         assert relmax(ug[c], u[c]) < 1e-10
 
 
+def test_advance_cfl_helper_matches_oracle():
+    """advance_cfl_ (Δt from cfl_timestep computed on the device, clamped to
+    tend: the loop of src/timestep.jl:92-111 after the initial projection)
+    against the oracle's cfl_timestep + step! loop: same Δt sequence, the
+    half-length last step, the end time and the fields."""
+    prob = channelproblem(n=(32, 24, 16), lims=((0, 4 * math.pi), (-1, 1), (0, 4 * math.pi / 3)),
+                          visc=1 / 180, bodyforce=(1.0, 0.0, 0.0))
+    s, o = pair(prob)
+    u = randomfield(prob, seed=13)
+    fill_velocity(prob, u, lambda x, y, z: 1 - y ** 2)
+    o.exchange_halo_u(u)
+    o.project(u, zeros(prob))
+    ug = [a.copy(order="F") for a in u]
+    p, q = o.scalarfield(), o.vectorfield()
+    dt0 = o.cfl_timestep(u, 0.5)
+    t, tend, dts = 0.0, 4.5 * dt0, []
+    while t < tend - 1e-10 * (abs(tend) + 1):
+        dt = min(o.cfl_timestep(u, 0.5), tend - t)
+        o.step(u, p, q, p, dt)
+        dts.append(dt)
+        t += dt
+    assert len(dts) >= 4 and dts[-1] < 0.9 * dts[-2]  # the last step is clamped
+    s.upload_u(ug)
+    for nm in ("p", "qx", "qy", "qz"):
+        s.upload(nm, zeros(prob))
+    tg, dtg = dns.advance_cfl_(s, 0.5, len(dts), tend)
+    ud = vzeros(prob)
+    s.download_u(ud)
+    np.testing.assert_allclose(dtg, dts, rtol=1e-12, atol=0)
+    assert math.isclose(tg, t, rel_tol=1e-12)
+    for c in range(3):
+        assert relmax(ud[c], u[c]) < 1e-10
+
+
 @pytest.mark.parametrize("n", [(32, 24, 16), (32, 128, 8)])
 def test_mixed_precision_error_is_small(n):
     prob = channelproblem(n=n, visc=0.02, bodyforce=(1.0, 0.0, 0.0), ydiffusion="implicit")
