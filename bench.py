@@ -28,55 +28,7 @@ HBM_PEAK_GBS = 8000.0
 FAMILIES = {0: "momentum_rk", 1: "x_pass_fft", 2: "z_pass_fft", 3: "y_solve", 4: "pressure_gradient", 5: "cn_ydiffusion"}
 
 
-def splitmix64(x):
-    x = (x + np.uint64(0x9E3779B97F4A7C15)).astype(np.uint64)
-    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)).astype(np.uint64)
-    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)).astype(np.uint64)
-    return x ^ (x >> np.uint64(31))
-
-
-def channel_ic(prob, seed=0, amp=0.1):
-    """Laminar 1 − y² plus amp·(2r − 1) on active DOFs (ghosted arrays)."""
-    nx, ny, nz = prob.n
-    co = prob.coordinates()
-    u = [np.zeros(prob.ghosted_shape, order="F") for _ in range(3)]
-    i = np.arange(nx, dtype=np.uint64)[:, None, None]
-    j = np.arange(ny, dtype=np.uint64)[None, :, None]
-    with np.errstate(over="ignore"):
-        for c in range(3):
-            for k0 in range(0, nz, 16):  # k-slabs bound the temporaries
-                k = np.arange(k0, min(nz, k0 + 16), dtype=np.uint64)[None, None, :]
-                key = ((np.uint64(c) * np.uint64(nz) + k) * np.uint64(ny) + j) * np.uint64(nx) + i
-                r = ((splitmix64(np.uint64(seed) + key) >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
-                u[c][1:-1, 1:-1, 1 + k0:1 + k0 + k.shape[2]] = amp * (2 * r - 1)
-    u[0][1:-1, 1:-1, 1:-1] += (1 - co["yc"] ** 2)[None, :, None]
-    if prob.wall:
-        u[1][:, -2, :] = 0.0  # top wall face
-    return u
-
-
-def channel_problem(n, ydiffusion="explicit", precision="fp64"):
-    from distributednavierstokes_amd.problem import FVProblem, tanhstretch
-
-    return FVProblem(n=n, lims=((0.0, 4 * math.pi), (-1.0, 1.0), (0.0, 4 * math.pi / 3)),
-                     bc=("periodic", "wall", "periodic"), stretch=tanhstretch(1.5), visc=1 / 180,
-                     bodyforce=(1.0, 0.0, 0.0), ydiffusion=ydiffusion, precision=precision)
-
-
-def tgv_problem(n=64):
-    from distributednavierstokes_amd.problem import FVProblem
-
-    return FVProblem(n=(n, n, n), visc=1 / 1600)
-
-
-def tgv_ic(prob):
-    co = prob.coordinates()
-    u = [np.zeros(prob.ghosted_shape, order="F") for _ in range(3)]
-    u[0][1:-1, 1:-1, 1:-1] = (np.sin(co["xf"])[:, None, None] * np.cos(co["yc"])[None, :, None]
-                              * np.cos(co["zc"])[None, None, :])
-    u[1][1:-1, 1:-1, 1:-1] = (-np.cos(co["xc"])[:, None, None] * np.sin(co["yf"])[None, :, None]
-                              * np.cos(co["zc"])[None, None, :])
-    return u
+from distributednavierstokes_amd.workloads import channel_ic, channel_problem, tgv_ic, tgv_problem  # noqa: E402
 
 
 def spectral_mmin_bytes(n, kcut, esize):
@@ -465,7 +417,7 @@ def run_gpu(prob, steps, warmup, cfl=0.5, fixed_dt=None, ic=None, events=True):
         nonlocal t
         if fixed_dt is None and not events:
             tt = np.array([t])
-            _lib.check(L.dns_fv_advance_cfl(s.handle, cfl, n, math.inf, _lib.dptr(tt), None))
+            _lib.check(L.dns_fv_advance_cfl(s.handle, cfl, n, math.inf, _lib.dptr(tt), None, None))
             t = float(tt[0])
         else:
             for _ in range(n):

@@ -46,12 +46,13 @@ struct FVBase {
     virtual int project(int psi) = 0;
     virtual int step(double dt) = 0;
     virtual int cfl(double cfl, double *dt) = 0;
-    virtual int advance_cfl(double cfl, int nsteps, double tend, double *t, double *dts) = 0;
+    virtual int advance_cfl(double cfl, int nsteps, double tend, double *t, double *dts, int *ndone) = 0;
     virtual int maxdiv(double *out) = 0;
     virtual int kinetic(double *out) = 0;
     virtual int chstats(double *acc) = 0;
     virtual int spectrum(double *E, int nbins) = 0;
     virtual int sync() = 0;
+    virtual int describe(char *buf, int len) = 0;
     LaunchTimer timer;
 };
 
@@ -95,8 +96,12 @@ template <typename T> struct FVImpl : FVBase {
     int lpb = 1, tbz = 1, tby = 1, tbzf = 1, tbyf = 1; // tile lines and FFT sub-batch lines
     int nthrz = 256, nthry = 256;                      // threads of the z / periodic-y FFT blocks
     size_t ldsx = 0, ldsz = 0, ldsy = 0;
+    // red/redout: reduction partials of fixed size, captured by the CFL graphs
+    // and therefore never reallocated; spectra use their own grow-only pair
     double *red = nullptr, *redout = nullptr, *upart = nullptr;
     size_t redcap = 0;
+    double *spart = nullptr, *sout = nullptr;
+    size_t spartcap = 0, soutcap = 0;
     double *pinned = nullptr;
     int N = 0;
     int nb_pgrad = 0;
@@ -139,8 +144,8 @@ template <typename T> struct FVImpl : FVBase {
         timer.flush();
         void *ptrs[] = {u[0], u[1], u[2], v[0], v[1], v[2], q[0], q[1], q[2], p, delta, A, Dth, Dcn, dy_t, rdy_t,
                         rdyu_t, dy_d, dyu_d, rdy_d, rdyu_d, lam1, lam2, lam3, ay, cy, iptab, ydc[0], ydc[1], ydc[2],
-                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout, upart, pr, pwork,
-                        pghost, pyb[0], pyb[1], pyb[2], pB, pC};
+                        ydc[3], ydc[4], ydc[5], cncoef, twN, twx, twz, twy, red, redout, upart, spart, sout,
+                        pr, pwork, pghost, pyb[0], pyb[1], pyb[2], pB, pC};
         for (void *q_ : ptrs)
             if (q_) hipFree(q_);
         if (pinned) hipHostFree(pinned);
@@ -588,10 +593,7 @@ template <typename T> struct FVImpl : FVBase {
             // over 256 CUs (measured: longer marches win until the grid stops
             // filling the chip; a partial last round costs a whole round)
             const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 8);
-            const long round = 256L * (sizeof(T) == 8 ? 3 : 6);
-            int zc = 32;
-            while (zc > 2 && (long)nbx * nby * ceil_div(g.nz, zc) < round) zc /= 2;
-            if (const char *ez = getenv("DNS_MOM_ZC")) zc = std::max(1, atoi(ez)); // sweeps
+            const int zc = momentum_zc();
             dim3 grd(nbx, nby, ceil_div(g.nz, zc));
             if (write_v) {
                 if (rq) hipLaunchKernelGGL((k_momentum_lz<T, true, true>), grd, dim3(256), 0, st, a, g, zc);
@@ -710,13 +712,14 @@ template <typename T> struct FVImpl : FVBase {
                            twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
-        XInvArgs<T> xi;
+        XInvArgs<T> xi{};
         xi.A = A;
         xi.out = psi;
         xi.tw = twN;
         xi.twx = twx;
         xi.lpb = lpb;
         xi.fN = make_fastdiv(g.nx / 2);
+        xi.skip_dt = mom_dtp;
         timer.begin(FAM_X);
         hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
         timer.end(FAM_X);
@@ -727,7 +730,7 @@ template <typename T> struct FVImpl : FVBase {
 
     int launch_pgrad(T **uu, T *psi, bool pupdate, double inv_adt)
     {
-        PGradArgs<T> a;
+        PGradArgs<T> a{};
         a.ux = uu[0];
         a.uy = uu[1];
         a.uz = uu[2];
@@ -738,6 +741,7 @@ template <typename T> struct FVImpl : FVBase {
         a.rdz = (T)(1.0 / dz);
         a.s = T(1);
         a.inv_adt = (T)inv_adt;
+        a.skip_dt = mom_dtp;
         timer.begin(FAM_PGRAD);
         if (pgrad_vec) {
             const dim3 blk(64, 4, 1), grd(ceil_div(g.nx / (16 / (int)sizeof(T)), 64), ceil_div(g.ny, 4), g.nz);
@@ -1001,6 +1005,42 @@ template <typename T> struct FVImpl : FVBase {
         return DNS_OK;
     }
 
+    // z planes per block of the LDS z-march momentum kernel
+    int momentum_zc() const
+    {
+        const int nbx = ceil_div(g.nx, 64), nby = ceil_div(g.ny, 8);
+        const long round = 256L * (sizeof(T) == 8 ? 3 : 6);
+        int zc = 32;
+        while (zc > 2 && (long)nbx * nby * ceil_div(g.nz, zc) < round) zc /= 2;
+        if (const char *ez = getenv("DNS_MOM_ZC")) zc = std::max(1, atoi(ez)); // sweeps
+        return zc;
+    }
+
+    // the kernel instances and launch shapes this handle's step uses
+    int describe(char *buf, int len) override
+    {
+        const char *mom = momentum_kind == 5 ? "lz" : momentum_kind == 3 ? "slab" : "direct";
+        char ys[64], cs[96];
+        if (!g.wall) snprintf(ys, sizeof ys, "fourier n=%d tile=%d/%d thr=%d", g.ny, tby, tbyf, nthry);
+        else if (thomas_tl && thomas_seg) snprintf(ys, sizeof ys, "seg R=%d tl=%d thr=%d", thomas_seg, thomas_tl, thomas_nthr);
+        else if (thomas_tl) snprintf(ys, sizeof ys, "tile tl=%d thr=%d", thomas_tl, thomas_nthr);
+        else snprintf(ys, sizeof ys, "sequential");
+        if (!imexy) snprintf(cs, sizeof cs, "none");
+        else if (cn_tl && cn_seg) snprintf(cs, sizeof cs, "seg R=%d tl=%d thr=%d nl=%d", cn_seg, cn_tl, cn_nthr, cn_nl);
+        else if (cn_tl) snprintf(cs, sizeof cs, "tile tl=%d thr=%d", cn_tl, cn_nthr);
+        else snprintf(cs, sizeof cs, "sequential");
+        const int n = snprintf(buf, (size_t)len,
+                               "prec=%s; momentum=%s zc=%d; xpass=half %d lpb=%d; zpass=n %d tile=%d/%d thr=%d; "
+                               "ysolve=%s; cn=%s; pgrad=%s",
+                               plain ? "plain" : sizeof(T) == 8 ? "fp64" : "mixed", mom, momentum_zc(), g.nx / 2, lpb,
+                               g.nz, tbz, tbzf, nthrz, ys, cs, pgrad_vec ? "vec" : "scalar");
+        if (n < 0 || n >= len) {
+            set_error("describe: buffer of %d bytes too small", len);
+            return DNS_EINVAL;
+        }
+        return DNS_OK;
+    }
+
     // momentum_kernel! as one launch writing q only (src/operators.jl:44-178)
     int momentum_plain(double Ac, double dt)
     {
@@ -1013,6 +1053,15 @@ template <typename T> struct FVImpl : FVBase {
 
     int step(double dt) override
     {
+        if (!std::isfinite(dt)) {
+            set_error("step!: Δt must be finite");
+            return DNS_EINVAL;
+        }
+        if (imexy && !(dt > 0)) {
+            // the lagged-pressure update p += δ/(αΔt) (src/timestep.jl:51) divides by Δt
+            set_error("step!: Δt must be positive with implicit y-diffusion");
+            return DNS_EINVAL;
+        }
         if (plain) return step_plain(dt);
         const bool replay = graphs && !timer.on && dt == last_dt;
         last_dt = dt;
@@ -1075,17 +1124,21 @@ template <typename T> struct FVImpl : FVBase {
         return DNS_OK;
     }
 
-    // nsteps CFL-adaptive steps (solve! with cfl, src/timestep.jl:92-111): Δt =
-    // min(cfl_timestep, tend − t) each step.  The explicit form computes Δt on
-    // the device from the pressure-gradient partials and replays one captured
-    // graph per buffer parity, so no step waits for the host; other forms (or
+    // Up to nsteps CFL-adaptive steps of solve! (src/timestep.jl:92-111): Δt =
+    // min(cfl_timestep, tend − t) each step, stopping once t >= tend −
+    // 1e-10(|tend| + 1) as the reference loop does.  The explicit form
+    // computes Δt on the device from the pressure-gradient partials and
+    // replays one captured graph per buffer parity, so no step waits for the
+    // host (steps after the end time leave u unchanged); other forms (or
     // per-launch timing) run the host loop.  t is advanced; dts (optional)
-    // receives every Δt.
-    int advance_cfl(double cflv, int nsteps, double tend, double *t_io, double *dts) override
+    // receives the Δt of every step taken, *ndone their count.
+    int advance_cfl(double cflv, int nsteps, double tend, double *t_io, double *dts, int *ndone) override
     {
+        *ndone = 0;
         if (nsteps <= 0) return DNS_OK;
         double t = *t_io;
         int done = 0;
+        const double tstop = std::isinf(tend) ? tend : tend - 1e-10 * (std::fabs(tend) + 1); // +inf: no end time
         auto host_step = [&]() -> int {
             double d = 0;
             DNS_TRY(cfl(cflv, &d));
@@ -1097,13 +1150,15 @@ template <typename T> struct FVImpl : FVBase {
             return DNS_OK;
         };
         if (plain || imexy || timer.on || !graphs) {
-            while (done < nsteps) DNS_TRY(host_step());
+            while (done < nsteps && t < tstop) DNS_TRY(host_step());
             *t_io = t;
+            *ndone = done;
             return DNS_OK;
         }
-        if (!umax_valid) DNS_TRY(host_step()); // the device Δt needs a pressure pass's partials
-        if (done == nsteps) {
+        if (!umax_valid && t < tstop) DNS_TRY(host_step()); // the device Δt needs a pressure pass's partials
+        if (done == nsteps || !(t < tstop)) {
             *t_io = t;
+            *ndone = done;
             return DNS_OK;
         }
         if (!ddt) {
@@ -1166,12 +1221,18 @@ template <typename T> struct FVImpl : FVBase {
             umax_valid = true;
         }
         last_dt = -1.0; // the fixed-Δt graphs stay valid, but the next step launches directly
-        if (dts)
-            DNS_HIP_CHECK(hipMemcpyAsync(dts + step0, dlog + step0, sizeof(double) * (nsteps - step0),
-                                         hipMemcpyDeviceToHost, st));
         DNS_HIP_CHECK(hipMemcpyAsync(pinned, dtime, sizeof(double), hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned + 1, dstep, sizeof(int), hipMemcpyDeviceToHost, st));
         DNS_HIP_CHECK(hipStreamSynchronize(st));
+        int taken = 0;
+        std::memcpy(&taken, pinned + 1, sizeof(int));
+        if (dts && taken > step0) {
+            DNS_HIP_CHECK(hipMemcpyAsync(dts + step0, dlog + step0, sizeof(double) * (taken - step0),
+                                         hipMemcpyDeviceToHost, st));
+            DNS_HIP_CHECK(hipStreamSynchronize(st));
+        }
         *t_io = pinned[0];
+        *ndone = taken;
         return DNS_OK;
     }
 
@@ -1250,16 +1311,26 @@ template <typename T> struct FVImpl : FVBase {
         }
         const int NC = g.nxc * g.ny * g.nz;
         const int nb = std::min(256, ceil_div(NC, NTHR));
-        size_t need = (size_t)nb * nbins;
-        if (need > redcap) {
-            hipStreamSynchronize(st);
-            hipFree(red);
-            redcap = need;
-            DNS_TRY(dalloc(&red, redcap));
+        const size_t need = (size_t)nb * nbins;
+        // grow-only buffers of their own: `red` is captured by the CFL graphs
+        if (need > spartcap || (size_t)nbins > soutcap) {
+            DNS_HIP_CHECK(hipStreamSynchronize(st));
+            if (need > spartcap) {
+                if (spart) DNS_HIP_CHECK(hipFree(spart));
+                spart = nullptr;
+                spartcap = 0;
+                DNS_TRY(dalloc(&spart, need));
+                spartcap = need;
+            }
+            if ((size_t)nbins > soutcap) {
+                if (sout) DNS_HIP_CHECK(hipFree(sout));
+                sout = nullptr;
+                soutcap = 0;
+                DNS_TRY(dalloc(&sout, (size_t)nbins));
+                soutcap = nbins;
+            }
         }
         std::vector<double> hostE(nbins);
-        double *dout = nullptr;
-        DNS_TRY(dalloc(&dout, nbins));
         const double N2 = (double)N * (double)N;
         for (int c = 0; c < 3; ++c) {
             XPassArgs<T> xa;
@@ -1282,15 +1353,15 @@ template <typename T> struct FVImpl : FVBase {
                                twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
                                st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, make_fastdiv(tby), g, fdy);
-            hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR), sizeof(double) * nbins, st, A, g, nbins,
-                               1.0 / N2, red);
-            hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nbins, NTHR)), dim3(NTHR), 0, st, red, nb, nbins, dout);
+            hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR),
+                               sizeof(double) * nbins + (sizeof(double) + sizeof(int)) * NTHR, st, A, g, nbins,
+                               1.0 / N2, spart);
+            hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nbins, NTHR)), dim3(NTHR), 0, st, spart, nb, nbins, sout);
             DNS_HIP_CHECK(hipGetLastError());
-            DNS_HIP_CHECK(hipMemcpyAsync(hostE.data(), dout, sizeof(double) * nbins, hipMemcpyDeviceToHost, st));
+            DNS_HIP_CHECK(hipMemcpyAsync(hostE.data(), sout, sizeof(double) * nbins, hipMemcpyDeviceToHost, st));
             DNS_HIP_CHECK(hipStreamSynchronize(st));
             for (int b = 0; b < nbins; ++b) E[b] += hostE[b];
         }
-        hipFree(dout);
         return DNS_OK;
     }
 
@@ -1411,11 +1482,14 @@ int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt)
     if (!dt) return DNS_EINVAL;
     return h->impl->cfl(cfl, dt);
 }
-int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts)
+int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts, int32_t *ndone)
 {
     FV_CHECK(h);
     if (!t || nsteps < 0) return DNS_EINVAL;
-    return h->impl->advance_cfl(cfl, nsteps, tend, t, dts);
+    int taken = 0;
+    const int rc = h->impl->advance_cfl(cfl, nsteps, tend, t, dts, &taken);
+    if (ndone) *ndone = taken;
+    return rc;
 }
 int dns_fv_maxdiv(dns_fv *h, double *out)
 {
@@ -1445,6 +1519,12 @@ int dns_fv_sync(dns_fv *h)
 {
     FV_CHECK(h);
     return h->impl->sync();
+}
+int dns_fv_describe(dns_fv *h, char *buf, int32_t len)
+{
+    FV_CHECK(h);
+    if (!buf || len <= 0) return DNS_EINVAL;
+    return h->impl->describe(buf, len);
 }
 int dns_fv_timing(dns_fv *h, int32_t enable)
 {

@@ -572,11 +572,15 @@ template <typename T> struct XInvArgs {
     const cplx<T> *twx; // length nx/2+1
     int lpb;
     FastDiv fN; // division by nx/2
+    // device Δt of the CFL loop (null otherwise): out is left as it is when
+    // Δt is 0 (after the loop reached its end time)
+    const double *skip_dt;
 };
 
 template <typename T, int EPT>
 __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDesc d)
 {
+    if (a.skip_dt && *a.skip_dt == 0.0) return; // uniform over the grid
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int N = d.n, lpb = a.lpb;
@@ -1040,16 +1044,25 @@ __global__ void __launch_bounds__(256, 2) k_thomas_seg(cplx<T> *__restrict__ A, 
 // Δt of a CFL-adaptive step on the device (cfl_timestep, src/timestep.jl:69-80,
 // the same expression as the host's): from the per-component velocity maxima,
 // clamped to the time left before tend; advances the device time and logs Δt.
-// One thread.
+// Once t >= tend − 1e-10(|tend| + 1) (the end test of solve!,
+// src/timestep.jl:102) it writes Δt = 0 and neither advances t nor counts a
+// step: the step that follows leaves u unchanged (q = 0, u′ = u, and the
+// pressure-gradient pass skips on Δt = 0).  One thread.
 __global__ void k_cfl_dt(const double *umax, double dx, double dymin, double dz, double diff, double cfl,
                          double tend, double *t, double *dt, double *dt_log, int *step)
 {
     const double eps = 2.220446049250313e-16;
+    const double tnow = *t;
+    const double tol = isinf(tend) ? 0.0 : 1e-10 * (fabs(tend) + 1); // tend = +inf: no end time
+    if (!(tnow < tend - tol)) {
+        *dt = 0.0;
+        return;
+    }
     const double conv = fmin(fmin(dx / (umax[0] + eps), dymin / (umax[1] + eps)), dz / (umax[2] + eps));
     double d = fmin(cfl * conv, cfl * diff);
-    d = fmin(d, tend - *t);
+    d = fmin(d, tend - tnow);
     *dt = d;
-    *t += d;
+    *t = tnow + d;
     dt_log[*step] = d;
     *step += 1;
 }
@@ -1064,11 +1077,15 @@ template <typename T> struct PGradArgs {
     T *p;
     const T *rdyu;
     T rdx, rdz, s, inv_adt;
+    // device Δt of the CFL loop (null otherwise): the pass does nothing when
+    // it is 0, i.e. after the loop reached its end time
+    const double *skip_dt;
 };
 
 template <typename T, bool PUPDATE, bool MAXU>
 __global__ void __launch_bounds__(256) k_pgrad(PGradArgs<T> a, FVGeom g, double *umax_part)
 {
+    if (a.skip_dt && *a.skip_dt == 0.0) return; // uniform over the grid
     int bx, by, bz;
     xcd_block3(bx, by, bz);
     const int i = bx * 64 + threadIdx.x;
@@ -1124,6 +1141,7 @@ __global__ void __launch_bounds__(256) k_pgrad(PGradArgs<T> a, FVGeom g, double 
 template <typename T, bool PUPDATE, bool MAXU>
 __global__ void __launch_bounds__(256) k_pgrad_vec(PGradArgs<T> a, FVGeom g, double *umax_part)
 {
+    if (a.skip_dt && *a.skip_dt == 0.0) return; // uniform over the grid
     constexpr int V = 16 / sizeof(T);
     using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
     int bx, by, bz;
@@ -1741,28 +1759,49 @@ __global__ void __launch_bounds__(256) k_chstats(const T *ux, const T *uy, const
 
 // Shell binning of one transformed component (src/stats.jl:143-157):
 // κ = round(|k|), weight 2 for 0 < kx < nx/2, adds w|Â|²/2/N² to per-block
-// LDS bins; part[block * nbins + κ].
+// bins; part[block * nbins + κ].  Deterministic: the block stages 256
+// (κ, e) pairs in LDS, then the thread that owns bin b adds that chunk's
+// contributions to b in index order, so every run sums in the same order
+// (the grid depends only on the sizes).  Dynamic LDS: nbins doubles +
+// 256 doubles + 256 ints.
 template <typename T>
 __global__ void __launch_bounds__(256) k_binshells(const cplx<T> *A, FVGeom g, int nbins, double invN2, double *part)
 {
     extern __shared__ unsigned char smem_raw[];
     double *bins = reinterpret_cast<double *>(smem_raw);
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) bins[b] = 0.0;
-    __syncthreads();
+    double *val = bins + nbins;
+    int *key = reinterpret_cast<int *>(val + 256);
+    const int tid = threadIdx.x;
+    for (int b = tid; b < nbins; b += 256) bins[b] = 0.0;
     const int N = g.nxc * g.ny * g.nz;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < N; t += gridDim.x * blockDim.x) {
-        int kx = t % g.nxc, r = t / g.nxc, j = r % g.ny, k = r / g.ny;
-        int ky = j > g.ny / 2 ? j - g.ny : j;
-        const size_t at = kx + (size_t)g.ldc * r;
-        int kz = k > g.nz / 2 ? k - g.nz : k;
-        int kap = (int)rint(sqrt((double)(kx * kx + ky * ky + kz * kz)));
-        double wgt = (kx > 0 && 2 * kx < g.nx) ? 2.0 : 1.0;
-        cplx<T> a = A[at];
-        double e = wgt * ((double)a.x * (double)a.x + (double)a.y * (double)a.y) / 2 * invN2;
-        if (kap < nbins) atomicAdd(&bins[kap], e);
+    for (int base = blockIdx.x * 256; base < N; base += gridDim.x * 256) {
+        const int t = base + tid;
+        int kap = -1;
+        double e = 0.0;
+        if (t < N) {
+            const int kx = t % g.nxc, r = t / g.nxc, j = r % g.ny, k = r / g.ny;
+            const int ky = j > g.ny / 2 ? j - g.ny : j;
+            const int kz = k > g.nz / 2 ? k - g.nz : k;
+            kap = (int)rint(sqrt((double)(kx * kx + ky * ky + kz * kz)));
+            const double wgt = (kx > 0 && 2 * kx < g.nx) ? 2.0 : 1.0;
+            const cplx<T> a = A[kx + (size_t)g.ldc * r];
+            e = wgt * ((double)a.x * (double)a.x + (double)a.y * (double)a.y) / 2 * invN2;
+            if (kap >= nbins) kap = -1;
+        }
+        __syncthreads(); // the previous chunk's readers are done with key/val
+        key[tid] = kap;
+        val[tid] = e;
+        __syncthreads();
+        const int cnt = min(256, N - base);
+        for (int b = tid; b < nbins; b += 256) {
+            double acc = bins[b];
+            for (int q = 0; q < cnt; ++q)
+                if (key[q] == b) acc += val[q];
+            bins[b] = acc;
+        }
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) part[(size_t)blockIdx.x * nbins + b] = bins[b];
+    for (int b = tid; b < nbins; b += 256) part[(size_t)blockIdx.x * nbins + b] = bins[b];
 }
 
 }  // namespace dns

@@ -10,6 +10,7 @@ and processors receive a device-backed state that downloads lazily.
 """
 from __future__ import annotations
 
+import ctypes as ct
 import math
 from types import SimpleNamespace
 
@@ -98,6 +99,13 @@ class Setup:
         if h is not None and _lib._LIB is not None:
             _lib._LIB.dns_fv_destroy(h)
             self.handle = None
+
+    def describe(self):
+        """Kernel instances and launch shapes of this handle's step
+        (``dns_fv_describe``) as a dict of ``key -> text``."""
+        buf = ct.create_string_buffer(512)
+        check(lib().dns_fv_describe(self.handle, buf, 512))
+        return dict(part.strip().split("=", 1) for part in buf.value.decode().split(";"))
 
     # host <-> device ------------------------------------------------------
     def upload(self, name, a):
@@ -257,13 +265,16 @@ def cfl_timestep(u, s, cfl):
 
 
 def advance_cfl_(s, cfl, nsteps, tend=math.inf, t=0.0):
-    """``nsteps`` CFL steps of ``solve!`` (src/timestep.jl:92-111) on the
-    device-resident state: Δt = min(cfl_timestep, tend − t) each step,
-    computed on the device in the explicit form.  Returns (t, Δt array)."""
+    """At most ``nsteps`` CFL steps of ``solve!`` (src/timestep.jl:92-111) on
+    the device-resident state: while t < tend − 1e-10(|tend| + 1),
+    Δt = min(cfl_timestep, tend − t), computed on the device in the explicit
+    form.  Returns (t, Δt of each step taken)."""
     tt = np.array([float(t)])
     dts = np.zeros(max(1, int(nsteps)))
-    check(lib().dns_fv_advance_cfl(s.handle, float(cfl), int(nsteps), float(tend), dptr(tt), dptr(dts)))
-    return float(tt[0]), dts[:int(nsteps)]
+    nd = ct.c_int32(0)
+    check(lib().dns_fv_advance_cfl(s.handle, float(cfl), int(nsteps), float(tend), dptr(tt), dptr(dts),
+                                   ct.byref(nd)))
+    return float(tt[0]), dts[:nd.value].copy()
 
 
 def maxdiv(u, s):

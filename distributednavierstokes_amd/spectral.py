@@ -286,7 +286,7 @@ def spectral_spectrum(uh, s):
 
 
 class SpecState:
-    """``state = (; uh, t, n)`` passed to processors (src/spectral/step.jl:263)."""
+    """``state = (; uh, t, n)`` passed to processors (src/spectral/step.jl:151,166)."""
 
     def __init__(self, s, t, n):
         self.setup, self.t, self.n = s, t, n

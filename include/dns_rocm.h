@@ -80,8 +80,8 @@ typedef struct dns_fv_params {
     int32_t precision;     /* DNS_FP64, DNS_MIXED or DNS_PLAIN_FP64 */
 } dns_fv_params;
 
-/* setup(; ...) -- src/setup.jl:34-113 (with plan_poisson, src/poisson.jl:167-231,
- * and plan_ydiffusion, src/ydiffusion.jl:187-201).  Allocates all device
+/* setup(; ...) -- src/setup.jl:34-113 (with plan_poisson, src/poisson.jl:56-120,
+ * and plan_ydiffusion, src/ydiffusion.jl:54-68).  Allocates all device
  * state (u, p, q, δ and the pipeline buffers) once; fields start at zero. */
 int dns_fv_create(const dns_fv_params *params, dns_fv **out);
 int dns_fv_destroy(dns_fv *h);
@@ -99,18 +99,24 @@ int dns_fv_momentum(dns_fv *h, double A, double dt);
 int dns_fv_project(dns_fv *h, int32_t psi_field);
 
 /* step!(u, p, q, δ, s, Δt) -- src/timestep.jl:41-63: one Williamson RK3 step
- * (explicit or semi-implicit per the handle).  Asynchronous. */
+ * (explicit or semi-implicit per the handle).  Asynchronous.  Δt must be
+ * finite, and positive with implicit y-diffusion (the lagged-pressure update
+ * p += δ/(αΔt), src/timestep.jl:51, divides by it): DNS_EINVAL otherwise. */
 int dns_fv_step(dns_fv *h, double dt);
 
 /* cfl_timestep(u, s; cfl) -- src/timestep.jl:69-80. */
 int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt);
 
-/* The CFL branch of solve! (src/timestep.jl:92-111) for nsteps steps:
- * Δt = min(cfl_timestep(u, s; cfl), tend − t) then step!, with *t advanced
- * and every Δt stored in dts (may be null).  In the explicit form Δt is
- * computed on the device and the steps replay captured HIP graphs without
- * waiting for the host; returns after the last step. */
-int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts);
+/* The CFL branch of solve! (src/timestep.jl:92-111) for at most nsteps
+ * steps: while t < tend − 1e-10(|tend| + 1) (src/timestep.jl:102),
+ * Δt = min(cfl_timestep(u, s; cfl), tend − t) then step!, with *t advanced,
+ * the Δt of every step taken stored in dts (nsteps entries, may be null) and
+ * the number of steps taken in *ndone (may be null); tend = +inf sets no
+ * end time (nsteps steps are taken).  In the explicit form
+ * Δt is computed on the device and the steps replay captured HIP graphs
+ * without waiting for the host (graphs replayed after the end time leave
+ * the state unchanged); returns after the last step. */
+int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts, int32_t *ndone);
 
 /* maxdiv(u, s) -- src/fields.jl:38-42 (divergence residual in FP64). */
 int dns_fv_maxdiv(dns_fv *h, double *out);
@@ -130,6 +136,12 @@ int dns_fv_spectrum_sample(dns_fv *h, double *E, int32_t nbins);
 
 /* Waits for all work queued on the handle's stream. */
 int dns_fv_sync(dns_fv *h);
+
+/* No counterpart in the reference (diagnostics): writes a one-line
+ * description of the kernel instances and launch shapes the handle's step
+ * uses (momentum march, FFT line lengths and tiles, y-solve and CN kernels
+ * with their segment lengths) into buf (len bytes, NUL-terminated). */
+int dns_fv_describe(dns_fv *h, char *buf, int32_t len);
 
 /* Kernel timing hook used by the benchmark: when enabled, the library
  * records HIP events around every launch of the named kernel family on the

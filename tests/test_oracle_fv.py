@@ -260,3 +260,81 @@ def test_poisson_inverts_dense_div_grad(bcy):
     o.poisson_from(r.reshape(n, order="F"), p)
     pv = p[1:-1, 1:-1, 1:-1].ravel(order="F")
     assert np.abs(Lmat @ pv - r).max() < 1e-10 * np.abs(r).max()
+
+
+def _mode1_phase(line):
+    """Phase of the k = 1 Fourier coefficient of a periodic sample line."""
+    n = len(line)
+    return np.angle(np.sum(line * np.exp(-2j * np.pi * np.arange(n) / n)))
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_convection_carries_a_shear_wave_downstream(direction):
+    """A shear wave in a uniform flow U along `direction` (periodic, ν = 0):
+    the wave component c = (direction + 2) % 3 depends only on that
+    coordinate, so the only nonzero term is its convection, which the
+    staggered skew-symmetric scheme (src/operators.jl:44-145) discretises as
+    the central flux U (w_{i+1} − w_{i−1}) / 2Δ.  The k = 1 mode therefore
+    moves downstream with U_eff = U sin(Δ)/Δ: its phase at time t is
+    −U_eff t.  Pins the sign and the size of each component's convection
+    term (a flipped sign advects upstream; a one-sided or doubled flux
+    changes U_eff)."""
+    n, U, t_end, dt = 16, 1.0, 0.5, 0.01
+    prob = FVProblem(n=(n, n, n), visc=0.0)
+    o = OracleFV(prob)
+    u = vzeros(prob)
+    wave = (direction + 2) % 3
+    carrier = [None, None, None]
+    carrier[direction] = lambda x, y, z: U
+    carrier[wave] = lambda x, y, z, d=direction: 0.5 * math.sin((x, y, z)[d])
+    fill_velocity(prob, u, *carrier)
+    o.exchange_halo_u(u)
+
+    def line(a):
+        a = a[1:-1, 1:-1, 1:-1]
+        return a.mean(axis=tuple(ax for ax in range(3) if ax != direction))
+
+    ph0 = _mode1_phase(line(u[wave]))
+    o.solve(u, (0.0, t_end), dt=dt)
+    shift = ph0 - _mode1_phase(line(u[wave]))
+    delta = 2 * math.pi / n
+    expected = U * math.sin(delta) / delta * t_end
+    assert abs(shift - expected) < 1e-4 * expected, (shift, expected)
+    assert np.abs(u[direction][1:-1, 1:-1, 1:-1] - U).max() < 1e-13  # the carrier is untouched
+
+
+def _hand_dymin(prob):
+    ny = prob.n[1]
+    y0, y1 = prob.lims[1]
+    yf = [y0 + (y1 - y0) * prob.stretch(j / ny) for j in range(ny + 1)]
+    return min(yf[j + 1] - yf[j] for j in range(ny))
+
+
+@pytest.mark.parametrize("ydiffusion", ["explicit", "implicit"])
+def test_cfl_timestep_closed_forms(ydiffusion):
+    """cfl_timestep (src/timestep.jl:69-80) from hand-computed metrics:
+    a uniform flow along each axis hits the convective bound cfl·Δ/U (Δymin
+    for y, the smallest stretched cell); a fluid at rest hits the diffusive
+    bound cfl/(2ν(1/Δx² + 1/Δymin² + 1/Δz²)), without the 1/Δymin² term
+    when y-diffusion is implicit."""
+    cfl, eps = 0.45, 2.220446049250313e-16
+    lims = ((0.0, 4 * math.pi), (-1.0, 1.0), (0.0, 4 * math.pi / 3))
+    stretch = tanhstretch(1.5)
+    prob_c = FVProblem(n=(16, 12, 8), lims=lims, bc=("periodic", "wall", "periodic"), stretch=stretch, visc=1e-12,
+                       ydiffusion=ydiffusion)
+    dx, dz, dymin = 4 * math.pi / 16, 4 * math.pi / 3 / 8, _hand_dymin(prob_c)
+    o = OracleFV(prob_c)
+    for c, (d, U) in enumerate(((dx, 3.0), (dymin, 2.5), (dz, 1.5))):
+        u = vzeros(prob_c)
+        f = [None, None, None]
+        f[c] = lambda x, y, z, U=U: U
+        fill_velocity(prob_c, u, *f)
+        o.exchange_halo_u(u)
+        assert math.isclose(o.cfl_timestep(u, cfl), cfl * d / (U + eps), rel_tol=1e-14), c
+    nu = 0.01
+    prob_d = FVProblem(n=(16, 12, 8), lims=lims, bc=("periodic", "wall", "periodic"), stretch=stretch, visc=nu,
+                       ydiffusion=ydiffusion)
+    o = OracleFV(prob_d)
+    invdy2 = 0.0 if ydiffusion == "implicit" else 1 / dymin ** 2
+    want = cfl / (2 * nu * (1 / dx ** 2 + invdy2 + 1 / dz ** 2) + eps)
+    assert math.isclose(o.cfl_timestep(vzeros(prob_d), cfl), want, rel_tol=1e-14)

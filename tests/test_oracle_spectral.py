@@ -130,3 +130,80 @@ def test_dissipation_closed_form():
     e = o.energy(uh)
     assert math.isclose(e, 0.125, rel_tol=1e-12)
     assert math.isclose(o.dissipation(uh), 0.01 * 3 * 2 * e, rel_tol=1e-12)
+
+
+def test_noise_known_answers():
+    """noise_kernel! (src/spectral/ic.jl:38-48) at fixed (n, seed, i, j, k, c)
+    against values computed from the published recipe with Python integers
+    (tests/golden/make_golden.py): pins the key order ((c·n + k)·n + j)·n + i,
+    the 2·key / 2·key + 1 counters, the 53-bit uniforms and Box–Muller."""
+    kat = json.load(open(os.path.join(GOLDEN, "noise_kat.json")))
+    fields = {}
+    for n, seed, i, j, k, c, want in kat["points"]:
+        seed, want = int(seed), float(want)
+        if (n, seed) not in fields:
+            fields[(n, seed)] = OracleSpectral(SpectralProblem(n=n)).noise(seed)
+        got = fields[(n, seed)][i, j, k, c]
+        assert abs(got - want) <= 2e-15 * max(1.0, abs(want)), (n, seed, i, j, k, c, got, want)
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_convection_carries_a_shear_wave_downstream(direction):
+    """A shear wave in a uniform mean flow U along `direction` (ν = 0): the
+    only nonzero term of du_i = −ik_j F[v_i v_j] (src/spectral/operators.jl:60-79)
+    is −ik U û for the wave component, so its k = 1 mode turns by exactly
+    −U t (Wray RK3 adds O((UΔt)⁵) per step).  Pins the sign and size of the
+    convection term along each axis (the mean mode is untouched by the
+    projection, src/spectral/operators.jl:9)."""
+    U, t_end, dt = 0.8, 0.6, 0.01
+    o = OracleSpectral(SpectralProblem(n=12, visc=0.0))
+    wave = (direction + 2) % 3
+    fs = [lambda x, y, z: 0 * x] * 3
+    fs[direction] = lambda x, y, z: U + 0 * x
+    fs[wave] = lambda x, y, z, d=direction: 0.5 * np.sin((x, y, z)[d])
+    uh = o.velocityfield(*fs)
+    idx = [0, 0, 0]
+    idx[direction] = 1
+    c0 = uh[idx[0], idx[1], idx[2], wave]
+    o.solve(uh, (0.0, t_end), dt=dt)
+    c1 = uh[idx[0], idx[1], idx[2], wave]
+    shift = -np.angle(c1 / c0)
+    assert abs(shift - U * t_end) < 1e-6 * U * t_end, (shift, U * t_end)
+    assert abs(abs(c1 / c0) - 1) < 1e-6
+    assert abs(uh[0, 0, 0, direction] - U) < 1e-14
+
+
+def test_rhs_equals_advective_form():
+    """The divergence form du_i = −ik_j F[v_i v_j] (src/spectral/operators.jl:60-79)
+    equals the advective form −F[(v·∇)v_i] for a divergence-free band-limited
+    v, computed here independently with numpy (spectral derivatives of v,
+    products in physical space).  With n = 12, kcut = 3 no product mode
+    aliases onto a retained one (n − 2kcut > kcut), so the two agree to
+    roundoff.  Pins every term of the nonlinearity, diagonal (v_i v_i) and
+    off-diagonal, with its sign."""
+    N, kcut = 12, 3
+    o = OracleSpectral(SpectralProblem(n=N, kcut=kcut, visc=0.0))
+    uh = o.randomfield(totalenergy=1.0, kpeak=2, seed=3)
+    du, _ = o.rhs(uh)
+    m = o.m
+    fine = lambda j: j if j <= kcut else j + (N - m)  # noqa: E731
+    Uf = np.zeros((N // 2 + 1, N, N, 3), dtype=complex)
+    for k in range(m):
+        for j in range(m):
+            Uf[:kcut + 1, fine(j), fine(k), :] = uh[:, j, k, :] * N ** 3
+    kx = np.arange(N // 2 + 1)[:, None, None]
+    ky = np.fft.fftfreq(N, 1.0 / N)[None, :, None]
+    kz = np.fft.fftfreq(N, 1.0 / N)[None, None, :]
+    kvec = (kx, ky, kz)
+
+    def irfft3(a):
+        return np.transpose(np.fft.irfftn(np.transpose(a), s=(N, N, N), axes=(0, 1, 2)))
+
+    v = [irfft3(Uf[..., c]) for c in range(3)]
+    adv = np.stack([sum(v[d] * irfft3(1j * kvec[d] * Uf[..., c]) for d in range(3)) for c in range(3)], -1)
+    A = rfft3(adv) / N ** 3
+    want = np.zeros_like(du)
+    for k in range(m):
+        for j in range(m):
+            want[:, j, k, :] = -A[:kcut + 1, fine(j), fine(k), :]
+    assert np.abs(du - want).max() < 1e-13 * np.abs(want).max()
