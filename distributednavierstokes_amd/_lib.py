@@ -58,6 +58,8 @@ _SIGS = {
     "dns_fv_project": (ct.c_int, [ct.c_void_p, ct.c_int32]),
     "dns_fv_step": (ct.c_int, [ct.c_void_p, ct.c_double]),
     "dns_fv_cfl_timestep": (ct.c_int, [ct.c_void_p, ct.c_double, _dp]),
+    "dns_fv_advance_cfl_noalloc": (ct.c_int, [ct.c_void_p, ct.c_double, ct.c_int32, ct.c_double, _dp, _dp,
+                                              ct.POINTER(ct.c_int32)]),
     "dns_fv_describe": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_int32]),
     "dns_fv_advance_cfl": (ct.c_int, [ct.c_void_p, ct.c_double, ct.c_int32, ct.c_double, _dp, _dp,
                                       ct.POINTER(ct.c_int32)]),

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "fv_kernels.hpp"
@@ -47,6 +48,7 @@ struct FVBase {
     virtual int step(double dt) = 0;
     virtual int cfl(double cfl, double *dt) = 0;
     virtual int advance_cfl(double cfl, int nsteps, double tend, double *t, double *dts, int *ndone) = 0;
+    virtual int advance_cfl_noalloc(double cfl, int nsteps, double tend, double *t, double *dts, int *ndone) = 0;
     virtual int maxdiv(double *out) = 0;
     virtual int kinetic(double *out) = 0;
     virtual int chstats(double *acc) = 0;
@@ -65,6 +67,45 @@ constexpr int THOMAS_RMAX = 24, CN_RMAX = 32; // rows per thread segment of the 
 // (a half-size variant of each kernel serves segments of at most RMAX / 2 rows)
 constexpr int CH_NCHUNK = 16;
 constexpr int RED_BLOCKS = 1024;
+
+// Scratch of the allocation-free CFL loop (dns_fv_advance_cfl_noalloc), one
+// file-scope array per GPU (160 KiB, held until the process ends).  Layout in
+// doubles: [0] Δt, [1] t, [2] step counter (an int in its first 4 bytes),
+// [NOALLOC_PART, +3·512) per-block maxima of the CFL reduction,
+// [NOALLOC_RED, +3) reduced maxima, [NOALLOC_LOG, +NOALLOC_MAX_STEPS) the
+// Δt log.  Host code never names the array in a launch or copy: it uses the
+// address hipGetSymbolAddress gave for the current device (noalloc_scratch).
+constexpr int NOALLOC_MAX_STEPS = 16384;
+constexpr size_t NOALLOC_PART = 256, NOALLOC_RED = 2048, NOALLOC_LOG = 4096;
+constexpr size_t NOALLOC_DOUBLES = NOALLOC_LOG + NOALLOC_MAX_STEPS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+// Device address of `scratch` on device dev (the current device), resolved
+// once per device and kept.
+static int noalloc_scratch(int dev, double **out)
+{
+    static std::mutex mu;
+    static double *addr[64] = {};
+    if (dev < 0 || dev >= 64) {
+        set_error("device ordinal %d out of range", dev);
+        return DNS_EUNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        DNS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev] = static_cast<double *>(p);
+    }
+    *out = addr[dev];
+    return DNS_OK;
+}
+
+// Start of an allocation-free CFL loop: t and a zero step counter.  One thread.
+__global__ void k_noalloc_start(double *s, double t)
+{
+    s[1] = t;
+    *reinterpret_cast<int *>(s + 2) = 0;
+}
 
 template <typename T> struct FVImpl : FVBase {
     // elements per thread of the z / periodic-y FFT tiles: FP32 transforms a
@@ -131,6 +172,7 @@ template <typename T> struct FVImpl : FVBase {
     double *ddt = nullptr, *dtime = nullptr, *dlog = nullptr;
     int *dstep = nullptr;
     int nlog = 0;
+    double *noalloc_s = nullptr; // this device's address of `scratch`
 
     ~FVImpl() override
     {
@@ -213,6 +255,11 @@ template <typename T> struct FVImpl : FVBase {
 
         DNS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         timer.st = st;
+        {
+            int dev = 0;
+            DNS_HIP_CHECK(hipGetDevice(&dev));
+            DNS_TRY(noalloc_scratch(dev, &noalloc_s));
+        }
         for (int c = 0; c < 3; ++c) {
             DNS_TRY(dalloc(&u[c], N));
             DNS_TRY(dalloc(&q[c], N));
@@ -1236,6 +1283,91 @@ template <typename T> struct FVImpl : FVBase {
         return DNS_OK;
     }
 
+    // advance_cfl without any allocation: the device Δt, time, step counter,
+    // Δt log and CFL partials live in the file-scope scratch, the steps are
+    // direct launches (no graph is captured or instantiated) and the host
+    // copies go to the pinned buffer of the handle or to the caller's arrays.
+    // Same kernels and arithmetic as advance_cfl, hence the same results.
+    int advance_cfl_noalloc(double cflv, int nsteps, double tend, double *t_io, double *dts, int *ndone) override
+    {
+        *ndone = 0;
+        if (nsteps < 0 || nsteps > NOALLOC_MAX_STEPS) {
+            set_error("advance_cfl_noalloc: nsteps must be in 0..%d (the size of its Δt log)", NOALLOC_MAX_STEPS);
+            return DNS_EINVAL;
+        }
+        if (plain) {
+            set_error("advance_cfl_noalloc: the plain form allocates its reference-structured arrays on first use");
+            return DNS_EUNSUPPORTED;
+        }
+        if (timer.on) {
+            set_error("advance_cfl_noalloc: per-launch timing creates events; disable it first");
+            return DNS_EUNSUPPORTED;
+        }
+        if (nsteps == 0) return DNS_OK;
+        double t = *t_io;
+        int done = 0;
+        const double tstop = std::isinf(tend) ? tend : tend - 1e-10 * (std::fabs(tend) + 1);
+        auto host_step = [&]() -> int {
+            double d = 0;
+            DNS_TRY(cfl(cflv, &d));
+            d = std::min(d, tend - t);
+            if (imexy && !(d > 0)) {
+                set_error("step!: Δt must be positive with implicit y-diffusion");
+                return DNS_EINVAL;
+            }
+            DNS_TRY(step_launch(d));
+            last_dt = -1.0;
+            t += d;
+            if (dts) dts[done] = d;
+            ++done;
+            return DNS_OK;
+        };
+        if (imexy) {
+            while (done < nsteps && t < tstop) DNS_TRY(host_step());
+            *t_io = t;
+            *ndone = done;
+            return DNS_OK;
+        }
+        if (!umax_valid && t < tstop) DNS_TRY(host_step()); // the device Δt needs a pressure pass's partials
+        if (done == nsteps || !(t < tstop)) {
+            *t_io = t;
+            *ndone = done;
+            return DNS_OK;
+        }
+        double *sdt = noalloc_s, *stime = noalloc_s + 1, *spart = noalloc_s + NOALLOC_PART,
+               *sred = noalloc_s + NOALLOC_RED, *slog = noalloc_s + NOALLOC_LOG;
+        int *sstep = reinterpret_cast<int *>(noalloc_s + 2);
+        const int step0 = done;
+        hipLaunchKernelGGL(k_noalloc_start, dim3(1), dim3(1), 0, st, noalloc_s, t);
+        const double eps = 2.220446049250313e-16;
+        const double diff = 1 / (2 * visc * (1 / (dx * dx) + 1 / (dymin * dymin) + 1 / (dz * dz)) + eps);
+        const int G = std::min(512, std::max(1, nb_pgrad / 1024));
+        for (; done < nsteps; ++done) {
+            hipLaunchKernelGGL(k_max_mid, dim3(G), dim3(NTHR), 0, st, upart, nb_pgrad, 3, spart);
+            hipLaunchKernelGGL(k_max_final, dim3(1), dim3(NTHR), 0, st, spart, G, 3, sred);
+            hipLaunchKernelGGL(k_cfl_dt, dim3(1), dim3(1), 0, st, sred, dx, dymin, dz, diff, cflv, tend, stime, sdt,
+                               slog, sstep);
+            mom_dtp = sdt;
+            const int rc = step_launch(0.0);
+            mom_dtp = nullptr;
+            DNS_TRY(rc);
+            umax_valid = true;
+        }
+        last_dt = -1.0;
+        DNS_HIP_CHECK(hipGetLastError());
+        DNS_HIP_CHECK(hipMemcpyAsync(pinned, stime, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        DNS_HIP_CHECK(hipStreamSynchronize(st));
+        int taken = 0;
+        std::memcpy(&taken, pinned + 1, sizeof(int));
+        if (dts && taken > 0) {
+            DNS_HIP_CHECK(hipMemcpyAsync(dts + step0, slog, sizeof(double) * taken, hipMemcpyDeviceToHost, st));
+            DNS_HIP_CHECK(hipStreamSynchronize(st));
+        }
+        *t_io = pinned[0];
+        *ndone = step0 + taken;
+        return DNS_OK;
+    }
+
     int cfl(double cflv, double *dt) override
     {
         if (umax_valid) {
@@ -1488,6 +1620,19 @@ int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, doubl
     if (!t || nsteps < 0) return DNS_EINVAL;
     int taken = 0;
     const int rc = h->impl->advance_cfl(cfl, nsteps, tend, t, dts, &taken);
+    if (ndone) *ndone = taken;
+    return rc;
+}
+int dns_fv_advance_cfl_noalloc(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts,
+                               int32_t *ndone)
+{
+    FV_CHECK(h);
+    if (!t) {
+        set_error("null t");
+        return DNS_EINVAL;
+    }
+    int taken = 0;
+    const int rc = h->impl->advance_cfl_noalloc(cfl, nsteps, tend, t, dts, &taken);
     if (ndone) *ndone = taken;
     return rc;
 }

@@ -118,6 +118,19 @@ int dns_fv_cfl_timestep(dns_fv *h, double cfl, double *dt);
  * the state unchanged); returns after the last step. */
 int dns_fv_advance_cfl(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts, int32_t *ndone);
 
+/* No counterpart in the reference: dns_fv_advance_cfl for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * results; it allocates nothing: the device Δt, time, step counter, Δt log
+ * and CFL reduction partials sit in a static 160 KiB device array of the
+ * library (one per GPU, kept until the process ends), the steps are direct
+ * launches (no graph), and nsteps is limited to 16384 (DNS_EINVAL beyond).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (several handles on concurrent host threads) race.  Not available for
+ * DNS_PLAIN_FP64 handles or with kernel timing on (DNS_EUNSUPPORTED). */
+int dns_fv_advance_cfl_noalloc(dns_fv *h, double cfl, int32_t nsteps, double tend, double *t, double *dts,
+                               int32_t *ndone);
+
 /* maxdiv(u, s) -- src/fields.jl:38-42 (divergence residual in FP64). */
 int dns_fv_maxdiv(dns_fv *h, double *out);
 
