@@ -15,3 +15,6 @@ from .spectral import (  # noqa: F401
     phys_to_spec_, spectral_project_, spectral_rhs_, spectral_step_, spectral_solve_, shell_energies, shellforcing,
     spectral_energy, spectral_dissipation, spectral_maxdiv, spectral_stats, spectral_spectrum, wray3,
 )
+from .spectral_io import (  # noqa: F401
+    spectral_save, spectral_load_, checkpointlist, spectral_latest, checkpointer, snapshotsaver, statswriter,
+)

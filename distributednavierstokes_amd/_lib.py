@@ -88,6 +88,8 @@ _SIGS = {
     "dns_spec_energy": (ct.c_int, [ct.c_void_p, _dp]),
     "dns_spec_dissipation": (ct.c_int, [ct.c_void_p, _dp]),
     "dns_spec_maxdiv": (ct.c_int, [ct.c_void_p, _dp]),
+    "dns_spec_save": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_double, ct.c_int64, ct.c_char_p]),
+    "dns_spec_load": (ct.c_int, [ct.c_void_p, ct.c_char_p, _dp, ct.POINTER(ct.c_int64)]),
     "dns_spec_sync": (ct.c_int, [ct.c_void_p]),
     "dns_spec_products": (ct.c_int, [ct.c_void_p]),
     "dns_spec_get_products": (ct.c_int, [ct.c_void_p, _dp]),

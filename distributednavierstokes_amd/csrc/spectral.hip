@@ -3,8 +3,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "closure.hpp"
@@ -42,6 +44,8 @@ struct SpecBase {
     virtual int sync() = 0;
     virtual int products() = 0;
     virtual int get_products(double *h) = 0;
+    // n, kcut, l, visc and precision (DNS_FP64 / DNS_MIXED) of the handle
+    virtual void params(dns_spec_params *out) const = 0;
     LaunchTimer timer;
 };
 
@@ -153,6 +157,15 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(dalloc(&factors, 4096));
         DNS_HIP_CHECK(hipHostMalloc((void **)&pinned, sizeof(double) * 4096));
         return DNS_OK;
+    }
+
+    void params(dns_spec_params *out) const override
+    {
+        out->n = g.n;
+        out->kcut = g.kcut;
+        out->l = l;
+        out->visc = visc;
+        out->precision = sizeof(T) == 8 ? DNS_FP64 : DNS_MIXED;
     }
 
     // --------------------------------------------------------- transfers
@@ -489,13 +502,16 @@ template <typename T> struct SpecImpl : SpecBase {
         int nb = (int)std::max<long>(1, std::min<long>(256, (count + SP_NTHR - 1) / SP_NTHR));
         size_t need = (size_t)nb * nsh;
         if (need > redcap) {
-            hipStreamSynchronize(st);
-            hipFree(red);
+            DNS_HIP_CHECK(hipStreamSynchronize(st));
+            DNS_HIP_CHECK(hipFree(red));
+            red = nullptr;
+            redcap = 0;
+            DNS_TRY(dalloc(&red, need));
             redcap = need;
-            DNS_TRY(dalloc(&red, redcap));
         }
         timer.begin(SFAM_FORCE);
-        hipLaunchKernelGGL((k_sp_shells<T>), dim3(nb), dim3(SP_NTHR), sizeof(double) * nsh, st, uh, s0, nsh, r, count,
+        hipLaunchKernelGGL((k_sp_shells<T>), dim3(nb), dim3(SP_NTHR),
+                           sizeof(double) * nsh + (sizeof(double) + sizeof(int)) * SP_NTHR, st, uh, s0, nsh, r, count,
                            g, red);
         hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nsh, SP_NTHR)), dim3(SP_NTHR), 0, st, red, nb, nsh, factors);
         timer.end(SFAM_FORCE);
@@ -629,6 +645,194 @@ template <typename T> int sfs_physical(SpecImpl<T> *L, double *ubar, double *tau
         DNS_TRY(fetch(tau));
         DNS_TRY(L->spec_to_phys_from(L->aux + 3 * L->ns));
         DNS_TRY(fetch(tau + 3 * n3));
+    }
+    return DNS_OK;
+}
+
+// ---------------------------------------------------------- snapshots
+// spectral_save / spectral_load! (src/spectral/io.jl:29-104) on one GPU:
+// <prefix>.bin holds the (kcut+1, m, m, 3) complex state column-major in
+// global index order, component after component, as Complex{T} with T the
+// handle's storage type (Float64, or Float32 in the mixed form); the
+// <prefix>.toml sidecar carries format, version, eltype, n, kcut, m, l, visc,
+// time, step, order, data and an optional [meta] table, keys sorted.  Both
+// files are written under .tmp names and renamed when complete (sidecar
+// first, then data, as the reference does).
+
+// shortest decimal that reads back as v, with a '.' or exponent (a TOML float)
+static std::string toml_float(double v)
+{
+    char buf[40];
+    if (std::isnan(v)) return "nan";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    for (int p = 1; p <= 17; ++p) {
+        snprintf(buf, sizeof buf, "%.*g", p, v);
+        if (strtod(buf, nullptr) == v) break;
+    }
+    std::string out(buf);
+    if (out.find_first_of(".en") == std::string::npos) out += ".0";
+    return out;
+}
+
+static std::string toml_str(const std::string &v)
+{
+    std::string out = "\"";
+    for (char c : v) {
+        if (c == '"' || c == '\\') out += '\\';
+        out += c;
+    }
+    return out + "\"";
+}
+
+static const char *eltype_name(int precision) { return precision == DNS_MIXED ? "Float32" : "Float64"; }
+
+static int spec_save(SpecBase &impl, const char *prefix, double time, int64_t step, const char *meta)
+{
+    dns_spec_params P;
+    impl.params(&P);
+    const size_t K = (size_t)P.kcut + 1, m = (size_t)2 * P.kcut + 1, ns = K * m * m;
+    std::vector<double> h(6 * ns);
+    DNS_TRY(impl.get_state(h.data(), 0));
+    const std::string data = std::string(prefix) + ".bin", side = std::string(prefix) + ".toml";
+    const std::string datatmp = data + ".tmp", sidetmp = side + ".tmp";
+    FILE *f = fopen(datatmp.c_str(), "wb");
+    if (!f) {
+        set_error("spectral_save: cannot open %s for writing", datatmp.c_str());
+        return DNS_EINVAL;
+    }
+    size_t wrote = 0;
+    if (P.precision == DNS_MIXED) {
+        std::vector<float> hf(h.begin(), h.end());
+        wrote = fwrite(hf.data(), sizeof(float), hf.size(), f);
+    } else {
+        wrote = fwrite(h.data(), sizeof(double), h.size(), f);
+    }
+    if (fclose(f) != 0 || wrote != h.size()) {
+        remove(datatmp.c_str());
+        set_error("spectral_save: short write to %s", datatmp.c_str());
+        return DNS_EINVAL;
+    }
+    const size_t slash = data.find_last_of('/');
+    const std::string base = slash == std::string::npos ? data : data.substr(slash + 1);
+    std::string t;
+    t += "data = " + toml_str(base) + "\n";
+    t += "eltype = " + toml_str(eltype_name(P.precision)) + "\n";
+    t += "format = " + toml_str("DistributedNavierStokes spectral state") + "\n";
+    t += "kcut = " + std::to_string(P.kcut) + "\n";
+    t += "l = " + toml_float(P.l) + "\n";
+    t += "m = " + std::to_string(m) + "\n";
+    t += "n = " + std::to_string(P.n) + "\n";
+    t += "order = " +
+         toml_str("column-major (kx, ky, kz, component) complex; kx = 0:kcut, ky/kz wrapped 0:kcut then -kcut:-1") +
+         "\n";
+    t += "step = " + std::to_string((long long)step) + "\n";
+    t += "time = " + toml_float(time) + "\n";
+    t += "version = 1\n";
+    t += "visc = " + toml_float(P.visc) + "\n";
+    if (meta && *meta) {
+        t += "\n[meta]\n";
+        t += meta;
+        if (t.back() != '\n') t += "\n";
+    }
+    f = fopen(sidetmp.c_str(), "wb");
+    if (!f) {
+        remove(datatmp.c_str());
+        set_error("spectral_save: cannot open %s for writing", sidetmp.c_str());
+        return DNS_EINVAL;
+    }
+    wrote = fwrite(t.data(), 1, t.size(), f);
+    if (fclose(f) != 0 || wrote != t.size()) {
+        remove(datatmp.c_str());
+        remove(sidetmp.c_str());
+        set_error("spectral_save: short write to %s", sidetmp.c_str());
+        return DNS_EINVAL;
+    }
+    if (rename(sidetmp.c_str(), side.c_str()) != 0 || rename(datatmp.c_str(), data.c_str()) != 0) {
+        set_error("spectral_save: cannot rename the snapshot of %s into place", prefix);
+        return DNS_EINVAL;
+    }
+    return DNS_OK;
+}
+
+// value text of a top-level `key = value` line of the sidecar ("" if absent)
+static std::string toml_lookup(const std::string &text, const char *key)
+{
+    size_t pos = 0;
+    const size_t klen = strlen(key);
+    while (pos < text.size()) {
+        size_t eol = text.find('\n', pos);
+        if (eol == std::string::npos) eol = text.size();
+        std::string line = text.substr(pos, eol - pos);
+        pos = eol + 1;
+        size_t a = line.find_first_not_of(" \t");
+        if (a == std::string::npos) continue;
+        if (line[a] == '[') break; // tables follow the top-level keys
+        if (line.compare(a, klen, key) != 0) continue;
+        size_t b = line.find_first_not_of(" \t", a + klen);
+        if (b == std::string::npos || line[b] != '=') continue;
+        b = line.find_first_not_of(" \t", b + 1);
+        if (b == std::string::npos) return "";
+        std::string v = line.substr(b);
+        while (!v.empty() && (v.back() == ' ' || v.back() == '\r' || v.back() == '\t')) v.pop_back();
+        if (v.size() >= 2 && v.front() == '"' && v.back() == '"') v = v.substr(1, v.size() - 2);
+        return v;
+    }
+    return "";
+}
+
+static int spec_load(SpecBase &impl, const char *prefix, double *time, int64_t *step)
+{
+    dns_spec_params P;
+    impl.params(&P);
+    const std::string side = std::string(prefix) + ".toml", data = std::string(prefix) + ".bin";
+    FILE *f = fopen(side.c_str(), "rb");
+    if (!f) {
+        set_error("spectral_load!: cannot open %s", side.c_str());
+        return DNS_EINVAL;
+    }
+    std::string text;
+    char buf[4096];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    fclose(f);
+    const std::string eltype = toml_lookup(text, "eltype"), kc = toml_lookup(text, "kcut"),
+                      ls = toml_lookup(text, "l");
+    const long kcut = kc.empty() ? -1 : strtol(kc.c_str(), nullptr, 10);
+    const double lv = ls.empty() ? NAN : strtod(ls.c_str(), nullptr);
+    if (eltype != eltype_name(P.precision) || kcut != P.kcut || !(lv == P.l)) {
+        set_error("snapshot %s (eltype %s, kcut %ld, l %s) does not match setup (eltype %s, kcut %d, l %.17g)",
+                  prefix, eltype.c_str(), kcut, ls.c_str(), eltype_name(P.precision), P.kcut, P.l);
+        return DNS_EINVAL;
+    }
+    const size_t K = (size_t)P.kcut + 1, m = (size_t)2 * P.kcut + 1, ns = K * m * m;
+    std::vector<double> h(6 * ns);
+    f = fopen(data.c_str(), "rb");
+    if (!f) {
+        set_error("spectral_load!: cannot open %s", data.c_str());
+        return DNS_EINVAL;
+    }
+    size_t nread = 0;
+    if (P.precision == DNS_MIXED) {
+        std::vector<float> hf(h.size());
+        nread = fread(hf.data(), sizeof(float), hf.size(), f);
+        std::copy(hf.begin(), hf.end(), h.begin());
+    } else {
+        nread = fread(h.data(), sizeof(double), h.size(), f);
+    }
+    const bool extra = fgetc(f) != EOF;
+    fclose(f);
+    if (nread != h.size() || extra) {
+        set_error("spectral_load!: %s does not hold %zu complex values", data.c_str(), 3 * ns);
+        return DNS_EINVAL;
+    }
+    DNS_TRY(impl.set_state(h.data(), 0));
+    if (time) {
+        const std::string tv = toml_lookup(text, "time");
+        *time = tv.empty() ? 0.0 : strtod(tv.c_str(), nullptr);
+    }
+    if (step) {
+        const std::string sv = toml_lookup(text, "step");
+        *step = sv.empty() ? 0 : strtoll(sv.c_str(), nullptr, 10);
     }
     return DNS_OK;
 }
@@ -882,6 +1086,24 @@ int dns_sfs_append(dns_spec *les, dns_cnn *net)
     return DNS_EINVAL;
 }
 
+int dns_spec_save(dns_spec *h, const char *prefix, double time, int64_t step, const char *meta_toml)
+{
+    SP_CHECK(h);
+    if (!prefix || !*prefix) {
+        set_error("null or empty prefix");
+        return DNS_EINVAL;
+    }
+    return spec_save(*h->impl, prefix, time, step, meta_toml);
+}
+int dns_spec_load(dns_spec *h, const char *prefix, double *time, int64_t *step)
+{
+    SP_CHECK(h);
+    if (!prefix || !*prefix) {
+        set_error("null or empty prefix");
+        return DNS_EINVAL;
+    }
+    return spec_load(*h->impl, prefix, time, step);
+}
 int dns_spec_sync(dns_spec *h)
 {
     SP_CHECK(h);

@@ -602,30 +602,54 @@ __device__ inline bool sp_cube(long t, int r, const SpecGeom &g, int &i, int &j,
 
 // Per-block shell energies (src/spectral/forcing.jl:12-24) of shells
 // s0..s0+ns−1 over the cube |k_i| <= smax; part[block * ns + s].
+// Deterministic: the block stages 256 (shell, energy) pairs in LDS, then the
+// thread that owns shell b adds that chunk's contributions to b in index
+// order, so every run sums in the same order.  Dynamic LDS: ns doubles +
+// 256 doubles + 256 ints.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sp_shells(const cplx<T> *uh, int s0, int ns, int r, long count, SpecGeom g,
                                                    double *part)
 {
     extern __shared__ unsigned char smem_raw[];
     double *bins = reinterpret_cast<double *>(smem_raw);
-    for (int b = threadIdx.x; b < ns; b += blockDim.x) bins[b] = 0.0;
-    __syncthreads();
+    double *val = bins + ns;
+    int *key = reinterpret_cast<int *>(val + 256);
+    const int tid = threadIdx.x;
+    for (int b = tid; b < ns; b += 256) bins[b] = 0.0;
     const size_t cs = (size_t)g.K * g.m * g.m;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        sp_cube(t, r, g, i, j, k);
-        int sh = sp_shell(i, j, k, g) - s0;
-        if (sh < 0 || sh >= ns) continue;
-        size_t base = i + (size_t)g.K * (j + (size_t)g.m * k);
+    for (long base = (long)blockIdx.x * 256; base < count; base += (long)gridDim.x * 256) {
+        const long t = base + tid;
+        int sh = -1;
         double e = 0;
-        for (int c = 0; c < 3; ++c) {
-            cplx<T> a = uh[base + c * cs];
-            e += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+        if (t < count) {
+            int i, j, k;
+            sp_cube(t, r, g, i, j, k);
+            sh = sp_shell(i, j, k, g) - s0;
+            if (sh >= 0 && sh < ns) {
+                const size_t at = i + (size_t)g.K * (j + (size_t)g.m * k);
+                for (int c = 0; c < 3; ++c) {
+                    cplx<T> a = uh[at + c * cs];
+                    e += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+                }
+                e = (i == 0 ? 1.0 : 2.0) * e / 2;
+            } else {
+                sh = -1;
+            }
         }
-        atomicAdd(&bins[sh], (i == 0 ? 1.0 : 2.0) * e / 2);
+        __syncthreads(); // the previous chunk's readers are done with key/val
+        key[tid] = sh;
+        val[tid] = e;
+        __syncthreads();
+        const int cnt = (int)min((long)256, count - base);
+        for (int b = tid; b < ns; b += 256) {
+            double acc = bins[b];
+            for (int q = 0; q < cnt; ++q)
+                if (key[q] == b) acc += val[q];
+            bins[b] = acc;
+        }
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < ns; b += blockDim.x) part[(size_t)blockIdx.x * ns + b] = bins[b];
+    for (int b = tid; b < ns; b += 256) part[(size_t)blockIdx.x * ns + b] = bins[b];
 }
 
 template <typename T>

@@ -230,6 +230,21 @@ int dns_spec_products(dns_spec *h);
 int dns_spec_get_products(dns_spec *h, double *host);
 int dns_spec_sync(dns_spec *h);
 
+/* spectral_save(prefix, uh, s; time, step, meta) -- src/spectral/io.jl:29-75:
+ * writes the state to <prefix>.bin, the (kcut+1, m, m, 3) complex array
+ * column-major in global index order, one component after the other, as
+ * Complex{Float64} (Complex{Float32} for DNS_MIXED handles), and the TOML
+ * sidecar <prefix>.toml (format, version, eltype, n, kcut, m, l, visc, time,
+ * step, order, data; keys sorted).  meta_toml (may be NULL) holds TOML
+ * `key = value` lines stored under a [meta] table.  Atomic: both files
+ * appear under their final names only when complete. */
+int dns_spec_save(dns_spec *h, const char *prefix, double time, int64_t step, const char *meta_toml);
+/* spectral_load!(uh, prefix, s) -- src/spectral/io.jl:77-104: reads a
+ * snapshot into the state; its eltype, kcut and l must match the handle's
+ * (DNS_EINVAL with the reference's message otherwise), n need not.  time and
+ * step (may be NULL) receive the sidecar values. */
+int dns_spec_load(dns_spec *h, const char *prefix, double *time, int64_t *step);
+
 /* -------------------------------------------- SFS data (learned closure)
  * sfs_sample! for one Gaussian bank cell (src/spectral/les.jl:313-382):
  * ū = T⅔(G(cut(û))) and τ = T⅔(G(cut(σ̂)) − F[ū_i ū_j]/M³), trace-free, with

@@ -589,3 +589,50 @@ def test_errors_raise():
         dns.setup(n=(8, 8, 8), bc=("periodic", "periodic", "periodic"), ydiffusion="implicit")
     with pytest.raises(dns.DNSError):
         dns.setup(n=(14, 8, 8))  # nx/2 = 7: unsupported radix
+
+
+def test_channelprofiles_laminar_and_moments():
+    """channelprofiles (src/stats.jl:65-98): the laminar profile U = 1 − y²
+    comes back exactly with zero fluctuations and the reference's lengths
+    (test/serialtests.jl:127-139); for a random field every profile equals
+    the plane averages and central moments computed with numpy from the
+    downloaded field (u interpolated to the y faces for uv, src/stats.jl:52-54)."""
+    prob = channelproblem(n=(16, 12, 8), visc=0.02)
+    s = Setup(prob, ROCmBackend())
+    u = vzeros(prob)
+    fill_velocity(prob, u, lambda x, y, z: 1 - y ** 2)
+    cs = dns.channelstats(s)
+    s.upload_u(u)
+    L = dns._lib.lib()
+    for _ in range(3):
+        dns._lib.check(L.dns_fv_channelstats_sample(s.handle, dns._lib.dptr(cs.acc)))
+        cs.nsamples += 1
+    pr = dns.channelprofiles(cs, s)
+    ny = prob.n[1]
+    assert len(pr.U) == len(pr.W) == len(pr.uu) == len(pr.ww) == ny
+    assert len(pr.V) == len(pr.vv) == len(pr.uv) == len(pr.yf) == ny - 1
+    assert np.abs(pr.U - (1 - pr.y ** 2)).max() < 1e-13
+    for a in (pr.V, pr.W, pr.uu, pr.vv, pr.ww, pr.uv):
+        assert np.abs(a).max() < 1e-13
+    assert pr.nsamples == 3
+    # random field: numpy plane statistics of the same field
+    ur = randomfield(prob, seed=61)
+    OracleFV(prob).exchange_halo_u(ur)
+    cs2 = dns.channelstats(s)
+    s.upload_u(ur)
+    dns._lib.check(L.dns_fv_channelstats_sample(s.handle, dns._lib.dptr(cs2.acc)))
+    cs2.nsamples = 1
+    pr2 = dns.channelprofiles(cs2, s)
+    ux, uy, uz = (a[1:-1, 1:-1, 1:-1] for a in ur)
+    U, W = ux.mean(axis=(0, 2)), uz.mean(axis=(0, 2))
+    V = uy[:, :ny - 1, :].mean(axis=(0, 2))
+    uf = (ur[0][1:-1, 1:ny, 1:-1] + ur[0][1:-1, 2:ny + 1, 1:-1]) / 2      # u at (xf, yf, zc)
+    vx = (ur[1][1:-1, 1:ny, 1:-1] + ur[1][2:, 1:ny, 1:-1]) / 2             # v at (xf, yf, zc)
+    Uf = uf.mean(axis=(0, 2))
+    np.testing.assert_allclose(pr2.U, U, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(pr2.W, W, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(pr2.V, V, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(pr2.uu, (ux ** 2).mean(axis=(0, 2)) - U ** 2, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(pr2.ww, (uz ** 2).mean(axis=(0, 2)) - W ** 2, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(pr2.vv, (uy[:, :ny - 1, :] ** 2).mean(axis=(0, 2)) - V ** 2, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(pr2.uv, (uf * vx).mean(axis=(0, 2)) - Uf * V, rtol=0, atol=1e-13)
