@@ -822,7 +822,6 @@ template <typename T> struct FVImpl : FVBase {
     int launch_cn(double B, double alpha, double dt)
     {
         const double theta = alpha * dt * visc / 2;
-        timer.begin(FAM_CN);
         hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(NTHR), (size_t)6 * g.ny * sizeof(double), st, ydc[0], ydc[1], ydc[2],
                            ydc[3], ydc[4], ydc[5], theta, g.ny, cncoef);
         for (int c = 0; c < 3; ++c) {
@@ -843,6 +842,7 @@ template <typename T> struct FVImpl : FVBase {
             a.theta = theta;
             a.rdx = 1.0 / dx;
             a.rdz = 1.0 / dz;
+            timer.begin(FAM_CN); // one component per timed launch
             if (cn_tl && cn_seg) {
                 const dim3 grd(ceil_div(g.nx, cn_tl), g.nz), blk(cn_nthr);
                 switch (cn_seg) {
@@ -869,8 +869,8 @@ template <typename T> struct FVImpl : FVBase {
                 else if (c == 1) hipLaunchKernelGGL((k_cn<T, 1>), grd, blk, 0, st, a, g);
                 else hipLaunchKernelGGL((k_cn<T, 2>), grd, blk, 0, st, a, g);
             }
+            timer.end(FAM_CN);
         }
-        timer.end(FAM_CN);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }

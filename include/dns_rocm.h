@@ -159,7 +159,8 @@ int dns_fv_describe(dns_fv *h, char *buf, int32_t len);
 /* Kernel timing hook used by the benchmark: when enabled, the library
  * records HIP events around every launch of the named kernel family on the
  * handle's stream and accumulates their durations.
- * family: 0 momentum, 1 x-pass, 2 z-pass, 3 y-solve, 4 pgrad, 5 CN. */
+ * family: 0 momentum, 1 x-pass, 2 z-pass, 3 y-solve, 4 pgrad, 5 CN (one
+ * velocity component per launch). */
 int dns_fv_timing(dns_fv *h, int32_t enable);
 int dns_fv_timing_read(dns_fv *h, int32_t family, double *total_ms, int64_t *launches);
 

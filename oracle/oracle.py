@@ -37,10 +37,18 @@ def lib():
         for name in ("ofv_cfl_timestep", "ofv_maxdiv", "ofv_kinetic_energy"):
             getattr(L, name).restype = ct.c_double
         L.ofv_rbuf.restype = _dp
+        L.oracle_set_threads.restype = ct.c_int
+        L.oracle_set_threads.argtypes = [ct.c_int]
         L.ospec_splitmix64.restype = ct.c_uint64
         L.ospec_splitmix64.argtypes = [ct.c_uint64]
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """OpenMP threads of later oracle calls (n <= 0: unchanged); returns the
+    count in effect."""
+    return int(lib().oracle_set_threads(int(n)))
 
 
 def _p(a):

@@ -73,6 +73,22 @@ static double modlam(int k0, int n, double d)
     return 2.0 * (c - 1.0) / (d * d);
 }
 
+/* OpenMP threads of every later oracle call (the CPU baseline times one
+ * thread and all host cores); returns the count in effect. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+int oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 void ofv_destroy(ofv *h)
 {
     if (!h) return;
