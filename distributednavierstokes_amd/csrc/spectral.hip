@@ -63,6 +63,7 @@ template <typename T> struct SpecImpl : SpecBase {
     FFTDesc fdn{}, fdh{};
     int tb = 1, tbf = 1, tbs = 1, lpb = 1;
     int xr_lpi = 0, xr_blocks = 0; // persistent RHS x pass: lines per block, blocks (0: k_sp_xmid)
+    bool xr_wave = false;          // RHS x pass with one wavefront per line (k_sp_xrhs256, n = 512)
     size_t ldsxr = 0;
     size_t lds = 0, ldss = 0, ldsx = 0;
     double *red = nullptr, *redout = nullptr, *factors = nullptr, *pinned = nullptr;
@@ -150,6 +151,10 @@ template <typename T> struct SpecImpl : SpecBase {
             ldsxr = (size_t)3 * g.n * sizeof(T) + ((size_t)3 * (g.n / 2) + 3 * g.K + g.n + 1) * sz;
             if ((e && atoi(e) == 0) || 3 * g.K > XR_RP * SP_NTHR || ldsxr + 2048 > (size_t)maxlds) xr_lpi = 0;
             xr_blocks = xr_lpi ? (int)ceil_div((long)g.n * g.n, xr_lpi) : 0;
+            // n = 512 (256-point half-length lines): one wavefront per line;
+            // DNS_SP_XWAVE=0 keeps the block-per-line kernels
+            const char *ew = getenv("DNS_SP_XWAVE");
+            xr_wave = g.n == 512 && !(ew && atoi(ew) == 0);
         }
         redcap = std::max<size_t>((size_t)SP_REDB * 4, 2 * ((size_t)g.n * g.n / lpb + 16));
         DNS_TRY(dalloc(&red, redcap));
@@ -261,7 +266,10 @@ template <typename T> struct SpecImpl : SpecBase {
         a.fK = make_fastdiv(g.K);
         a.flpb = make_fastdiv(lpb);
         timer.begin(SFAM_X);
-        if (MODE == XMID_RHS && xr_lpi)
+        if (MODE == XMID_RHS && xr_wave)
+            hipLaunchKernelGGL((k_sp_xrhs256<T>), dim3(ceil_div((long)g.n * g.n, 4)), dim3(SP_NTHR),
+                               (size_t)(256 + 257 + 4 * 256) * sizeof(cplx<T>), st, a, g);
+        else if (MODE == XMID_RHS && xr_lpi)
             hipLaunchKernelGGL((k_sp_xrhs<T, SP_XEPT>), dim3(xr_blocks), dim3(SP_NTHR), ldsxr, st, a, g, fdh, xr_lpi);
         else
             hipLaunchKernelGGL((k_sp_xmid<T, SP_XEPT, MODE>), dim3(ceil_div((long)g.n * g.n, lpb)), dim3(SP_NTHR),
@@ -449,7 +457,11 @@ template <typename T> struct SpecImpl : SpecBase {
     }
 
     // vmax partials written by the RHS x pass
-    int rhs_parts() const { return xr_lpi ? xr_blocks : (int)ceil_div((long)g.n * g.n, lpb); }
+    int rhs_parts() const
+    {
+        if (xr_wave) return (int)ceil_div((long)g.n * g.n, 4);
+        return xr_lpi ? xr_blocks : (int)ceil_div((long)g.n * g.n, lpb);
+    }
 
     int vmax_final(int nparts, double *out)
     {

@@ -787,4 +787,145 @@ __global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *s
     }
 }
 
+// ------------------------------------------- wavefront x pass (n = 512)
+
+// Orders the LDS accesses of one wavefront around an exchange (the lanes of
+// a wavefront run in lockstep and LDS serves one wavefront's requests in
+// order, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 256-point complex FFT by one wavefront: lane l holds a[r] = x[l + 64 r] on
+// entry and X[l + 64 r] on return.  Stockham radix-4 in four passes, the
+// same schedule as fft_lds (pass with sub-transform size Ns: inputs
+// x[j + 64 r] twiddled by w^(r k 64 / Ns), k = j mod Ns, outputs at
+// (j div Ns) 4 Ns + k + r Ns) with j = l: pass 0 reads the registers as
+// loaded, passes 1-3 read the previous pass's outputs from the wavefront's
+// 256-entry LDS buffer sh, and the outputs of pass 3 land at l + 64 r, in
+// the registers.  tw[t] = exp(-2πi t / 256); unnormalised.
+template <typename T, bool INV>
+__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw, int lane)
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int Ns = 1 << (2 * p);
+        const int k = lane & (Ns - 1);
+        if (p > 0) {
+#pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                const cplx<T> w = tw[r * k * (64 / Ns)];
+                a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
+            }
+        }
+        Dft<T, 4, INV>::run(a);
+        if (p < 3) {
+            const int base = (lane - k) * 4 + k;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sh[base + r * Ns] = a[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = sh[lane + 64 * r];
+            wave_lds_sync();
+        }
+    }
+}
+
+// The RHS x pass (k_sp_xmid, XMID_RHS) for n = 512: one wavefront per
+// (y, z) line L, four lines per block.  The wavefront loads the three
+// velocity rows P[L, 0:K, c], forms the half-length complex sequence of
+// each c2r (Z_k = E_k + i O_k from X_k and conj(X_{N-k})), transforms it
+// with wave_fft256 and keeps the 512 physical values of each component in
+// registers (8 per lane); the velocity bound max Σ_c |v_c| is a wavefront
+// reduction; each of the six products v_a v_b is formed in registers,
+// packed as z_t = (p_2t, p_2t+1), transformed forward and unpacked
+// (P_k = ½[(Z_k + conj Z_{N-k}) − i w^k (Z_k − conj Z_{N-k})], the partner
+// Z_{N-k} through LDS) into P[L, 0:K, p].  Same arithmetic as k_sp_xmid with
+// a different radix order (4·4·4·4 instead of 8·8·4).  No workgroup
+// barrier after the twiddle staging.  One vmax partial per block.
+// LDS: the 256 transform and 257 real-transform twiddles, 4 × 256 complex.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
+{
+    constexpr int N = 256;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *tw = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *twx = tw + N;
+    cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * N;
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = a.tw[t];
+    for (int t = threadIdx.x; t <= N; t += blockDim.x) twx[t] = a.twx[t];
+    __syncthreads();
+    const int K = g.K, n = g.n, nlines = n * n;
+    const int L = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+    const bool live = L < nlines;
+    const size_t pstride = (size_t)K * n * n;
+    const cplx<T> zero = mk<T>(T(0), T(0));
+    T v[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const cplx<T> *row = a.P + (size_t)(live ? L : 0) * K + c * pstride;
+        cplx<T> z[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = lane + 64 * r;
+            cplx<T> Xk = (live && k < K) ? row[k] : zero;
+            cplx<T> Xn = (live && N - k < K) ? row[N - k] : zero;
+            if (k == 0) {
+                Xk.y = T(0);
+                Xn.y = T(0);
+            }
+            const cplx<T> Xc = cconj<T>(Xn);
+            const cplx<T> ev = cadd<T>(Xk, Xc);
+            const cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), twx[k]);
+            z[r] = mk<T>(ev.x - od.y, ev.y + od.x);
+        }
+        wave_fft256<T, true>(z, sh, tw, lane);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            v[c][2 * r] = z[r].x;
+            v[c][2 * r + 1] = z[r].y;
+        }
+    }
+    double mx = 0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        mx = fmax(mx, fabs((double)v[0][e]) + fabs((double)v[1][e]) + fabs((double)v[2][e]));
+    if (!live) mx = 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) red[wv] = mx;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        const int ca = p < 3 ? p : (p == 3 ? 0 : p == 4 ? 1 : 2);
+        const int cb = p < 3 ? p : (p == 3 ? 1 : p == 4 ? 2 : 0);
+        cplx<T> z[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) z[r] = mk<T>(v[ca][2 * r] * v[cb][2 * r], v[ca][2 * r + 1] * v[cb][2 * r + 1]);
+        wave_fft256<T, false>(z, sh, tw, lane);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sh[lane + 64 * r] = z[r];
+        wave_lds_sync();
+        cplx<T> *out = a.P + (size_t)L * K + p * pstride;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int kx = lane + 64 * r;
+            if (live && kx < K) {
+                const cplx<T> Zk = z[r];
+                const cplx<T> Zc = cconj<T>(sh[(N - kx) & (N - 1)]);
+                const cplx<T> sm = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
+                const cplx<T> wd = cmul<T>(twx[kx], df);
+                out[kx] = mk<T>(T(0.5) * (sm.x + wd.y), T(0.5) * (sm.y - wd.x));
+            }
+        }
+        wave_lds_sync();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) a.vpart[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 }  // namespace dns
