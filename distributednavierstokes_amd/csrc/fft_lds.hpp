@@ -240,4 +240,70 @@ __device__ inline void fft_lds(cplx<T> *buf, int nlines, int ls, const FFTDesc &
     }
 }
 
+// ------------------------------------------------ wavefront-group FFTs
+
+// Orders the LDS accesses of one wavefront around an exchange (the lanes of
+// a wavefront run in lockstep and LDS serves one wavefront's requests in
+// order, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// N-point complex FFT (N = 4^a or 2·4^a, 16 <= N <= 256) by a group of
+// G = N/4 lanes of one wavefront (64/G lines per wavefront).  Lane t of the
+// group holds a[r] = x[t + r G] on entry and X[t + r G] on return.  The same
+// Stockham schedule as fft_lds (pass of radix R with sub-transform size Ns:
+// inputs x[j + r N/R] twiddled by w^(r k N/(Ns R)), k = j mod Ns, outputs at
+// (j div Ns) R Ns + k + r Ns): radix-4 passes, then one radix-2 pass when N
+// is 2·4^a, whose two butterflies per lane (j = t, t + G) take a[0], a[2] and
+// a[1], a[3].  Between passes the outputs go through the group's N-entry LDS
+// line sh; the last pass's outputs land at t + r G, in the registers.
+// tw[t] = exp(-2πi t / N); unnormalised.
+template <typename T, bool INV, int N>
+__device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw, int t)
+{
+    constexpr int G = N / 4;
+    constexpr int P4 = (N == 16 || N == 32) ? 2 : (N == 64 || N == 128) ? 3 : 4; // radix-4 passes
+    constexpr bool R2 = (N == 32 || N == 128 || N == 512);
+    static_assert(N == 16 || N == 32 || N == 64 || N == 128 || N == 256, "group_fft: N not supported");
+#pragma unroll
+    for (int p = 0; p < P4; ++p) {
+        const int Ns = 1 << (2 * p);
+        const int k = t & (Ns - 1);
+        if (p > 0) {
+#pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                const cplx<T> w = tw[r * k * (N / (4 * Ns))];
+                a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
+            }
+        }
+        Dft<T, 4, INV>::run(a);
+        if (p < P4 - 1 || R2) {
+            const int base = (t - k) * 4 + k;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sh[base + r * Ns] = a[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = sh[t + G * r];
+            wave_lds_sync();
+        }
+    }
+    if (R2) {
+        // Ns = N/2: butterflies j = t (a[0], a[2]) and j = t + G (a[1], a[3]),
+        // k = j, outputs at j and j + N/2: a[r] ends as X[t + r G]
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            cplx<T> b[2] = {a[h], a[h + 2]};
+            const cplx<T> w = tw[t + h * G];
+            b[1] = INV ? cmulc<T>(b[1], w) : cmul<T>(b[1], w);
+            Dft<T, 2, INV>::run(b);
+            a[h] = b[0];
+            a[h + 2] = b[1];
+        }
+    }
+}
+
 }  // namespace dns

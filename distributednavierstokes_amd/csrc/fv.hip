@@ -147,6 +147,7 @@ template <typename T> struct FVImpl : FVBase {
     int N = 0;
     int nb_pgrad = 0;
     bool pgrad_vec = false; // k_pgrad_vec (16-byte accesses) instead of k_pgrad
+    bool xwave = false;     // x passes by wavefront lane groups (nx/2 in 16..256, a power of two)
     bool umax_valid = false; // upart holds max |u_c| partials of the current u
     int momentum_kind = 5; // 5 LDS z-march (k_momentum_lz), 3 slab-ordered (k_momentum_slab), 1 direct (k_momentum)
     // reference-structured ("plain") step: extra arrays, allocated on first use
@@ -373,6 +374,12 @@ template <typename T> struct FVImpl : FVBase {
         if (maxlds <= 0) maxlds = 65536;
         lpb = std::max(1, std::min(64, X_EPT * NTHR / std::max(1, Nh)));
         if (sizeof(T) == 8) lpb = std::min(lpb, 4); // FP64: four lines per block measured fastest (C2, C3)
+        {
+            // the wavefront-group x passes serve nx/2 = 16, 32, ..., 256;
+            // DNS_FV_XWAVE=0 keeps the block kernels
+            const char *ew = getenv("DNS_FV_XWAVE");
+            xwave = (Nh == 16 || Nh == 32 || Nh == 64 || Nh == 128 || Nh == 256) && !(ew && atoi(ew) == 0);
+        }
         if (const char *el = getenv("DNS_X_LPB")) // sweeps: fewer lines per x-pass block
             lpb = std::max(1, std::min(lpb, atoi(el)));
         ldsx = (size_t)(lpb + 1) * Nh * sizeof(cplx<T>); // lines + LDS twiddle table
@@ -705,7 +712,20 @@ template <typename T> struct FVImpl : FVBase {
         xa.fny = make_fastdiv(g.ny);
         const int nlines = g.ny * g.nz;
         timer.begin(FAM_X);
-        hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 0>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xa, g, fdx);
+        if (xwave) {
+            const int N = g.nx / 2, lpbw = 256 / (N / 4);
+            const dim3 grd(ceil_div(nlines, lpbw));
+            const size_t lds = (size_t)(N + N + 1 + lpbw * N) * sizeof(cplx<T>);
+            switch (N) {
+                case 16: hipLaunchKernelGGL((k_rfft_xw<T, 16>), grd, dim3(NTHR), lds, st, xa, g); break;
+                case 32: hipLaunchKernelGGL((k_rfft_xw<T, 32>), grd, dim3(NTHR), lds, st, xa, g); break;
+                case 64: hipLaunchKernelGGL((k_rfft_xw<T, 64>), grd, dim3(NTHR), lds, st, xa, g); break;
+                case 128: hipLaunchKernelGGL((k_rfft_xw<T, 128>), grd, dim3(NTHR), lds, st, xa, g); break;
+                default: hipLaunchKernelGGL((k_rfft_xw<T, 256>), grd, dim3(NTHR), lds, st, xa, g); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 0>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xa, g, fdx);
+        }
         timer.end(FAM_X);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
@@ -768,7 +788,20 @@ template <typename T> struct FVImpl : FVBase {
         xi.fN = make_fastdiv(g.nx / 2);
         xi.skip_dt = mom_dtp;
         timer.begin(FAM_X);
-        hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
+        if (xwave) {
+            const int N = g.nx / 2, lpbw = 256 / (N / 4);
+            const dim3 grd(ceil_div(nlines, lpbw));
+            const size_t lds = (size_t)(N + N + 1 + lpbw * N) * sizeof(cplx<T>);
+            switch (N) {
+                case 16: hipLaunchKernelGGL((k_irfft_xw<T, 16>), grd, dim3(NTHR), lds, st, xi, g); break;
+                case 32: hipLaunchKernelGGL((k_irfft_xw<T, 32>), grd, dim3(NTHR), lds, st, xi, g); break;
+                case 64: hipLaunchKernelGGL((k_irfft_xw<T, 64>), grd, dim3(NTHR), lds, st, xi, g); break;
+                case 128: hipLaunchKernelGGL((k_irfft_xw<T, 128>), grd, dim3(NTHR), lds, st, xi, g); break;
+                default: hipLaunchKernelGGL((k_irfft_xw<T, 256>), grd, dim3(NTHR), lds, st, xi, g); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_irfft_x<T, X_EPT>), dim3(ceil_div(nlines, lpb)), dim3(NTHR), ldsx, st, xi, g, fdx);
+        }
         timer.end(FAM_X);
         DNS_HIP_CHECK(hipGetLastError());
         (void)mode;
@@ -1067,6 +1100,9 @@ template <typename T> struct FVImpl : FVBase {
     int describe(char *buf, int len) override
     {
         const char *mom = momentum_kind == 5 ? "lz" : momentum_kind == 3 ? "slab" : "direct";
+        char xdesc[32];
+        if (xwave) snprintf(xdesc, sizeof xdesc, "wave groups of %d", g.nx / 8);
+        else snprintf(xdesc, sizeof xdesc, "lpb=%d", lpb);
         char ys[64], cs[96];
         if (!g.wall) snprintf(ys, sizeof ys, "fourier n=%d tile=%d/%d thr=%d", g.ny, tby, tbyf, nthry);
         else if (thomas_tl && thomas_seg) snprintf(ys, sizeof ys, "seg R=%d tl=%d thr=%d", thomas_seg, thomas_tl, thomas_nthr);
@@ -1077,9 +1113,9 @@ template <typename T> struct FVImpl : FVBase {
         else if (cn_tl) snprintf(cs, sizeof cs, "tile tl=%d thr=%d", cn_tl, cn_nthr);
         else snprintf(cs, sizeof cs, "sequential");
         const int n = snprintf(buf, (size_t)len,
-                               "prec=%s; momentum=%s zc=%d; xpass=half %d lpb=%d; zpass=n %d tile=%d/%d thr=%d; "
+                               "prec=%s; momentum=%s zc=%d; xpass=half %d %s; zpass=n %d tile=%d/%d thr=%d; "
                                "ysolve=%s; cn=%s; pgrad=%s",
-                               plain ? "plain" : sizeof(T) == 8 ? "fp64" : "mixed", mom, momentum_zc(), g.nx / 2, lpb,
+                               plain ? "plain" : sizeof(T) == 8 ? "fp64" : "mixed", mom, momentum_zc(), g.nx / 2, xdesc,
                                g.nz, tbz, tbzf, nthrz, ys, cs, pgrad_vec ? "vec" : "scalar");
         if (n < 0 || n >= len) {
             set_error("describe: buffer of %d bytes too small", len);

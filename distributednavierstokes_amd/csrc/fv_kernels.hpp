@@ -621,6 +621,122 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
 }
 
 // ------------------------------------------------------------------------
+// x passes with the transforms held by wavefront lane groups (nx/2 = N in
+// 16..256): G = N/4 lanes per line, 64/G lines per wavefront, 4 wavefronts
+// per block; the transform runs in registers with the group's own LDS line
+// for the exchanges (group_fft), so no workgroup barrier follows the twiddle
+// staging.  Same arithmetic as k_rfft_x<MODE 0> / k_irfft_x with the radix
+// order 4·…·4(·2) instead of 8·…
+// LDS: the N transform and N+1 real-transform twiddles, then N complex per
+// line (256/G lines per block).
+// ------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_rfft_xw(XPassArgs<T> a, FVGeom g)
+{
+    constexpr int G = N / 4, LPB = 256 / G;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *tw = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *twx = tw + N;
+    const int t = threadIdx.x % G, lb = threadIdx.x / G;
+    cplx<T> *sh = twx + (N + 1) + lb * N;
+    for (int q = threadIdx.x; q < N; q += blockDim.x) tw[q] = a.tw[q];
+    for (int q = threadIdx.x; q <= N; q += blockDim.x) twx[q] = a.twx[q];
+    const int nlines = g.ny * g.nz;
+    const int L = xcd_block(blockIdx.x, gridDim.x) * LPB + lb;
+    const bool live = L < nlines;
+    const int LL = live ? L : 0;
+    const int k = fdiv(LL, a.fny), j = LL - k * g.ny;
+    const int km = wrapm(k, g.nz);
+    const T rdyj = a.rdy[j + 1];
+    cplx<T> z[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        // divergence of cells i0 = 2(t + rG) and i0 + 1 (one pair load per row)
+        const int i0 = 2 * (t + r * G);
+        const int c0 = IX(i0, j, k, g);
+        const cplx<T> X = *reinterpret_cast<const cplx<T> *>(a.ux + c0);
+        const T xm = a.ux[IX(wrapm(i0, g.nx), j, k, g)];
+        const cplx<T> Y0 = *reinterpret_cast<const cplx<T> *>(a.uy + c0);
+        // face below: j − 1, the wall (0) or the periodic image ny − 1
+        cplx<T> Ym = mk<T>(T(0), T(0));
+        if (j > 0) Ym = *reinterpret_cast<const cplx<T> *>(a.uy + c0 - g.nx);
+        else if (!g.wall) Ym = *reinterpret_cast<const cplx<T> *>(a.uy + IX(i0, g.ny - 1, k, g));
+        const cplx<T> Z0 = *reinterpret_cast<const cplx<T> *>(a.uz + c0);
+        const cplx<T> Zm = *reinterpret_cast<const cplx<T> *>(a.uz + IX(i0, j, km, g));
+        const T y0 = Y0.x - Ym.x, y1 = Y0.y - Ym.y;
+        const T z0 = Z0.x - Zm.x, z1 = Z0.y - Zm.y;
+        z[r].x = (X.x - xm) * a.rdx + y0 * rdyj + z0 * a.rdz;
+        z[r].y = (X.y - X.x) * a.rdx + y1 * rdyj + z1 * a.rdz;
+    }
+    __syncthreads(); // the twiddles are staged
+    group_fft<T, false, N>(z, sh, tw, t);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sh[t + r * G] = z[r];
+    wave_lds_sync();
+    if (live) {
+        cplx<T> *out = a.A + (size_t)g.ldc * L;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            // kx = t + rG for r < 4, and kx = N (r = 4, lane t = 0)
+            const int kx = r < 4 ? t + r * G : N;
+            if (r == 4 && t != 0) continue;
+            const cplx<T> Zk = r < 4 ? z[r] : z[0];
+            const cplx<T> Zc = cconj<T>(sh[(N - kx) & (N - 1)]);
+            const cplx<T> sm = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
+            // X = (Zk + Zc)/2 − (i/2) w^kx (Zk − Zc)
+            const cplx<T> wd = cmul<T>(twx[kx], df);
+            out[kx] = mk<T>(T(0.5) * (sm.x + wd.y), T(0.5) * (sm.y - wd.x));
+        }
+    }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_irfft_xw(XInvArgs<T> a, FVGeom g)
+{
+    if (a.skip_dt && *a.skip_dt == 0.0) return; // uniform over the grid
+    constexpr int G = N / 4, LPB = 256 / G;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *tw = reinterpret_cast<cplx<T> *>(smem_raw);
+    cplx<T> *twx = tw + N;
+    const int t = threadIdx.x % G, lb = threadIdx.x / G;
+    cplx<T> *sh = twx + (N + 1) + lb * N;
+    for (int q = threadIdx.x; q < N; q += blockDim.x) tw[q] = a.tw[q];
+    for (int q = threadIdx.x; q <= N; q += blockDim.x) twx[q] = a.twx[q];
+    const int nlines = g.ny * g.nz;
+    const int L = xcd_block(blockIdx.x, gridDim.x) * LPB + lb;
+    const bool live = L < nlines;
+    const cplx<T> *row = a.A + (size_t)g.ldc * (live ? L : 0);
+    cplx<T> xk[4], xn[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = t + r * G;
+        xk[r] = row[k];
+        xn[r] = row[N - k];
+    }
+    __syncthreads(); // the twiddles are staged
+    cplx<T> z[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = t + r * G;
+        cplx<T> Xk = xk[r], Xn = xn[r];
+        if (k == 0) {
+            Xk.y = T(0);
+            Xn.y = T(0);
+        }
+        const cplx<T> Xc = cconj<T>(Xn);
+        const cplx<T> ev = cadd<T>(Xk, Xc);                  // 2 Ev
+        const cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), twx[k]); // 2 Od = w^-k (Xk − Xc)
+        z[r] = mk<T>(ev.x - od.y, ev.y + od.x);
+    }
+    group_fft<T, true, N>(z, sh, tw, t);
+    if (live) {
+        cplx<T> *out = reinterpret_cast<cplx<T> *>(a.out + (size_t)g.nx * L);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[t + r * G] = z[r];
+    }
+}
+
+// ------------------------------------------------------------------------
 // z pass: in-place complex FFT along z of A(kx, j, kz) for a tile of `tb`
 // consecutive kx at fixed j (coalesced tb·16-byte row segments).
 // ------------------------------------------------------------------------

@@ -789,16 +789,6 @@ __global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *s
 
 // ------------------------------------------- wavefront x pass (n = 512)
 
-// Orders the LDS accesses of one wavefront around an exchange (the lanes of
-// a wavefront run in lockstep and LDS serves one wavefront's requests in
-// order, so no workgroup barrier is needed).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // 256-point complex FFT by one wavefront: lane l holds a[r] = x[l + 64 r] on
 // entry and X[l + 64 r] on return.  Stockham radix-4 in four passes, the
 // same schedule as fft_lds (pass with sub-transform size Ns: inputs
