@@ -306,4 +306,85 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
     }
 }
 
+// 512-point complex FFT by one wavefront: lane t holds a[r] = x[t + 64 r]
+// (r = 0..7) on entry and X[t + 64 r] on return; Stockham radix 8 in three
+// passes (Ns = 1, 8, 64) with the exchanges through the line's LDS storage
+// sh, as group_fft.  tw[t] = exp(-2πi t / 512); unnormalised.
+template <typename T, bool INV>
+__device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const cplx<T> *tw, int t)
+{
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const int Ns = 1 << (3 * p);
+        const int k = t & (Ns - 1);
+        if (p > 0) {
+#pragma unroll
+            for (int r = 1; r < 8; ++r) {
+                const cplx<T> w = tw[r * k * (64 / Ns)];
+                a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
+            }
+        }
+        Dft<T, 8, INV>::run(a);
+        if (p < 2) {
+            const int base = (t - k) * 8 + k;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) sh[base + r * Ns] = a[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 8; ++r) a[r] = sh[t + 64 * r];
+            wave_lds_sync();
+        }
+    }
+}
+
+// In-place transforms of nlines LDS lines of N points (buf[line * ls + t])
+// by wavefront lane groups: G = N/4 lanes per line (64 for N = 512, 8 points
+// per lane), 64/G lines per wavefront at a time, the wavefronts of the block
+// taking lines in turn.  Needs nlines % (64/G) == 0.  Ends with a workgroup
+// barrier, as fft_lds.
+template <typename T, bool INV, int N>
+__device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls, const cplx<T> *tw)
+{
+    constexpr int G = N == 512 ? 64 : N / 4, LPW = 64 / G;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int t = lane % G, lg = lane / G;
+    for (int l0 = wv * LPW; l0 < nlines; l0 += nw * LPW) {
+        cplx<T> *line = buf + (size_t)(l0 + lg) * ls;
+        if constexpr (N == 512) {
+            cplx<T> a[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) a[r] = line[t + 64 * r];
+            wave_lds_sync();
+            wave_fft512<T, INV>(a, line, tw, t);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) line[t + 64 * r] = a[r];
+        } else {
+            cplx<T> a[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = line[t + G * r];
+            wave_lds_sync();
+            group_fft<T, INV, N>(a, line, tw, t);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) line[t + G * r] = a[r];
+        }
+        wave_lds_sync();
+    }
+    __syncthreads();
+}
+
+// The batched LDS transforms of a tile: lane groups for the lengths they
+// serve (16 ... 512, powers of two, nlines a multiple of the lines per
+// wavefront), otherwise fft_lds over sub-batches of at most sub lines.
+template <typename T, int EPT, bool INV>
+__device__ __forceinline__ void fft_lds_tile(cplx<T> *buf, int nlines, int ls, const FFTDesc &d,
+                                             const cplx<T> *tw, int sub)
+{
+    const int n = d.n;
+    if (n == 512) return fft_lds_groups<T, INV, 512>(buf, nlines, ls, tw);
+    if (n == 256) return fft_lds_groups<T, INV, 256>(buf, nlines, ls, tw);
+    if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128>(buf, nlines, ls, tw);
+    if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64>(buf, nlines, ls, tw);
+    for (int sb = 0; sb < nlines; sb += sub) fft_lds<T, EPT, INV>(buf + (size_t)sb * ls, min(sub, nlines - sb), ls, d, tw);
+}
+
 }  // namespace dns

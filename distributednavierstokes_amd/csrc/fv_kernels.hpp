@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, INV>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
+    fft_lds_tile<T, EPT, INV>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb;
         int kx = kx0 + b;
@@ -800,7 +800,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, false>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
+    fft_lds_tile<T, EPT, false>(buf, tb, ls, d, twl, tbf);
     if (DIVIDE) {
         for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
             int t = fdiv(idx, ftb), b = idx - t * tb;
@@ -812,7 +812,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = mk<T>(T((double)v.x * f), T((double)v.y * f));
         }
         __syncthreads();
-        for (int sb = 0; sb < tb; sb += tbf) fft_lds<T, EPT, true>(buf + sb * ls, min(tbf, tb - sb), ls, d, twl);
+        fft_lds_tile<T, EPT, true>(buf, tb, ls, d, twl, tbf);
     }
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb;
