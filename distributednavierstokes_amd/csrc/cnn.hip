@@ -399,11 +399,13 @@ struct CNN {
     // ------------------------------------------------------------ dataset
     template <typename T> int append(const T *ub, const T *tau, hipStream_t s)
     {
-        float *du_ = nullptr, *dt_ = nullptr;
-        DNS_TRY(dalloc(&du_, (size_t)3 * Vs));
-        DNS_TRY(dalloc(&dt_, (size_t)6 * Vs));
+        DevTemp<float> ub_, tb_;
+        DNS_TRY(ub_.alloc((size_t)3 * Vs));
+        DNS_TRY(tb_.alloc((size_t)6 * Vs));
+        float *du_ = ub_.p, *dt_ = tb_.p;
         ds_u.push_back(du_);
         ds_t.push_back(dt_);
+        ub_.p = tb_.p = nullptr;  // now owned by the dataset
         hipLaunchKernelGGL((k_cast_f32<T>), dim3(1024), dim3(256), 0, s, ub, 3 * Vs, du_);
         hipLaunchKernelGGL((k_cast_f32<T>), dim3(1024), dim3(256), 0, s, tau, 6 * Vs, dt_);
         DNS_HIP_CHECK(hipGetLastError());
@@ -461,8 +463,9 @@ struct CNN {
             set_error("empty dataset");
             return DNS_EINVAL;
         }
-        double *dl = nullptr;
-        DNS_TRY(dalloc(&dl, (size_t)std::max(1, nsteps)));
+        DevTemp<double> dlbuf;
+        DNS_TRY(dlbuf.alloc((size_t)std::max(1, nsteps)));
+        double *dl = dlbuf.p;
         std::mt19937_64 gen(seed);
         std::vector<int> perm(nds);
         size_t pos = nds;
@@ -488,7 +491,6 @@ struct CNN {
             if (hipMemcpyAsync(losses, dl, sizeof(double) * nsteps, hipMemcpyDeviceToHost, st) != hipSuccess) rc = DNS_EHIP;
         }
         if (hipStreamSynchronize(st) != hipSuccess) rc = DNS_EHIP;
-        hipFree(dl);
         if (rc != DNS_OK) set_error("HIP error in training loop");
         return rc;
     }
@@ -660,14 +662,14 @@ int dns_cnn_forward(dns_cnn *h, const float *x, float *yout)
 {
     if (!h || !x || !yout) return DNS_EINVAL;
     CNN &c = *h->impl;
-    float *tmp = nullptr;
-    DNS_TRY(dalloc(&tmp, (size_t)c.V * 3));
+    DevTemp<float> tbuf;
+    DNS_TRY(tbuf.alloc((size_t)c.V * 3));
+    float *tmp = tbuf.p;
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, x, sizeof(float) * c.V * 3, hipMemcpyHostToDevice, c.st));
     hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 16);
     DNS_TRY(c.forward(c.st, c.B));
     DNS_HIP_CHECK(hipMemcpyAsync(yout, c.y, sizeof(float) * c.V * 6, hipMemcpyDeviceToHost, c.st));
     DNS_HIP_CHECK(hipStreamSynchronize(c.st));
-    hipFree(tmp);
     return DNS_OK;
 }
 
@@ -676,8 +678,9 @@ int dns_cnn_loss_grad(dns_cnn *h, const float *x, const float *tgt, double *loss
 {
     if (!h || !x || !tgt) return DNS_EINVAL;
     CNN &c = *h->impl;
-    float *tmp = nullptr;
-    DNS_TRY(dalloc(&tmp, (size_t)c.V * 3));
+    DevTemp<float> tbuf;
+    DNS_TRY(tbuf.alloc((size_t)c.V * 3));
+    float *tmp = tbuf.p;
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, x, sizeof(float) * c.V * 3, hipMemcpyHostToDevice, c.st));
     DNS_HIP_CHECK(hipMemcpyAsync(c.t, tgt, sizeof(float) * c.V * 6, hipMemcpyHostToDevice, c.st));
     hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, c.st, tmp, 3, 1.0f, c.V, c.x0, 16);
@@ -686,7 +689,6 @@ int dns_cnn_loss_grad(dns_cnn *h, const float *x, const float *tgt, double *loss
     double l = 0;
     DNS_TRY(c.read_scalar(c.lval, &l));
     if (loss) *loss = l;
-    hipFree(tmp);
     return DNS_OK;
 }
 
@@ -694,13 +696,13 @@ int dns_cnn_add_sample(dns_cnn *h, const float *ubar, const float *tau)
 {
     if (!h || !ubar || !tau) return DNS_EINVAL;
     CNN &c = *h->impl;
-    float *tmp = nullptr;
-    DNS_TRY(dalloc(&tmp, (size_t)c.Vs * 9));
+    DevTemp<float> tbuf;
+    DNS_TRY(tbuf.alloc((size_t)c.Vs * 9));
+    float *tmp = tbuf.p;
     DNS_HIP_CHECK(hipMemcpyAsync(tmp, ubar, sizeof(float) * c.Vs * 3, hipMemcpyHostToDevice, c.st));
     DNS_HIP_CHECK(hipMemcpyAsync(tmp + 3 * c.Vs, tau, sizeof(float) * c.Vs * 6, hipMemcpyHostToDevice, c.st));
     DNS_TRY(c.append<float>(tmp, tmp + 3 * c.Vs, c.st));
     DNS_HIP_CHECK(hipStreamSynchronize(c.st));
-    hipFree(tmp);
     return DNS_OK;
 }
 

@@ -47,6 +47,24 @@ template <typename V> inline int dalloc(V **dptr, size_t count)
     return DNS_OK;
 }
 
+// A temporary device buffer released on every return path (a DNS_TRY or
+// DNS_HIP_CHECK returning early included).  The release synchronises the
+// device first, so work still queued on a stream never reads freed memory.
+template <typename V> struct DevTemp {
+    V *p = nullptr;
+    DevTemp() = default;
+    DevTemp(const DevTemp &) = delete;
+    DevTemp &operator=(const DevTemp &) = delete;
+    int alloc(size_t count) { return dalloc(&p, count); }
+    ~DevTemp()
+    {
+        if (p) {
+            hipDeviceSynchronize();
+            hipFree(p);
+        }
+    }
+};
+
 // Per-family HIP-event timing of kernel launches on the handle's stream.
 struct LaunchTimer {
     bool on = false;
