@@ -161,15 +161,17 @@ def run_les_closure(steps, warmup, M=64):
 
 
 def run_closure_experiment(n=512, M=64, nsnap=64, every=40, spinup_t=2.0, train_steps=2000, batch=16,
-                           nheld=8, window=0.5, nwin=5, dns_precision="mixed", cfl=0.5, log=print):
+                           nheld=8, window_tint=3.0, nwin=12, dns_precision="mixed", cfl=0.5, log=print):
     """Learned-closure experiment: forced HIT DNS (random IC k^4 exp(-2(k/4)^2),
     E = 1, seed 0, shells 1-2 clamped, nu = 2e-4) spun up to t = spinup_t,
     then nsnap SFS samples (Gaussian, filter width 2 l/M) every `every` steps
     on the M^3 grid; the CNN trains on the first nsnap - nheld (batch 16,
     Adam lr 1e-3, seed 0); a-priori score on the held-out samples; then M^3
     LES (FP64) with and without the closure from the filtered DNS at the
-    last sample, over `window` time units, against filtered-DNS spectra at
-    nwin equally spaced times."""
+    last sample, over `window_tint` integral times (t_int of the DNS at the
+    last sample), against filtered-DNS spectra at nwin equally spaced times:
+    per-time and time-averaged shell errors, over all shells and over the
+    unforced shells 3..kcut."""
     from distributednavierstokes_amd import closure as cl
     from distributednavierstokes_amd.cnn import SFSNet
     from distributednavierstokes_amd.spectral import DeviceState, _shells
@@ -198,6 +200,8 @@ def run_closure_experiment(n=512, M=64, nsnap=64, every=40, spinup_t=2.0, train_
     # LES initial condition: filtered DNS at the last sample
     ub0, _ = sam.spectral()
     # continue the DNS over the window, filtered spectra at the stop times
+    t_int = float(dns.spectral_stats(dev, s).t_int)
+    window = window_tint * t_int
     stops = [times[-1] + window * (q + 1) / nwin for q in range(nwin)]
     E_dns = []
 
@@ -236,13 +240,26 @@ def run_closure_experiment(n=512, M=64, nsnap=64, every=40, spinup_t=2.0, train_
     Ed, Enm, Ecl = np.mean(E_dns, axis=0), np.mean(E_nm, axis=0), np.mean(E_cl, axis=0)
     err_nm = np.abs(Enm / Ed - 1.0)
     err_cl = np.abs(Ecl / Ed - 1.0)
+    # per stop time over the unforced shells 3..kcut (index 2.. of shells 1..kcut)
+    per_time = {"t": stops,
+                "max_closure": [float(np.max(np.abs(np.asarray(c)[2:] / np.asarray(d)[2:] - 1))) for c, d in zip(E_cl, E_dns)],
+                "mean_closure": [float(np.mean(np.abs(np.asarray(c)[2:] / np.asarray(d)[2:] - 1))) for c, d in zip(E_cl, E_dns)],
+                "max_no_model": [float(np.max(np.abs(np.asarray(c)[2:] / np.asarray(d)[2:] - 1))) for c, d in zip(E_nm, E_dns)],
+                "mean_no_model": [float(np.mean(np.abs(np.asarray(c)[2:] / np.asarray(d)[2:] - 1))) for c, d in zip(E_nm, E_dns)]}
     log(f"LES done ({time.perf_counter() - t_start:.0f} s)")
     return {"workload": f"HIT {n}^3 DNS ({dns_precision}) -> {nsnap} SFS samples on {M}^3 (Gaussian, 2l/M) -> "
                         f"CNN {train_steps} Adam steps batch {batch} -> {M}^3 LES closure vs no model",
             "sample_times": [times[0], times[-1]], "scales": [su, stau],
             "train_seconds": train_s, "train_loss_first_last": [float(losses[0]), float(losses[-1])],
             "a_priori": {"held_out_mse_trained": held, "held_out_mse_no_model": nofit, "tau_correlation": corr},
-            "a_posteriori": {"window": [times[-1], stops[-1]], "times": nwin,
+            "a_posteriori": {"window": [times[-1], stops[-1]], "times": nwin, "t_int": t_int,
+                             "window_integral_times": window_tint,
+                             "unforced_shells": [3, len(Ed)],
+                             "unforced_rel_err_mean_closure": float(err_cl[2:].mean()),
+                             "unforced_rel_err_max_closure": float(err_cl[2:].max()),
+                             "unforced_rel_err_mean_no_model": float(err_nm[2:].mean()),
+                             "unforced_rel_err_max_no_model": float(err_nm[2:].max()),
+                             "per_time_unforced": per_time,
                              "spectrum_rel_err_max_closure": float(err_cl.max()),
                              "spectrum_rel_err_max_no_model": float(err_nm.max()),
                              "spectrum_rel_err_mean_closure": float(err_cl.mean()),
@@ -589,6 +606,12 @@ def closure_error_record():
             "spectrum_rel_err_max_closure": ap.get("spectrum_rel_err_max_closure"),
             "spectrum_rel_err_mean_no_model": ap.get("spectrum_rel_err_mean_no_model"),
             "spectrum_rel_err_max_no_model": ap.get("spectrum_rel_err_max_no_model"),
+            "window": ap.get("window"), "window_integral_times": ap.get("window_integral_times"),
+            "unforced_shells": ap.get("unforced_shells"),
+            "unforced_rel_err_mean_closure": ap.get("unforced_rel_err_mean_closure"),
+            "unforced_rel_err_max_closure": ap.get("unforced_rel_err_max_closure"),
+            "unforced_rel_err_mean_no_model": ap.get("unforced_rel_err_mean_no_model"),
+            "unforced_rel_err_max_no_model": ap.get("unforced_rel_err_max_no_model"),
             "tau_correlation": rec.get("a_priori", {}).get("tau_correlation")}
 
 
@@ -612,6 +635,8 @@ def main():
                     help="steps of the mixed-vs-FP64 comparison in the line (0: skip)")
     ap.add_argument("--spectral-n", type=int, default=0,
                     help="run only the pseudo-spectral HIT form at this n (kcut = n/3; --precision fp64|mixed)")
+    ap.add_argument("--cnn", action="store_true",
+                    help="run only the closure-CNN training step (batch 16 x 64^3, --steps Adam steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (scales the FETCH/WRITE counters)")
     ap.add_argument("--closure-experiment", action="store_true",
@@ -639,6 +664,9 @@ def main():
     if args.spectral_n:
         print(json.dumps(run_spectral(args.spectral_n, args.spectral_n // 3, args.precision, args.steps, args.warmup)),
               flush=True)
+        return
+    if args.cnn:
+        print(json.dumps(run_cnn_train(args.steps, args.warmup)), flush=True)
         return
     if args.copy_calibration:
         import torch
