@@ -144,6 +144,7 @@ struct CNN {
         mfma_k8 = env && !strcmp(env, "k8");
         const char *envc = getenv("DNS_CNN_CONV");
         row_conv = !(envc && !strcmp(envc, "direct"));
+        conv_pf = row_conv && !(envc && !strcmp(envc, "rows"));
         const int kin[4] = {3, 64, 64, 64}, nout[4] = {64, 64, 64, 6};
         size_t off = 0;
         for (int l = 0; l < 4; ++l) {
@@ -191,6 +192,10 @@ struct CNN {
         nsplit = (int)std::max<long>(1, std::min<long>(64, V / 4096));
         // the row-staged weight-gradient kernel splits the B·M² x-rows over at most 256 blocks
         row_wgrad = !mfma_k8 && M % 16 == 0 && M <= 256;
+        const char *envw = getenv("DNS_CNN_WGRAD");
+        rows_wgrad = row_wgrad && M <= 64 && envw && !strcmp(envw, "rows");
+        planes_wgrad = row_wgrad && M <= 64 && !(envw && (!strcmp(envw, "rows") || !strcmp(envw, "row16")));
+        if (const char *es = getenv("DNS_CNN_WSPLIT")) wgrad_splits = std::max(1, std::min(256, atoi(es)));
         const int nsplit16 = row_wgrad ? (int)std::min<long>(256, (long)B * M * M) : 0;
         DNS_TRY(dalloc(&wpart, (size_t)std::max(nsplit, nsplit16) * 27 * 64 * 64));
         DNS_TRY(dalloc(&bpart, (size_t)64 * bblocks));
@@ -217,7 +222,11 @@ struct CNN {
     // the K = 16 MFMA; DNS_CNN_MFMA=k8 selects the 32x32x8 kernel instead.
     bool mfma_k8 = false;
     bool row_wgrad = false; // k_conv_wgrad16 (M % 16 == 0) instead of k_conv_wgrad
+    bool rows_wgrad = false;   // k_conv_wgrad_rows (M % 16 == 0, M <= 64); DNS_CNN_WGRAD=rows
+    bool planes_wgrad = false; // k_conv_wgrad_planes (M % 16 == 0, M <= 64), the default
+    int wgrad_splits = 128;    // plane splits of k_conv_wgrad_planes (DNS_CNN_WSPLIT, sweep)
     bool row_conv = false;  // k_conv_rows (M = 32 or 64) instead of k_conv16; DNS_CNN_CONV=direct disables
+    bool conv_pf = false;   // k_conv_rows_pf at M = 64 (kin 16 or 64); DNS_CNN_CONV=rows selects k_conv_rows
     template <int EPI> void conv(const ConvArgs &ca, int np, long Vn, hipStream_t s)
     {
         if (mfma_k8) {
@@ -233,7 +242,15 @@ struct CNN {
         if (row_conv && (M == 32 || M == 64) && ca.kin % 16 == 0 && (np == 32 || np == 64)) {
             const dim3 grd((unsigned)(M / 4), (unsigned)M, (unsigned)(Vn / Vs));
             const size_t lds = sizeof(unsigned short) * 6 * (size_t)M * (ca.kin + 8);
-            if (M == 64) {
+            if (conv_pf && M == 64 && (ca.kin == 16 || ca.kin == 64)) {
+                if (ca.kin == 64) {
+                    if (np == 64) hipLaunchKernelGGL((k_conv_rows_pf<2, 2, 64, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+                    else hipLaunchKernelGGL((k_conv_rows_pf<2, 1, 64, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+                } else {
+                    if (np == 64) hipLaunchKernelGGL((k_conv_rows_pf<2, 2, 16, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+                    else hipLaunchKernelGGL((k_conv_rows_pf<2, 1, 16, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
+                }
+            } else if (M == 64) {
                 if (np == 64) hipLaunchKernelGGL((k_conv_rows<2, 2, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
                 else hipLaunchKernelGGL((k_conv_rows<2, 1, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
             } else {
@@ -293,7 +310,58 @@ struct CNN {
         wa.np = c.np;
         wa.kp = c.kpt;
         int ns = nsplit;
-        if (row_wgrad) {
+        bool bias_done = false; // the plane kernel also forms the bias partials
+        if (planes_wgrad && (c.np == 32 || c.np == 64) && (c.kpt == 32 || c.kpt == 64)) {
+            bias_done = true;
+            WGradPlaneArgs wp;
+            wp.dout = dout;
+            wp.in = wa.in;
+            wp.part = wpart;
+            wp.bpart = bpart;
+            wp.nout = c.nout;
+            wp.ldd = ldd;
+            wp.kin = c.kinp;
+            const long nplanes = (long)B * M;
+            wp.planes_per_block = (int)((nplanes + wgrad_splits - 1) / wgrad_splits);
+            ns = (int)((nplanes + wp.planes_per_block - 1) / wp.planes_per_block);
+            wp.lds_d = wgrad_row_stride(M);
+            wp.lds_i = wgrad_row_stride(M + 16);
+            const int NC = (c.np / 32) * (c.kpt / 32);
+            const size_t stage = sizeof(unsigned short) * ((size_t)c.np * wp.lds_d + (size_t)3 * c.kpt * wp.lds_i);
+            const size_t lds = std::max(stage, NC < 4 ? sizeof(float) * 1024 * 9 * (size_t)NC : (size_t)0);
+            const dim3 grd(ns, 3);
+            if (c.np == 64 && c.kpt == 64)
+                hipLaunchKernelGGL((k_conv_wgrad_planes<2, 2>), grd, dim3(256), lds, st, wp, geom(V));
+            else if (c.np == 64)
+                hipLaunchKernelGGL((k_conv_wgrad_planes<2, 1>), grd, dim3(256), lds, st, wp, geom(V));
+            else if (c.kpt == 64)
+                hipLaunchKernelGGL((k_conv_wgrad_planes<1, 2>), grd, dim3(256), lds, st, wp, geom(V));
+            else
+                hipLaunchKernelGGL((k_conv_wgrad_planes<1, 1>), grd, dim3(256), lds, st, wp, geom(V));
+        } else if (rows_wgrad && (c.np == 32 || c.np == 64) && (c.kpt == 32 || c.kpt == 64)) {
+            WGradRowArgs wr;
+            wr.dout = dout;
+            wr.in = wa.in;
+            wr.part = wpart;
+            wr.ldd = ldd;
+            wr.kin = c.kinp;
+            const long nrows = (long)B * M * M;
+            wr.rows_per_block = (int)((nrows + 255) / 256);
+            ns = (int)((nrows + wr.rows_per_block - 1) / wr.rows_per_block);
+            wr.lds_d = wgrad_row_stride(M);
+            wr.lds_i = wgrad_row_stride(M + 16);
+            const size_t stage = sizeof(unsigned short) * ((size_t)c.np * wr.lds_d + (size_t)c.kpt * wr.lds_i);
+            const size_t lds = std::max(stage, sizeof(float) * 1024 * (size_t)(c.np / 32) * (c.kpt / 32) * 3);
+            const dim3 grd(ns, 9);
+            if (c.np == 64 && c.kpt == 64)
+                hipLaunchKernelGGL((k_conv_wgrad_rows<2, 2>), grd, dim3(256), lds, st, wr, geom(V));
+            else if (c.np == 64)
+                hipLaunchKernelGGL((k_conv_wgrad_rows<2, 1>), grd, dim3(256), lds, st, wr, geom(V));
+            else if (c.kpt == 64)
+                hipLaunchKernelGGL((k_conv_wgrad_rows<1, 2>), grd, dim3(256), lds, st, wr, geom(V));
+            else
+                hipLaunchKernelGGL((k_conv_wgrad_rows<1, 1>), grd, dim3(256), lds, st, wr, geom(V));
+        } else if (row_wgrad) {
             WGrad16Args w16;
             w16.dout = dout;
             w16.in = wa.in;
@@ -319,8 +387,12 @@ struct CNN {
             set_error("bias gradient: channel stride %d is not a multiple of 8 dividing 2048", ldd);
             return DNS_EINVAL;
         }
-        hipLaunchKernelGGL(k_bias_grad, dim3(bblocks), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);
-        hipLaunchKernelGGL(k_bias_reduce, dim3(1), dim3(64), 0, st, bpart, bblocks, c.nout, g + c.boff);
+        if (bias_done) {
+            hipLaunchKernelGGL(k_bias_reduce, dim3(c.nout), dim3(256), 0, st, bpart, ns, c.nout, g + c.boff);
+        } else {
+            hipLaunchKernelGGL(k_bias_grad, dim3(bblocks), dim3(256), 0, st, dout, ldd, c.nout, V, bpart);
+            hipLaunchKernelGGL(k_bias_reduce, dim3(c.nout), dim3(256), 0, st, bpart, bblocks, c.nout, g + c.boff);
+        }
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
     }

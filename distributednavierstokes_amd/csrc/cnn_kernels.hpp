@@ -20,6 +20,10 @@ namespace dns {
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+// 16-byte register vector for staged loads: a native vector type, so arrays
+// of them stay in registers (HIP's uint4 is a struct whose copies are
+// memcpys, which keeps such arrays in scratch memory)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline unsigned short f2bf(float f)
 {
@@ -307,6 +311,126 @@ __global__ void __launch_bounds__(256) k_conv_rows(ConvArgs a, ConvGeom g)
         }
 }
 
+// k_conv_rows with the input channel count KIN (16 or 64) fixed at compile
+// time, the weight fragments of the next offset loaded into registers while
+// the current offset's MFMAs run (the 27 offsets unrolled, two register sets
+// used in turn; a third, two offsets ahead, measured no faster) and the next
+// dz plane's input rows loaded into registers while the current plane is
+// computed, so the L2 latency of the weight stream and the HBM latency of
+// the rows overlap the arithmetic.
+template <int MB, int NB, int KIN, int EPI>
+__global__ void __launch_bounds__(256) k_conv_rows_pf(ConvArgs a, ConvGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    unsigned short *sa = reinterpret_cast<unsigned short *>(smem_raw);
+    constexpr int M = 32 * MB, KS = KIN + 8, KST = KIN / 16, KC8 = KIN / 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, h = lane >> 5;
+    const int y0 = blockIdx.x * 4, z = blockIdx.y, b = blockIdx.z;
+    f32x16 acc[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[mb][nb][r] = 0.0f;
+    // kernel arguments as locals: the lambdas below capture by reference,
+    // and a captured by-value argument struct would be kept in scratch
+    const unsigned short *const gin = a.in;
+    const int sign = a.sign, np = a.np;
+    u32x4 bf[2][NB][KST];
+    const unsigned short *wrow = a.W + (size_t)row * KIN + 8 * h;
+    auto loadb = [&](int o, u32x4 (&dst)[NB][KST]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int ks = 0; ks < KST; ++ks)
+                dst[nb][ks] = *reinterpret_cast<const u32x4 *>(wrow + ((size_t)o * np + 32 * nb) * KIN + 16 * ks);
+    };
+    // input rows y0-1 .. y0+4 of plane z + sign·dz: NIT 16-byte pieces per
+    // thread, loaded for the next dz while the current one is computed
+    constexpr int NPC = 6 * M * KC8, NIT = (NPC + 255) / 256;
+    u32x4 st[NIT];
+    auto fetch_rows = [&](int dzi) __attribute__((always_inline)) {
+        const int zs = (z + sign * (dzi - 1) + M) % M;
+#pragma unroll
+        for (int i = 0; i < NIT; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (NPC % 256 == 0 || e < NPC) {
+                const int c8 = e % KC8, t = e / KC8, x = t % M, rr = t / M;
+                const int ys = (y0 - 1 + rr + M) % M;
+                const long v = x + (long)M * (ys + (long)M * (zs + (long)M * b));
+                st[i] = *reinterpret_cast<const u32x4 *>(gin + v * KIN + 8 * c8);
+            }
+        }
+    };
+    loadb(0, bf[0]);
+    fetch_rows(0);
+#pragma unroll
+    for (int dzi = 0; dzi < 3; ++dzi) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NIT; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (NPC % 256 == 0 || e < NPC) {
+                const int c8 = e % KC8, t = e / KC8, x = t % M, rr = t / M;
+                *reinterpret_cast<u32x4 *>(sa + (rr * M + x) * KS + 8 * c8) = st[i];
+            }
+        }
+        __syncthreads();
+        if (dzi < 2) fetch_rows(dzi + 1);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int o = q + 9 * dzi, cur = o & 1;
+            if (o + 1 < 27) loadb(o + 1, bf[cur ^ 1]);
+            const int dyi = q / 3, dxi = q % 3;
+            const int rr = wave + 1 + a.sign * (dyi - 1);
+            const int dx = a.sign * (dxi - 1);
+            const unsigned short *ap[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+                const int xs = (32 * mb + row + dx + M) % M;
+                ap[mb] = sa + (rr * M + xs) * KS + 8 * h;
+            }
+#pragma unroll
+            for (int ks = 0; ks < KST; ++ks) {
+                bf16x8 av[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) av[mb] = *reinterpret_cast<const bf16x8 *>(ap[mb] + 16 * ks);
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            av[mb], __builtin_bit_cast(bf16x8, bf[cur][nb][ks]), acc[mb][nb], 0, 0, 0);
+            }
+        }
+    }
+    const long vrow = (long)M * ((y0 + wave) + (long)M * (z + (long)M * b));
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int col = 32 * nb + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long vv = vrow + 32 * mb + 8 * (r >> 2) + 4 * h + (r & 3);
+                if (col >= a.nout) continue;
+                const float c = acc[mb][nb][r];
+                if (EPI == 0) {
+                    const float zz = c + a.bias[col];
+                    a.zbuf[vv * a.ldo + col] = f2bf(zz);
+                    a.out[vv * a.ldo + col] = f2bf(gelu_f(zz));
+                } else if (EPI == 1) {
+                    a.yout[vv * a.ldo + col] = c + a.bias[col];
+                } else {
+                    const float zp = bf2f(a.zprev[vv * a.ldo + col]);
+                    a.out[vv * a.ldo + col] = f2bf(c * gelu_d(zp));
+                }
+            }
+        }
+}
+
 // Weight gradient of one offset per block-row: for o = blockIdx.y,
 // dW[o][n][k] += Σ_{v in chunk} dout[v][n] · in[v + d_o][k], tiles of 64
 // voxels staged (transposed) through LDS.  Each block covers a range of
@@ -381,9 +505,9 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(WGradArgs a, ConvGeom g)
 // Halves q of four 16-byte vectors (8 BF16 channels of 4 voxels) packed
 // into one 8-byte word in voxel order: the 4×8 → 8×4 register transpose of
 // the weight-gradient staging.
-__device__ inline uint2 pack4_bf16(const uint4 (&v)[4], int q)
+__device__ inline uint2 pack4_bf16(const u32x4 (&v)[4], int q)
 {
-    auto half = [&](const uint4 &w) -> unsigned int {
+    auto half = [&](const u32x4 &w) -> unsigned int {
         const unsigned int word = (q >> 1) == 0 ? w.x : (q >> 1) == 1 ? w.y : (q >> 1) == 2 ? w.z : w.w;
         return (q & 1) ? (word >> 16) : (word & 0xffffu);
     };
@@ -418,7 +542,7 @@ __global__ void __launch_bounds__(256) k_conv_wgrad16(WGrad16Args a, ConvGeom g)
     const long nrows = (long)g.B * M * M;
     const long r0 = (long)blockIdx.x * a.rows_per_block;
     const long r1 = min(nrows, r0 + a.rows_per_block);
-    const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
     for (long r = r0; r < r1; ++r) {
         const int y = (int)(r % M);
         const long t = r / M;
@@ -432,10 +556,10 @@ __global__ void __launch_bounds__(256) k_conv_wgrad16(WGrad16Args a, ConvGeom g)
         // written transposed as eight 8-byte LDS stores
         for (int e = threadIdx.x; e < (M / 4) * nc8; e += blockDim.x) {
             const int xg = e / nc8, c8 = (e - xg * nc8) * 8, x = 4 * xg;
-            uint4 v[4];
+            u32x4 v[4];
 #pragma unroll
             for (int vi = 0; vi < 4; ++vi)
-                v[vi] = c8 + 8 <= a.ldd ? *reinterpret_cast<const uint4 *>(a.dout + (vrow + x + vi) * a.ldd + c8) : zero4;
+                v[vi] = c8 + 8 <= a.ldd ? *reinterpret_cast<const u32x4 *>(a.dout + (vrow + x + vi) * a.ldd + c8) : zero4;
 #pragma unroll
             for (int q = 0; q < 8; ++q)
                 *reinterpret_cast<uint2 *>(sd + (c8 + q) * LD + x) = pack4_bf16(v, q);
@@ -443,12 +567,12 @@ __global__ void __launch_bounds__(256) k_conv_wgrad16(WGrad16Args a, ConvGeom g)
         for (int e = threadIdx.x; e < 3 * (M / 4) * kc8; e += blockDim.x) {
             const int dxi = e / ((M / 4) * kc8), f = e - dxi * ((M / 4) * kc8);
             const int xg = f / kc8, c8 = (f - xg * kc8) * 8, x = 4 * xg;
-            uint4 v[4];
+            u32x4 v[4];
 #pragma unroll
             for (int vi = 0; vi < 4; ++vi) {
                 int xs = x + vi + dxi - 1; // si[dx][k][x] = in[(x + dx) mod M][k]
                 xs = xs < 0 ? xs + M : (xs >= M ? xs - M : xs);
-                v[vi] = c8 + 8 <= a.kin ? *reinterpret_cast<const uint4 *>(a.in + (vsrc + xs) * a.kin + c8) : zero4;
+                v[vi] = c8 + 8 <= a.kin ? *reinterpret_cast<const u32x4 *>(a.in + (vsrc + xs) * a.kin + c8) : zero4;
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q)
@@ -484,6 +608,394 @@ __global__ void __launch_bounds__(256) k_conv_wgrad16(WGrad16Args a, ConvGeom g)
     }
 }
 
+// Weight gradient by x-rows with one staged input row for all three dx
+// offsets.  A block owns a range of rows (y, z, b) and one (dy, dz) pair
+// (blockIdx.y); per row it stages the output gradient transposed,
+// sd[n][x], and the input row at (y + dy, z + dz) transposed with an
+// 8-voxel periodic halo on each side, si[k][x + 8].  Wave w takes the
+// 16-voxel reduction slices x0 = 16 s (s ≡ w mod 4) and accumulates every
+// (n-tile, k-tile, dx) tile of the block, NT·KT·3 ≤ 12: each slice reads one
+// A fragment per n-tile and, per k-tile, the aligned 16-byte chunk at x0 plus
+// the neighbouring dwords, from which the dx = ±1 operands are built by
+// 16-bit funnel shifts (the reduction index pairs dout[x] with in[x + dx]).
+// The global loads of the next row are issued into registers before the
+// current row's MFMAs, so their latency overlaps the arithmetic.  Staging
+// writes 8 voxels × 8 channels per thread as eight 16-byte transposed rows;
+// row strides are ≡ 4 dwords (mod 8), which keeps both the transposed
+// writes (lanes along x) and the fragment reads (lanes along channels) free
+// of bank conflicts.  At the end the waves' tiles are summed in LDS in wave
+// order (deterministic) and written as part[blockIdx.x][o][n][k].
+// Needs M % 16 == 0, M <= 64, NT = np/32 and KT = kp/32 in {1, 2}, ldd and
+// kin multiples of 8.
+struct WGradRowArgs {
+    const unsigned short *dout; // [V][ldd]
+    const unsigned short *in;   // [V][kin]
+    float *part;
+    int ldd, kin;
+    int rows_per_block;
+    int lds_d, lds_i; // row strides of sd and si in halves
+};
+
+__host__ __device__ constexpr int wgrad_row_stride(int halves)
+{
+    // dwords rounded up to ≡ 4 (mod 8), returned in halves
+    return 2 * ((halves / 2 + 3) / 8 * 8 + 4);
+}
+
+__device__ inline unsigned int bf_pair(unsigned int a, unsigned int b, int q)
+{
+    // element q (0..1 inside the dword) of a and of b as (a_q | b_q << 16)
+    return q ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
+}
+
+template <int NT, int KT>
+__global__ void __launch_bounds__(256) k_conv_wgrad_rows(WGradRowArgs a, ConvGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    unsigned short *lds = reinterpret_cast<unsigned short *>(smem_raw);
+    constexpr int NP = 32 * NT, KP = 32 * KT, NTILE = NT * KT * 3;
+    const int M = g.M, LDD = a.lds_d, LDI = a.lds_i;
+    unsigned short *sd = lds;            // [NP][LDD]
+    unsigned short *si = lds + NP * LDD; // [KP][LDI], column x + 8
+    const int grp = blockIdx.y, dy = grp % 3 - 1, dz = grp / 3 - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, h = lane >> 5;
+    const int MG = M / 8;                        // 8-voxel groups per row
+    const int nd = MG * (NP / 8);                // staging items of dout
+    const int ni = (MG + 2) * (KP / 8);          // staging items of the input (with halo)
+    const long nrows = (long)g.B * M * M;
+    const long r0 = (long)blockIdx.x * a.rows_per_block;
+    const long r1 = min(nrows, r0 + a.rows_per_block);
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+
+    f32x16 acc[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+
+    // this thread's staging item (at most one of the 2 kinds; nd + ni <= 256)
+    const int e = threadIdx.x;
+    const bool isd = e < nd, isi = !isd && e < nd + ni;
+    int xg = 0, c8 = 0;
+    if (isd) {
+        xg = e % MG;
+        c8 = 8 * (e / MG);
+    } else if (isi) {
+        xg = (e - nd) % (MG + 2);
+        c8 = 8 * ((e - nd) / (MG + 2));
+    }
+    u32x4 pf[8];
+    auto fetch = [&](long r) __attribute__((always_inline)) {
+        const int y = (int)(r % M);
+        const long t = r / M;
+        const int z = (int)(t % M);
+        const long b = t / M;
+        if (isd) {
+            const long v0 = r * M + 8 * xg;
+            const bool ok = c8 + 8 <= a.ldd;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                pf[i] = ok ? *reinterpret_cast<const u32x4 *>(a.dout + (v0 + i) * a.ldd + c8) : zero4;
+        } else if (isi) {
+            const int ys = y + dy < 0 ? y + dy + M : (y + dy >= M ? y + dy - M : y + dy);
+            const int zs = z + dz < 0 ? z + dz + M : (z + dz >= M ? z + dz - M : z + dz);
+            const long vrow = (long)M * (ys + (long)M * (zs + (long)M * b));
+            int xs = 8 * xg - 8; // halo groups wrap periodically
+            xs = xs < 0 ? xs + M : (xs >= M ? xs - M : xs);
+            const bool ok = c8 + 8 <= a.kin;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                pf[i] = ok ? *reinterpret_cast<const u32x4 *>(a.in + (vrow + xs + i) * a.kin + c8) : zero4;
+        }
+    };
+    auto stage = [&]() __attribute__((always_inline)) {
+        if (!isd && !isi) return;
+        unsigned short *dst = isd ? sd + c8 * LDD + 8 * xg : si + c8 * LDI + 8 * xg;
+        const int ld = isd ? LDD : LDI;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int wq = q >> 1, hq = q & 1;
+            auto w = [&](const u32x4 &v) { return wq == 0 ? v.x : wq == 1 ? v.y : wq == 2 ? v.z : v.w; };
+            u32x4 o;
+            o.x = bf_pair(w(pf[0]), w(pf[1]), hq);
+            o.y = bf_pair(w(pf[2]), w(pf[3]), hq);
+            o.z = bf_pair(w(pf[4]), w(pf[5]), hq);
+            o.w = bf_pair(w(pf[6]), w(pf[7]), hq);
+            *reinterpret_cast<u32x4 *>(dst + q * ld) = o;
+        }
+    };
+
+    if (r0 < r1) fetch(r0);
+    for (long r = r0; r < r1; ++r) {
+        __syncthreads(); // the previous row's fragments are read
+        stage();
+        __syncthreads();
+        if (r + 1 < r1) fetch(r + 1);
+        for (int s = wave; s < M / 16; s += 4) {
+            const int x0 = 16 * s;
+            bf16x8 av[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                av[nt] = *reinterpret_cast<const bf16x8 *>(sd + (32 * nt + row) * LDD + x0 + 8 * h);
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) {
+                const unsigned short *pk = si + (32 * kt + row) * LDI + 8 + x0 + 8 * h;
+                const u32x4 c = *reinterpret_cast<const u32x4 *>(pk);
+                const unsigned int prev = *reinterpret_cast<const unsigned int *>(pk - 2); // x - 2, x - 1
+                const unsigned int next = *reinterpret_cast<const unsigned int *>(pk + 8); // x + 8, x + 9
+                const unsigned int s01 = __builtin_amdgcn_alignbit(c.y, c.x, 16);
+                const unsigned int s12 = __builtin_amdgcn_alignbit(c.z, c.y, 16);
+                const unsigned int s23 = __builtin_amdgcn_alignbit(c.w, c.z, 16);
+                const u32x4 bm = {__builtin_amdgcn_alignbit(c.x, prev, 16), s01, s12, s23}; // in[x − 1]
+                const u32x4 bp = {s01, s12, s23, __builtin_amdgcn_alignbit(next, c.w, 16)}; // in[x + 1]
+                const bf16x8 bv[3] = {__builtin_bit_cast(bf16x8, bm), __builtin_bit_cast(bf16x8, c),
+                                      __builtin_bit_cast(bf16x8, bp)};
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int dxi = 0; dxi < 3; ++dxi) {
+                        f32x16 &ac = acc[(nt * KT + kt) * 3 + dxi];
+                        ac = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[nt], bv[dxi], ac, 0, 0, 0);
+                    }
+            }
+        }
+    }
+    // ordered sum of the waves' tiles in LDS, then the block's partials
+    float *red = reinterpret_cast<float *>(smem_raw);
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int idx = t * 1024 + (8 * (r >> 2) + 4 * h + (r & 3)) * 32 + row;
+                    red[idx] = w == 0 ? acc[t][r] : red[idx] + acc[t][r];
+                }
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < NTILE * 1024; idx += blockDim.x) {
+        const int t = idx >> 10, i = (idx >> 5) & 31, j = idx & 31;
+        const int dxi = t % 3, kt = (t / 3) % KT, nt = t / (3 * KT);
+        const int o = dxi + 3 * grp;
+        a.part[(((size_t)blockIdx.x * 27 + o) * NP + 32 * nt + i) * KP + 32 * kt + j] = red[idx];
+    }
+}
+
+// Weight gradient by (z, b) planes, one dz per block (blockIdx.y): the block
+// walks the x-rows y = 0..M-1 of its planes and accumulates the nine (dy, dx)
+// offsets of its dz at once, so each output-gradient row is staged once per
+// dz (three times in all, not nine) and each input row once per plane: the
+// input rows y−1, y, y+1 of plane z + dz sit in a ring of three LDS slots and
+// advancing y stages only row y+2 (slot (y' + 1) mod 3 holds row y').  Each
+// unit of work stages at most one 8 × 8 item per thread (the output-gradient
+// row and the next input row, or the first two input rows of a plane), whose
+// global loads are issued into registers before the previous unit's MFMAs.
+// Wave w takes the (n-tile, k-tile) pair w mod NT·KT and the 16-voxel
+// reduction slices s ≡ w / (NT·KT) (mod 4 / (NT·KT)), accumulating its nine
+// (dy, dx) tiles; waves that share a pair are summed in LDS in wave order.
+// Operands, halo and funnel shifts as k_conv_wgrad_rows.  Partials
+// part[blockIdx.x][o][n][k] for the block's nine offsets.
+// Needs M % 16 == 0, M <= 64, NT = np/32 and KT = kp/32 in {1, 2}.
+struct WGradPlaneArgs {
+    const unsigned short *dout; // [V][ldd]
+    const unsigned short *in;   // [V][kin]
+    float *part;
+    float *bpart; // bias-gradient partials bpart[c · gridDim.x + blockIdx.x] (dz = 0 blocks), or null
+    int nout;
+    int ldd, kin;
+    int planes_per_block;
+    int lds_d, lds_i; // row strides of sd and si in halves
+};
+
+template <int NT, int KT>
+__global__ void __launch_bounds__(256, 2) k_conv_wgrad_planes(WGradPlaneArgs a, ConvGeom g)
+{
+    extern __shared__ unsigned char smem_raw[];
+    unsigned short *lds = reinterpret_cast<unsigned short *>(smem_raw);
+    constexpr int NP = 32 * NT, KP = 32 * KT, NC = NT * KT, WPC = 4 / NC;
+    const int M = g.M, LDD = a.lds_d, LDI = a.lds_i;
+    unsigned short *sd = lds;            // [NP][LDD]
+    unsigned short *si = lds + NP * LDD; // [3][KP][LDI], column x + 8
+    const int dzi = blockIdx.y, dz = dzi - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, h = lane >> 5;
+    const int combo = wave % NC, sg = wave / NC;
+    const int nt = combo % NT, kt = combo / NT;
+    const int MG = M / 8;
+    const int nd = MG * (NP / 8), ni = (MG + 2) * (KP / 8);
+    const long nplanes = (long)g.B * M;
+    const long p0 = (long)blockIdx.x * a.planes_per_block;
+    const long p1 = min(nplanes, p0 + a.planes_per_block);
+    const long nunits = (p1 - p0) * (M + 1); // per plane: a start unit, then rows 0..M-1
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+
+    // what this thread stages in unit u: kind 0 none, 1 dout row, 2 input row
+    u32x4 pf[8];
+    int kind = 0, xg = 0, c8 = 0, slot = 0;
+    auto fetch = [&](long u) __attribute__((always_inline)) {
+        const long p = p0 + u / (M + 1);
+        const int yu = (int)(u % (M + 1)) - 1; // -1: plane start
+        const int z = (int)(p % M);
+        const long b = p / M;
+        const int e = threadIdx.x;
+        int yin = 0;
+        kind = 0;
+        if (yu < 0) {
+            if (e < 2 * ni) {
+                kind = 2;
+                const int q = e / ni, f = e - q * ni;
+                yin = q - 1; // rows -1 and 0
+                xg = f % (MG + 2);
+                c8 = 8 * (f / (MG + 2));
+            }
+        } else if (e < nd) {
+            kind = 1;
+            xg = e % MG;
+            c8 = 8 * (e / MG);
+            const long v0 = (long)M * (yu + (long)M * p) + 8 * xg;
+            const bool ok = c8 + 8 <= a.ldd;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                pf[i] = ok ? *reinterpret_cast<const u32x4 *>(a.dout + (v0 + i) * a.ldd + c8) : zero4;
+        } else if (e < nd + ni) {
+            kind = 2;
+            yin = yu + 1;
+            const int f = e - nd;
+            xg = f % (MG + 2);
+            c8 = 8 * (f / (MG + 2));
+        }
+        if (kind == 2) {
+            slot = (yin + 1) % 3;
+            const int ys = yin < 0 ? yin + M : (yin >= M ? yin - M : yin);
+            const int zs = z + dz < 0 ? z + dz + M : (z + dz >= M ? z + dz - M : z + dz);
+            const long vrow = (long)M * (ys + (long)M * (zs + (long)M * b));
+            int xs = 8 * xg - 8;
+            xs = xs < 0 ? xs + M : (xs >= M ? xs - M : xs);
+            const bool ok = c8 + 8 <= a.kin;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                pf[i] = ok ? *reinterpret_cast<const u32x4 *>(a.in + (vrow + xs + i) * a.kin + c8) : zero4;
+        }
+    };
+    // bias gradient (Σ_v dout[v][c]) from the output-gradient rows the dz = 0
+    // blocks stage anyway: 8 channels per staging thread, FP32 per block
+    float bsum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const bool dobias = a.bpart != nullptr && dz == 0;
+    auto stage = [&]() __attribute__((always_inline)) {
+        if (kind == 0) return;
+        if (kind == 1 && dobias) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const unsigned int wd = q >> 1 == 0 ? pf[i].x : q >> 1 == 1 ? pf[i].y : q >> 1 == 2 ? pf[i].z : pf[i].w;
+                    bsum[q] += bf2f((unsigned short)((q & 1) ? (wd >> 16) : (wd & 0xffffu)));
+                }
+        }
+        unsigned short *dst = kind == 1 ? sd + c8 * LDD + 8 * xg : si + (slot * KP + c8) * LDI + 8 * xg;
+        const int ld = kind == 1 ? LDD : LDI;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int wq = q >> 1, hq = q & 1;
+            auto w = [&](const u32x4 &v) { return wq == 0 ? v.x : wq == 1 ? v.y : wq == 2 ? v.z : v.w; };
+            u32x4 o;
+            o.x = bf_pair(w(pf[0]), w(pf[1]), hq);
+            o.y = bf_pair(w(pf[2]), w(pf[3]), hq);
+            o.z = bf_pair(w(pf[4]), w(pf[5]), hq);
+            o.w = bf_pair(w(pf[6]), w(pf[7]), hq);
+            *reinterpret_cast<u32x4 *>(dst + q * ld) = o;
+        }
+    };
+
+    if (nunits > 0) fetch(0);
+    for (long u = 0; u < nunits; ++u) {
+        __syncthreads(); // the previous unit's fragments are read
+        stage();
+        __syncthreads();
+        if (u + 1 < nunits) fetch(u + 1);
+        const int y = (int)(u % (M + 1)) - 1;
+        if (y < 0) continue;
+        for (int s = sg; s < M / 16; s += WPC) {
+            const int x0 = 16 * s;
+            const bf16x8 av = *reinterpret_cast<const bf16x8 *>(sd + (32 * nt + row) * LDD + x0 + 8 * h);
+#pragma unroll
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                const int sl = (y + dyi) % 3; // input row y + dyi − 1
+                const unsigned short *pk = si + (sl * KP + 32 * kt + row) * LDI + 8 + x0 + 8 * h;
+                const u32x4 c = *reinterpret_cast<const u32x4 *>(pk);
+                const unsigned int prev = *reinterpret_cast<const unsigned int *>(pk - 2);
+                const unsigned int next = *reinterpret_cast<const unsigned int *>(pk + 8);
+                const unsigned int s01 = __builtin_amdgcn_alignbit(c.y, c.x, 16);
+                const unsigned int s12 = __builtin_amdgcn_alignbit(c.z, c.y, 16);
+                const unsigned int s23 = __builtin_amdgcn_alignbit(c.w, c.z, 16);
+                const u32x4 bm = {__builtin_amdgcn_alignbit(c.x, prev, 16), s01, s12, s23};
+                const u32x4 bp = {s01, s12, s23, __builtin_amdgcn_alignbit(next, c.w, 16)};
+                acc[3 * dyi + 0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bm),
+                                                                           acc[3 * dyi + 0], 0, 0, 0);
+                acc[3 * dyi + 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, c),
+                                                                           acc[3 * dyi + 1], 0, 0, 0);
+                acc[3 * dyi + 2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, __builtin_bit_cast(bf16x8, bp),
+                                                                           acc[3 * dyi + 2], 0, 0, 0);
+            }
+        }
+    }
+    // waves sharing an (n, k) tile pair are summed in LDS in wave order
+    float *red = reinterpret_cast<float *>(smem_raw);
+    if (dobias) {
+        // the block's bias partials: the dout-staging threads (x-group
+        // fastest) hold 8 channels each; summed over the x-groups in order
+        __syncthreads();
+        if (threadIdx.x < nd)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) red[threadIdx.x * 8 + q] = bsum[q];
+        __syncthreads();
+        if (threadIdx.x < NP) {
+            const int c = threadIdx.x, cg = c / 8, cq = c % 8;
+            float t = 0.f;
+            for (int x = 0; x < MG; ++x) t += red[(x + MG * cg) * 8 + cq];
+            if (c < a.nout) a.bpart[(size_t)c * gridDim.x + blockIdx.x] = t;
+        }
+    }
+    const size_t pbase = (size_t)blockIdx.x * 27 + 9 * dzi;
+    if (WPC == 1) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 8 * (r >> 2) + 4 * h + (r & 3);
+                a.part[((pbase + t) * NP + 32 * nt + i) * KP + 32 * kt + row] = acc[t][r];
+            }
+        return;
+    }
+    __syncthreads();
+    for (int q = 0; q < WPC; ++q) {
+        if (sg == q) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int idx = (combo * 9 + t) * 1024 + (8 * (r >> 2) + 4 * h + (r & 3)) * 32 + row;
+                    red[idx] = q == 0 ? acc[t][r] : red[idx] + acc[t][r];
+                }
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < NC * 9 * 1024; idx += blockDim.x) {
+        const int cb = idx / (9 * 1024), t = (idx >> 10) % 9, i = (idx >> 5) & 31, j = idx & 31;
+        const int ntc = cb % NT, ktc = cb / NT;
+        a.part[((pbase + t) * NP + 32 * ntc + i) * KP + 32 * ktc + j] = red[idx];
+    }
+}
+
 // Sums the partials over the voxel splits into the FP32 gradient
 // grad[o][n][k] (n < nout, k < kin) of the unpadded layout.
 __global__ void k_wgrad_reduce(const float *part, int nsplit, int np, int kp, int nout, int kin, float *grad)
@@ -491,9 +1003,15 @@ __global__ void k_wgrad_reduce(const float *part, int nsplit, int np, int kp, in
     const int total = 27 * nout * kin;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         const int k = t % kin, n = (t / kin) % nout, o = t / (kin * nout);
-        double s = 0;
-        for (int q = 0; q < nsplit; ++q) s += part[(((size_t)q * 27 + o) * np + n) * kp + k];
-        grad[t] = (float)s;
+        // four independent chains keep four loads in flight; fixed order
+        const size_t stride = (size_t)27 * np * kp, base = ((size_t)o * np + n) * kp + k;
+        double s4[4] = {0, 0, 0, 0};
+        int q = 0;
+        for (; q + 4 <= nsplit; q += 4)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s4[e] += part[base + (size_t)(q + e) * stride];
+        for (; q < nsplit; ++q) s4[0] += part[base + (size_t)q * stride];
+        grad[t] = (float)((s4[0] + s4[1]) + (s4[2] + s4[3]));
     }
 }
 
@@ -533,13 +1051,21 @@ __global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *dout, i
     }
 }
 
-__global__ void k_bias_reduce(const float *part, int nblocks, int nout, float *grad)
+// One block of 256 threads per channel (grid nout): strided FP64 partial
+// sums, then a tree in LDS (fixed order).
+__global__ void __launch_bounds__(256) k_bias_reduce(const float *part, int nblocks, int nout, float *grad)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nout) return;
-    double s = 0;
-    for (int q = 0; q < nblocks; ++q) s += part[(size_t)c * nblocks + q];
-    grad[c] = (float)s;
+    __shared__ double s[256];
+    const int c = blockIdx.x;
+    double acc = 0;
+    for (int q = threadIdx.x; q < nblocks; q += blockDim.x) acc += part[(size_t)c * nblocks + q];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && c < nout) grad[c] = (float)s[0];
 }
 
 // MSE loss over `nout` channels: dy = 2(y − t)/(V nout) (BF16, padded to
@@ -547,19 +1073,32 @@ __global__ void k_bias_reduce(const float *part, int nblocks, int nout, float *g
 __global__ void __launch_bounds__(256) k_mse(const float *y, const float *t, int nout, long V, unsigned short *dy,
                                              int ldd, double *part)
 {
+    // one voxel per thread and pass: the ldd BF16 gradient channels (zero
+    // past nout) leave as 16-byte stores of 8 channels (ldd % 8 == 0)
     __shared__ double s[256];
     const double scale = 2.0 / ((double)V * nout);
     double acc = 0;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < V * ldd; e += (long)gridDim.x * blockDim.x) {
-        const long v = e / ldd;
-        const int c = (int)(e - v * ldd);
-        float d = 0.f;
-        if (c < nout) {
-            const float r = y[v * nout + c] - t[v * nout + c];
-            acc += (double)r * r;
-            d = (float)(scale * r);
+    for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < V; v += (long)gridDim.x * blockDim.x) {
+        for (int c0 = 0; c0 < ldd; c0 += 8) {
+            unsigned int w[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                unsigned short hb[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int c = c0 + 2 * q + e;
+                    float d = 0.f;
+                    if (c < nout) {
+                        const float r = y[v * nout + c] - t[v * nout + c];
+                        acc += (double)r * r;
+                        d = (float)(scale * r);
+                    }
+                    hb[e] = f2bf(d);
+                }
+                w[q] = (unsigned int)hb[0] | ((unsigned int)hb[1] << 16);
+            }
+            *reinterpret_cast<u32x4 *>(dy + v * ldd + c0) = u32x4{w[0], w[1], w[2], w[3]};
         }
-        dy[e] = f2bf(d);
     }
     s[threadIdx.x] = acc;
     __syncthreads();

@@ -33,11 +33,14 @@ def test_forward_matches_oracle(M, B):
     assert rel(y, yo) < 2e-2
 
 
-@pytest.mark.parametrize("M,B", [(8, 2), (16, 2), (32, 1)])
+@pytest.mark.parametrize("M,B", [(8, 2), (16, 2), (32, 1), (64, 1)])
 def test_gradients_match_oracle(M, B):
-    """M = 8 takes the voxel-tile weight-gradient kernel, M = 16 and 32 the
-    row-staged one (x-rows of a multiple of 16); M = 32 also the row-staged
-    forward and data-gradient convolutions."""
+    """M = 8 takes the voxel-tile weight-gradient kernel, M = 16, 32 and 64
+    the plane kernel (nine (dy, dx) offsets per block, a ring of three
+    staged input rows; x-rows of a multiple of 16); M = 32 and 64 also the
+    row-staged forward and data-gradient convolutions, with register
+    prefetch at M = 64.  M = 64 is the production grid (the oracle takes
+    ~30 s there)."""
     net, p, x, t = setup_case(B=B, M=M, seed=1)
     loss = net.loss_grad(x, t)
     g = net.grads()
@@ -51,10 +54,15 @@ def test_gradients_match_oracle(M, B):
         off += nw + co
 
 
-@pytest.mark.parametrize("env", [{"DNS_CNN_MFMA": "k8"}, {"DNS_CNN_CONV": "direct"}])
+@pytest.mark.parametrize("env", [{"DNS_CNN_MFMA": "k8"}, {"DNS_CNN_CONV": "direct"}, {"DNS_CNN_CONV": "rows"},
+                                 {"DNS_CNN_WGRAD": "row16"}, {"DNS_CNN_WGRAD": "rows"}])
 def test_kernel_variants_match_oracle(env, monkeypatch):
-    """The 32x32x8 MFMA kernels (DNS_CNN_MFMA=k8) and the direct convolution
-    at M = 32 (DNS_CNN_CONV=direct): forward and gradients against the oracle."""
+    """The 32x32x8 MFMA kernels (DNS_CNN_MFMA=k8), the direct convolution
+    at M = 32 (DNS_CNN_CONV=direct), the row convolution without register
+    prefetch (DNS_CNN_CONV=rows; the default takes it below M = 64), and the
+    weight gradients by rows with three x-shifted staged copies
+    (DNS_CNN_WGRAD=row16) or one staged row per dx triple
+    (DNS_CNN_WGRAD=rows): forward and gradients against the oracle."""
     for key, val in env.items():
         monkeypatch.setenv(key, val)
     net, p, x, t = setup_case(B=1, M=32, seed=4)
