@@ -24,6 +24,7 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 // of them stay in registers (HIP's uint4 is a struct whose copies are
 // memcpys, which keeps such arrays in scratch memory)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
 
 __device__ inline unsigned short f2bf(float f)
 {
@@ -35,11 +36,32 @@ __device__ inline unsigned short f2bf(float f)
 }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-__device__ inline float gelu_f(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// Normal CDF Φ(x) = ½(1 + erf(x/√2)) with erf from Abramowitz & Stegun
+// 7.1.26 (|error| ≤ 1.5e-7): erf(u) = 1 − t(a1 + t(a2 + t(a3 + t(a4 + t a5))))
+// e^{−u²}, t = 1/(1 + p u), u ≥ 0, on the hardware reciprocal and exp2;
+// the tail ½ t(…) e^{−u²} is used directly for x < 0 (no cancellation).
+// ex returns e^{−x²/2}, which the GELU derivative shares.  The activations
+// are rounded to BF16 (2^−8) after every layer, far above this error.
+__device__ inline float phi_cdf(float x, float &ex)
+{
+    const float u = fabsf(x) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, u, 1.0f));
+    ex = __expf(-0.5f * x * x);
+    const float poly =
+        t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
+    const float tail = 0.5f * poly * ex; // ½ erfc(u)
+    return x >= 0.0f ? 1.0f - tail : tail;
+}
+__device__ inline float gelu_f(float x)
+{
+    float ex;
+    return x * phi_cdf(x, ex);
+}
 __device__ inline float gelu_d(float x)
 {
-    const float phi = 0.39894228040143268f * expf(-0.5f * x * x);
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * phi;
+    float ex;
+    const float c = phi_cdf(x, ex);
+    return c + x * 0.39894228040143268f * ex;
 }
 
 struct ConvGeom {
@@ -402,33 +424,57 @@ __global__ void __launch_bounds__(256) k_conv_rows_pf(ConvArgs a, ConvGeom g)
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
                         acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            av[mb], __builtin_bit_cast(bf16x8, bf[cur][nb][ks]), acc[mb][nb], 0, 0, 0);
+                            __builtin_bit_cast(bf16x8, bf[cur][nb][ks]), av[mb], acc[mb][nb], 0, 0, 0);
             }
         }
     }
+    // The MFMAs take the weights as A and the voxels as B, so the accumulators
+    // hold C^T: lane l has voxel 32·mb + l % 32 and, in accumulators 4g..4g+3,
+    // the four consecutive channels 32·nb + 8g + 4(l / 32) + 0..3, which
+    // leave as one 8-byte store per group (channels-last), converted to BF16
+    // in pairs by the hardware (round to nearest even).
     const long vrow = (long)M * ((y0 + wave) + (long)M * (z + (long)M * b));
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
+    for (int mb = 0; mb < MB; ++mb) {
+        const long vv = vrow + 32 * mb + (lane & 31);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int col = 32 * nb + (lane & 31);
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const long vv = vrow + 32 * mb + 8 * (r >> 2) + 4 * h + (r & 3);
-                if (col >= a.nout) continue;
-                const float c = acc[mb][nb][r];
+            for (int gq = 0; gq < 4; ++gq) {
+                const int c0 = 32 * nb + 8 * gq + 4 * h;
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = acc[mb][nb][4 * gq + k];
+                if (EPI == 1) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (c0 + k < a.nout) a.yout[vv * a.ldo + c0 + k] = v[k] + a.bias[c0 + k];
+                    continue;
+                }
+                if (c0 >= a.nout) continue; // nout is a multiple of 4 for the BF16 outputs
+                const size_t off = (size_t)vv * a.ldo + c0;
                 if (EPI == 0) {
-                    const float zz = c + a.bias[col];
-                    a.zbuf[vv * a.ldo + col] = f2bf(zz);
-                    a.out[vv * a.ldo + col] = f2bf(gelu_f(zz));
-                } else if (EPI == 1) {
-                    a.yout[vv * a.ldo + col] = c + a.bias[col];
+                    float zz[4], gg[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        zz[k] = v[k] + a.bias[c0 + k];
+                        gg[k] = gelu_f(zz[k]);
+                    }
+                    const bf16x4v zb = {(__bf16)zz[0], (__bf16)zz[1], (__bf16)zz[2], (__bf16)zz[3]};
+                    const bf16x4v gb = {(__bf16)gg[0], (__bf16)gg[1], (__bf16)gg[2], (__bf16)gg[3]};
+                    *reinterpret_cast<bf16x4v *>(a.zbuf + off) = zb;
+                    *reinterpret_cast<bf16x4v *>(a.out + off) = gb;
                 } else {
-                    const float zp = bf2f(a.zprev[vv * a.ldo + col]);
-                    a.out[vv * a.ldo + col] = f2bf(c * gelu_d(zp));
+                    const bf16x4v zp = *reinterpret_cast<const bf16x4v *>(a.zprev + off);
+                    float d[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) d[k] = v[k] * gelu_d((float)zp[k]);
+                    const bf16x4v db = {(__bf16)d[0], (__bf16)d[1], (__bf16)d[2], (__bf16)d[3]};
+                    *reinterpret_cast<bf16x4v *>(a.out + off) = db;
                 }
             }
-        }
+    }
+
 }
 
 // Weight gradient of one offset per block-row: for o = blockIdx.y,
