@@ -145,6 +145,7 @@ struct CNN {
         const char *envc = getenv("DNS_CNN_CONV");
         row_conv = !(envc && !strcmp(envc, "direct"));
         conv_pf = row_conv && !(envc && !strcmp(envc, "rows"));
+        if (const char *ew = getenv("DNS_CNN_WLDS")) conv_wlds = atoi(ew) != 0;
         const int kin[4] = {3, 64, 64, 64}, nout[4] = {64, 64, 64, 6};
         size_t off = 0;
         for (int l = 0; l < 4; ++l) {
@@ -227,6 +228,7 @@ struct CNN {
     int wgrad_splits = 128;    // plane splits of k_conv_wgrad_planes (DNS_CNN_WSPLIT, sweep)
     bool row_conv = false;  // k_conv_rows (M = 32 or 64) instead of k_conv16; DNS_CNN_CONV=direct disables
     bool conv_pf = false;   // k_conv_rows_pf at M = 64 (kin 16 or 64); DNS_CNN_CONV=rows selects k_conv_rows
+    bool conv_wlds = true;  // its weights shared through LDS; DNS_CNN_WLDS=0 streams them per wavefront
     template <int EPI> void conv(const ConvArgs &ca, int np, long Vn, hipStream_t s)
     {
         if (mfma_k8) {
@@ -242,7 +244,16 @@ struct CNN {
         if (row_conv && (M == 32 || M == 64) && ca.kin % 16 == 0 && (np == 32 || np == 64)) {
             const dim3 grd((unsigned)(M / 4), (unsigned)M, (unsigned)(Vn / Vs));
             const size_t lds = sizeof(unsigned short) * 6 * (size_t)M * (ca.kin + 8);
-            if (conv_pf && M == 64 && (ca.kin == 16 || ca.kin == 64)) {
+            if (conv_pf && M == 64 && (ca.kin == 16 || ca.kin == 64) && conv_wlds) {
+                const size_t ldsw = lds + sizeof(unsigned short) * 2 * (size_t)np * (ca.kin + 8);
+                if (ca.kin == 64) {
+                    if (np == 64) hipLaunchKernelGGL((k_conv_rows_pf<2, 2, 64, EPI, true>), grd, dim3(256), ldsw, s, ca, geom(Vn));
+                    else hipLaunchKernelGGL((k_conv_rows_pf<2, 1, 64, EPI, true>), grd, dim3(256), ldsw, s, ca, geom(Vn));
+                } else {
+                    if (np == 64) hipLaunchKernelGGL((k_conv_rows_pf<2, 2, 16, EPI, true>), grd, dim3(256), ldsw, s, ca, geom(Vn));
+                    else hipLaunchKernelGGL((k_conv_rows_pf<2, 1, 16, EPI, true>), grd, dim3(256), ldsw, s, ca, geom(Vn));
+                }
+            } else if (conv_pf && M == 64 && (ca.kin == 16 || ca.kin == 64)) {
                 if (ca.kin == 64) {
                     if (np == 64) hipLaunchKernelGGL((k_conv_rows_pf<2, 2, 64, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));
                     else hipLaunchKernelGGL((k_conv_rows_pf<2, 1, 64, EPI>), grd, dim3(256), lds, s, ca, geom(Vn));

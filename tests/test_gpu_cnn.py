@@ -102,3 +102,23 @@ def test_training_reduces_loss():
         net.loss_grad(x, t)
         net.adam_step(lr=1e-3)
     assert net.loss_grad(x, t) < 0.7 * l0
+
+
+def test_lds_weights_match_streamed_weights(monkeypatch):
+    """The production convolution at M = 64 with the weights shared through
+    LDS (default) and streamed per wavefront (DNS_CNN_WLDS=0): same operands
+    in the same order, so forward output, loss and gradients are bitwise
+    equal."""
+    res = []
+    for val in ("1", "0"):
+        monkeypatch.setenv("DNS_CNN_WLDS", val)
+        net, p, x, t = setup_case(B=1, M=64, seed=5)
+        y = net.forward(x)
+        loss = net.loss_grad(x, t)
+        res.append((y, loss, net.grads()))
+        del net
+    (y0, l0, g0), (y1, l1, g1) = res
+    assert np.array_equal(y0, y1)
+    assert l0 == l1
+    assert np.array_equal(g0, g1)
+
