@@ -431,6 +431,11 @@ template <typename T> struct FVImpl : FVBase {
                       },
                       thomas_tl, thomas_nthr, ldst);
             if (env && !strcmp(env, "sequential")) thomas_tl = 0;
+            if (const char *et = getenv("DNS_THOMAS_TL")) { // lines per block (sweep)
+                const int tl = atoi(et);
+                if (thomas_tl && tl >= 1 && 256 % tl == 0 && (ny + 256 / tl - 1) / (256 / tl) <= THOMAS_RMAX)
+                    thomas_tl = tl;
+            }
             if (!thomas_tl) DNS_TRY(dalloc(&Dth, (size_t)g.ldc * ny * nz));
             // equal segments of R rows (ny = S·R) use k_thomas_seg; DNS_THOMAS=tile keeps k_thomas_tile
             thomas_seg = 0;
@@ -1478,7 +1483,9 @@ template <typename T> struct FVImpl : FVBase {
             return DNS_EINVAL;
         }
         const int NC = g.nxc * g.ny * g.nz;
-        const int nb = std::min(256, ceil_div(NC, NTHR));
+        // per-wavefront binning (nbins <= 2048) scales to more blocks than the
+        // chunk-owner kernel
+        const int nb = std::min(nbins <= 2048 ? 2048 : 256, ceil_div(NC, NTHR));
         const size_t need = (size_t)nb * nbins;
         // grow-only buffers of their own: `red` is captured by the CFL graphs
         if (need > spartcap || (size_t)nbins > soutcap) {
@@ -1521,9 +1528,13 @@ template <typename T> struct FVImpl : FVBase {
                                twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
                                st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, make_fastdiv(tby), g, fdy);
-            hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR),
-                               sizeof(double) * nbins + (sizeof(double) + sizeof(int)) * NTHR, st, A, g, nbins,
-                               1.0 / N2, spart);
+            if (nbins <= 2048)
+                hipLaunchKernelGGL((k_binshells_w<T>), dim3(nb), dim3(NTHR), sizeof(double) * 4 * nbins, st, A, g,
+                                   nbins, 1.0 / N2, spart);
+            else
+                hipLaunchKernelGGL((k_binshells<T>), dim3(nb), dim3(NTHR),
+                                   sizeof(double) * nbins + (sizeof(double) + sizeof(int)) * NTHR, st, A, g, nbins,
+                                   1.0 / N2, spart);
             hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nbins, NTHR)), dim3(NTHR), 0, st, spart, nb, nbins, sout);
             DNS_HIP_CHECK(hipGetLastError());
             DNS_HIP_CHECK(hipMemcpyAsync(hostE.data(), sout, sizeof(double) * nbins, hipMemcpyDeviceToHost, st));

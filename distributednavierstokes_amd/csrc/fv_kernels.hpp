@@ -1920,4 +1920,37 @@ __global__ void __launch_bounds__(256) k_binshells(const cplx<T> *A, FVGeom g, i
     for (int b = tid; b < nbins; b += 256) part[(size_t)blockIdx.x * nbins + b] = bins[b];
 }
 
+// k_binshells with per-wavefront bins (4 nbins doubles of LDS, nbins ≤
+// 2048): each wavefront bins its 64 elements per pass with wave_bin_add; the
+// four arrays are summed in wave order.  Deterministic.
+template <typename T>
+__global__ void __launch_bounds__(256) k_binshells_w(const cplx<T> *A, FVGeom g, int nbins, double invN2, double *part)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *bins = reinterpret_cast<double *>(smem_raw); // [4][nbins]
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int b = tid; b < 4 * nbins; b += 256) bins[b] = 0.0;
+    __syncthreads();
+    const int N = g.nxc * g.ny * g.nz;
+    for (int base = blockIdx.x * 256; base < N; base += gridDim.x * 256) {
+        const int t = base + tid;
+        int kap = -1;
+        double e = 0.0;
+        if (t < N) {
+            const int kx = t % g.nxc, r = t / g.nxc, j = r % g.ny, k = r / g.ny;
+            const int ky = j > g.ny / 2 ? j - g.ny : j;
+            const int kz = k > g.nz / 2 ? k - g.nz : k;
+            kap = (int)rint(sqrt((double)(kx * kx + ky * ky + kz * kz)));
+            const double wgt = (kx > 0 && 2 * kx < g.nx) ? 2.0 : 1.0;
+            const cplx<T> a = A[kx + (size_t)g.ldc * r];
+            e = wgt * ((double)a.x * (double)a.x + (double)a.y * (double)a.y) / 2 * invN2;
+            if (kap >= nbins) kap = -1;
+        }
+        wave_bin_add(bins + wave * nbins, kap, e);
+    }
+    __syncthreads();
+    for (int b = tid; b < nbins; b += 256)
+        part[(size_t)blockIdx.x * nbins + b] = ((bins[b] + bins[nbins + b]) + bins[2 * nbins + b]) + bins[3 * nbins + b];
+}
+
 }  // namespace dns

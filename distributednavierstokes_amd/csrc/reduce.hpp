@@ -73,6 +73,27 @@ static __global__ void k_dd_final(const dd *part, int nparts, int nslots, double
 }
 
 // Sums per-block bins in block order (deterministic final pass).
+// Adds each lane's value v to bins[key] (lanes with key < 0 add nothing),
+// deterministically and without atomics: the wavefront walks its distinct
+// keys (lowest pending lane first), sums the lanes of each key with a fixed
+// shuffle tree (other lanes contribute an exact 0) and lane 0 adds the sum.
+// bins is this wavefront's own array, so no other wavefront writes it.
+__device__ inline void wave_bin_add(double *bins, int key, double v)
+{
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(key >= 0);
+    while (pending) {
+        const int first = __ffsll(pending) - 1;
+        const int k = __shfl(key, first, 64);
+        const unsigned long long same = __ballot(key == k);
+        double x = key == k ? v : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) bins[k] += x;
+        pending &= ~same;
+    }
+}
+
 static __global__ void k_bins_final(const double *part, int nblocks, int nbins, double *out)
 {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nbins; b += gridDim.x * blockDim.x) {

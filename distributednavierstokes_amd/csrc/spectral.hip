@@ -511,7 +511,9 @@ template <typename T> struct SpecImpl : SpecBase {
         }
         int r = s0 + nsh - 1;
         long count = cube_count(r);
-        int nb = (int)std::max<long>(1, std::min<long>(256, (count + SP_NTHR - 1) / SP_NTHR));
+        // per-wavefront binning (nsh <= 2048) scales to more blocks than the
+        // chunk-owner kernel
+        int nb = (int)std::max<long>(1, std::min<long>(nsh <= 2048 ? 2048 : 256, (count + SP_NTHR - 1) / SP_NTHR));
         size_t need = (size_t)nb * nsh;
         if (need > redcap) {
             DNS_HIP_CHECK(hipStreamSynchronize(st));
@@ -522,9 +524,13 @@ template <typename T> struct SpecImpl : SpecBase {
             redcap = need;
         }
         timer.begin(SFAM_FORCE);
-        hipLaunchKernelGGL((k_sp_shells<T>), dim3(nb), dim3(SP_NTHR),
-                           sizeof(double) * nsh + (sizeof(double) + sizeof(int)) * SP_NTHR, st, uh, s0, nsh, r, count,
-                           g, red);
+        if (nsh <= 2048)
+            hipLaunchKernelGGL((k_sp_shells_w<T>), dim3(nb), dim3(SP_NTHR), sizeof(double) * 4 * nsh, st, uh, s0, nsh,
+                               r, count, g, red);
+        else
+            hipLaunchKernelGGL((k_sp_shells<T>), dim3(nb), dim3(SP_NTHR),
+                               sizeof(double) * nsh + (sizeof(double) + sizeof(int)) * SP_NTHR, st, uh, s0, nsh, r,
+                               count, g, red);
         hipLaunchKernelGGL(k_bins_final, dim3(ceil_div(nsh, SP_NTHR)), dim3(SP_NTHR), 0, st, red, nb, nsh, factors);
         timer.end(SFAM_FORCE);
         DNS_HIP_CHECK(hipGetLastError());

@@ -658,6 +658,46 @@ __global__ void __launch_bounds__(256) k_sp_shells(const cplx<T> *uh, int s0, in
     for (int b = tid; b < ns; b += 256) part[(size_t)blockIdx.x * ns + b] = bins[b];
 }
 
+// k_sp_shells with per-wavefront bins (4 ns doubles of LDS, ns ≤ 2048):
+// each wavefront bins its 64 elements per pass with wave_bin_add (a few
+// distinct shells per pass) instead of every bin owner scanning the whole
+// chunk; the four arrays are summed in wave order.  Deterministic.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sp_shells_w(const cplx<T> *uh, int s0, int ns, int r, long count, SpecGeom g,
+                                                     double *part)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *bins = reinterpret_cast<double *>(smem_raw); // [4][ns]
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int b = tid; b < 4 * ns; b += 256) bins[b] = 0.0;
+    __syncthreads();
+    const size_t cs = (size_t)g.K * g.m * g.m;
+    for (long base = (long)blockIdx.x * 256; base < count; base += (long)gridDim.x * 256) {
+        const long t = base + tid;
+        int sh = -1;
+        double e = 0;
+        if (t < count) {
+            int i, j, k;
+            sp_cube(t, r, g, i, j, k);
+            sh = sp_shell(i, j, k, g) - s0;
+            if (sh >= 0 && sh < ns) {
+                const size_t at = i + (size_t)g.K * (j + (size_t)g.m * k);
+                for (int c = 0; c < 3; ++c) {
+                    cplx<T> a = uh[at + c * cs];
+                    e += (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+                }
+                e = (i == 0 ? 1.0 : 2.0) * e / 2;
+            } else {
+                sh = -1;
+            }
+        }
+        wave_bin_add(bins + wave * ns, sh, e);
+    }
+    __syncthreads();
+    for (int b = tid; b < ns; b += 256)
+        part[(size_t)blockIdx.x * ns + b] = ((bins[b] + bins[ns + b]) + bins[2 * ns + b]) + bins[3 * ns + b];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_sp_rescale(cplx<T> *uh, int s0, int ns, int r, long count,
                                                     const double *factors, SpecGeom g)
