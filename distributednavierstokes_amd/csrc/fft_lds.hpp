@@ -221,10 +221,12 @@ __device__ inline const cplx<T> *stage_twiddles(cplx<T> *dst, const cplx<T> *tw,
 // Transforms nlines lines of d.n points held in LDS at buf[line*ls + t].
 // tw[t] = exp(-2πi t / n) for t in [0, n).  Unnormalised in both directions.
 // Must be called by every thread of the block (contains barriers).
-template <typename T, int EPT, bool INV>
+// NT > 0: the block size is known at compile time (blockDim.x is then not
+// read, which would be a memory load).
+template <typename T, int EPT, bool INV, int NT = 0>
 __device__ inline void fft_lds(cplx<T> *buf, int nlines, int ls, const FFTDesc &d, const cplx<T> *__restrict__ tw)
 {
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int tid = threadIdx.x, nthr = NT > 0 ? NT : (int)blockDim.x;
     int Ns = 1;
     for (int p = 0; p < d.npass; ++p) {
         int R = d.radix[p];
@@ -310,9 +312,13 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
 // (r = 0..7) on entry and X[t + 64 r] on return; Stockham radix 8 in three
 // passes (Ns = 1, 8, 64) with the exchanges through the line's LDS storage
 // sh, as group_fft.  tw[t] = exp(-2πi t / 512); unnormalised.
-template <typename T, bool INV>
+// PAD: exchange element i sits at sh[i + i/8] (sh then spans 575 elements):
+// the first pass writes element 8 t + r from lane t, which without the pad
+// puts every other lane on the same LDS banks.
+template <typename T, bool INV, bool PAD = false>
 __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const cplx<T> *tw, int t)
 {
+    auto x = [](int i) { return PAD ? i + (i >> 3) : i; };
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const int Ns = 1 << (3 * p);
@@ -328,10 +334,10 @@ __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const 
         if (p < 2) {
             const int base = (t - k) * 8 + k;
 #pragma unroll
-            for (int r = 0; r < 8; ++r) sh[base + r * Ns] = a[r];
+            for (int r = 0; r < 8; ++r) sh[x(base + r * Ns)] = a[r];
             wave_lds_sync();
 #pragma unroll
-            for (int r = 0; r < 8; ++r) a[r] = sh[t + 64 * r];
+            for (int r = 0; r < 8; ++r) a[r] = sh[x(t + 64 * r)];
             wave_lds_sync();
         }
     }
@@ -341,12 +347,12 @@ __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const 
 // by wavefront lane groups: G = N/4 lanes per line (64 for N = 512, 8 points
 // per lane), 64/G lines per wavefront at a time, the wavefronts of the block
 // taking lines in turn.  Needs nlines % (64/G) == 0.  Ends with a workgroup
-// barrier, as fft_lds.
-template <typename T, bool INV, int N>
+// barrier, as fft_lds.  PAD (N = 512): padded exchanges, needs ls >= 575.
+template <typename T, bool INV, int N, int NT = 0, bool PAD = false>
 __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls, const cplx<T> *tw)
 {
     constexpr int G = N == 512 ? 64 : N / 4, LPW = 64 / G;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (NT > 0 ? NT : (int)blockDim.x) >> 6;
     const int t = lane % G, lg = lane / G;
     for (int l0 = wv * LPW; l0 < nlines; l0 += nw * LPW) {
         cplx<T> *line = buf + (size_t)(l0 + lg) * ls;
@@ -355,7 +361,7 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 #pragma unroll
             for (int r = 0; r < 8; ++r) a[r] = line[t + 64 * r];
             wave_lds_sync();
-            wave_fft512<T, INV>(a, line, tw, t);
+            wave_fft512<T, INV, PAD>(a, line, tw, t);
 #pragma unroll
             for (int r = 0; r < 8; ++r) line[t + 64 * r] = a[r];
         } else {
@@ -375,16 +381,19 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 // The batched LDS transforms of a tile: lane groups for the lengths they
 // serve (16 ... 512, powers of two, nlines a multiple of the lines per
 // wavefront), otherwise fft_lds over sub-batches of at most sub lines.
-template <typename T, int EPT, bool INV>
+// PAD512: the 512-point lines use padded exchanges (the caller's line
+// stride ls is then at least 575).
+template <typename T, int EPT, bool INV, int NT = 0, bool PAD512 = false>
 __device__ __forceinline__ void fft_lds_tile(cplx<T> *buf, int nlines, int ls, const FFTDesc &d,
                                              const cplx<T> *tw, int sub)
 {
     const int n = d.n;
-    if (n == 512) return fft_lds_groups<T, INV, 512>(buf, nlines, ls, tw);
-    if (n == 256) return fft_lds_groups<T, INV, 256>(buf, nlines, ls, tw);
-    if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128>(buf, nlines, ls, tw);
-    if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64>(buf, nlines, ls, tw);
-    for (int sb = 0; sb < nlines; sb += sub) fft_lds<T, EPT, INV>(buf + (size_t)sb * ls, min(sub, nlines - sb), ls, d, tw);
+    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PAD512>(buf, nlines, ls, tw);
+    if (n == 256) return fft_lds_groups<T, INV, 256, NT>(buf, nlines, ls, tw);
+    if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128, NT>(buf, nlines, ls, tw);
+    if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64, NT>(buf, nlines, ls, tw);
+    for (int sb = 0; sb < nlines; sb += sub)
+        fft_lds<T, EPT, INV, NT>(buf + (size_t)sb * ls, min(sub, nlines - sb), ls, d, tw);
 }
 
 }  // namespace dns

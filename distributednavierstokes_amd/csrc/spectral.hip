@@ -116,14 +116,14 @@ template <typename T> struct SpecImpl : SpecBase {
         // y/z pencil tiles: FFT sub-batches of tbf lines; a block may load
         // tb >= tbf lines (wider rows measured no faster, so tb = tbf)
         tbf = std::max(1, std::min(64, SP_EPT * SP_NTHR / g.n));
-        while (tbf > 1 && ((size_t)tbf * (g.n + 1) + g.n) * sz > (size_t)maxlds) --tbf;
+        while (tbf > 1 && ((size_t)tbf * sp_line_stride<T>(g.n) + g.n) * sz > (size_t)maxlds) --tbf;
         tbf = std::min(tbf, g.K);
         tb = tbf;
-        lds = ((size_t)tb * (g.n + 1) + g.n) * sz; // tile + LDS twiddle table
+        lds = ((size_t)tb * sp_line_stride<T>(g.n) + g.n) * sz; // tile + LDS twiddle table
         tbs = std::max(1, std::min({64, SP_SEPT * SP_NTHR / g.n, SP_MAXO * SP_NTHR / g.m}));
-        while (tbs > 1 && ((size_t)tbs * (g.n + 1) + g.n) * sz > (size_t)maxlds) --tbs;
+        while (tbs > 1 && ((size_t)tbs * sp_line_stride<T>(g.n) + g.n) * sz > (size_t)maxlds) --tbs;
         tbs = std::min(tbs, g.K);
-        ldss = ((size_t)tbs * (g.n + 1) + g.n) * sz;
+        ldss = ((size_t)tbs * sp_line_stride<T>(g.n) + g.n) * sz;
         // per line: three real velocity lines and three half-length complex work lines
         const size_t perline = (size_t)6 * g.n * sizeof(T);
         {
@@ -136,7 +136,8 @@ template <typename T> struct SpecImpl : SpecBase {
         }
         while (lpb > 1 && 3 * lpb * (g.n / 2) > SP_XEPT * SP_NTHR) --lpb;
         ldsx = perline * lpb + (size_t)(sizeof(T) == 4 ? g.n + 1 : g.n / 2) * sz; // + the LDS twiddle tables
-        if (lds > (size_t)maxlds || ldss > (size_t)maxlds || ldsx + 2048 > (size_t)maxlds || tbs * g.m > SP_MAXO * SP_NTHR) {
+        if (lds > (size_t)maxlds || ldss > (size_t)maxlds || ldsx + 2048 > (size_t)maxlds || tbs * g.m > SP_MAXO * SP_NTHR ||
+            tbs * g.n > SP_SEPT * SP_NTHR) {
             set_error("spectral tiles do not fit the LDS limit");
             return DNS_EUNSUPPORTED;
         }
@@ -306,7 +307,7 @@ template <typename T> struct SpecImpl : SpecBase {
         s.nudcdt = (T)(visc * W_DC[stage] * dt);
         s.dt = (T)dt;
         timer.begin(SFAM_STAGE);
-        hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, MODE>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR),
+        hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, MODE, SP_NTHR>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR),
                            ldss, st, s, g, fdn);
         timer.end(SFAM_STAGE);
         DNS_HIP_CHECK(hipGetLastError());
@@ -346,7 +347,7 @@ template <typename T> struct SpecImpl : SpecBase {
         sa.ftb = make_fastdiv(tbs);
         sa.first = 1;
         sa.a = sa.b = sa.nudcdt = sa.dt = T(0);
-        hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, 2>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR), ldss,
+        hipLaunchKernelGGL((k_sp_ystage<T, SP_SEPT, SP_MAXO, 2, SP_NTHR>), dim3(ceil_div(g.K, tbs), g.m), dim3(SP_NTHR), ldss,
                            st, sa, g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;

@@ -36,6 +36,16 @@ struct SpecGeom {
     double kfac; // 2π / l
 };
 
+// LDS line stride (elements) of the y/z pencil tiles.  FP64 512-point lines
+// take 580: room for the padded transform exchanges (575), and a stride of
+// 16 banks between the lines of a tile, so the tile's row-wise stores and
+// reads (line fastest across lanes) fall on distinct banks.
+template <typename T> __host__ __device__ constexpr int sp_line_stride(int n)
+{
+    return sizeof(T) == 8 && n == 512 ? 580 : n + 1;
+}
+#define SP_PAD512(T) (sizeof(T) == 8)
+
 __device__ inline int sp_fine(int c, const SpecGeom &g) { return c <= g.kcut ? c : c + (g.n - g.m); }
 __device__ inline int sp_freq(int c, const SpecGeom &g) { return c <= g.kcut ? c : c - g.m; }
 // fine position → compact index, or -1 in the killed band
@@ -56,7 +66,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
     const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
@@ -75,7 +85,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, true>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
@@ -90,7 +100,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
     const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
@@ -109,7 +119,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, true>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
@@ -124,7 +134,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
     const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
@@ -140,7 +150,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int kc = fdiv(idx, ftb), b = idx - kc * tb, i = i0 + b;
         if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
@@ -156,7 +166,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int n = g.n, ls = n + 1, K = g.K, m = g.m;
+    const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
     const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
@@ -172,7 +182,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
     const T inv = T(1.0 / ((double)n * n * n));
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int jc = fdiv(idx, ftb), b = idx - jc * tb, i = i0 + b;
@@ -453,12 +463,12 @@ template <typename T> struct StageArgs {
     T a, b, nudcdt, dt;
 };
 
-template <typename T, int EPT, int MAXO, int MODE>
-__global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g, FFTDesc d)
+template <typename T, int EPT, int MAXO, int MODE, int NT>
+__global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-    const int n = g.n, ls = n + 1, K = g.K, m = g.m, tb = s.tb;
+    const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m, tb = s.tb;
     const FastDiv ftb = s.ftb;
     const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
     int bxr, byr;
@@ -471,28 +481,52 @@ __global__ void __launch_bounds__(256, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g
     for (int o = 0; o < MAXO; ++o)
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[o][c] = mk<T>(T(0), T(0));
+    // The tile of the next product is loaded into registers while the
+    // current one is transformed (tb n <= EPT NT, checked by the host).
+    // Buffer loads over the (K, n) slab of (kc, p) with 32-bit offsets; the
+    // elements outside the tile (i >= K, idx >= tb n) get the slab size as
+    // offset, which the buffer range check reads as zero.  No branch around
+    // the loads, so they all stay in flight.
+    const int slab = K * n * (int)sizeof(cplx<T>);
+    int voff[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int idx = threadIdx.x + q * NT;
+        const int t = fdiv(idx, ftb), i = i0 + (idx - t * tb);
+        voff[q] = (idx < tb * n && i < K) ? (i + K * t) * (int)sizeof(cplx<T>) : slab;
+    }
+    cplx<T> pre[EPT];
+    auto load_tile = [&](int p) {
+        const bufres r = make_bufres(s.Q + (size_t)K * n * (kc + (size_t)m * p), (uint32_t)slab);
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) pre[q] = bload<cplx<T>>(r, voff[q], 0);
+    };
+    load_tile(0);
     for (int p = 0; p < 6; ++p) {
-        batched_for<SP_LOAD_U(EPT), cplx<T>>(
-            tb * n,
-            [&](int idx) {
-                int t = fdiv(idx, ftb), i = i0 + (idx - t * tb);
-                return i < K ? s.Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
-            },
-            [&](int idx, cplx<T> v) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < tb * n) {
                 int t = fdiv(idx, ftb), b = idx - t * tb;
-                buf[b * ls + t] = v;
-            });
+                buf[b * ls + t] = pre[q];
+            }
+        }
         __syncthreads();
+        if (p < 5) load_tile(p + 1);
         // FP32: the block-wide engine (the lane-group transforms measured
         // slower here, 3.5 -> 4.3 ms per HIT 512 step, at 16 elements per
         // thread); FP64: lane groups.
         if constexpr (sizeof(T) == 4)
-            fft_lds<T, EPT, false>(buf, tb, ls, d, twl);
+            fft_lds<T, EPT, false, NT>(buf, tb, ls, d, twl);
         else
-            fft_lds_tile<T, EPT, false>(buf, tb, ls, d, twl, tb);
+            fft_lds_tile<T, EPT, false, NT>(buf, tb, ls, d, twl, tb);
+        // the output indices are recomputed for each product rather than
+        // kept live across the loop (registers)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
 #pragma unroll
         for (int o = 0; o < MAXO; ++o) {
-            int oi = threadIdx.x + o * blockDim.x;
+            int oi = tid + o * NT;
             if (oi < tb * m) {
                 int jc = fdiv(oi, ftb), b = oi - jc * tb, i = i0 + b;
                 if (i < K && MODE == 2) {
