@@ -876,10 +876,18 @@ __global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *s
 // (j div Ns) 4 Ns + k + r Ns) with j = l: pass 0 reads the registers as
 // loaded, passes 1-3 read the previous pass's outputs from the wavefront's
 // 256-entry LDS buffer sh, and the outputs of pass 3 land at l + 64 r, in
-// the registers.  tw[t] = exp(-2πi t / 256); unnormalised.
+// the registers.  tw[t] = exp(-2πi t / 256); unnormalised.  The exchanges
+// keep element i at sh[i + i/4] (sh spans 319 elements): the writes of
+// passes 0 and 1 (elements 4 l + r and 16 (l div 4) + l mod 4 + 4 r) would
+// otherwise put four lanes of every sixteen on the same LDS banks.
 template <typename T, bool INV>
-__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw, int lane)
+__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw, int lane_)
 {
+    auto x = [](int i) { return i + (i >> 2); };
+    // the exchange addresses are recomputed at each call rather than kept
+    // live between the nine transforms of a line (registers)
+    int lane = lane_;
+    asm volatile("" : "+v"(lane));
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const int Ns = 1 << (2 * p);
@@ -895,10 +903,10 @@ __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const 
         if (p < 3) {
             const int base = (lane - k) * 4 + k;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sh[base + r * Ns] = a[r];
+            for (int r = 0; r < 4; ++r) sh[x(base + r * Ns)] = a[r];
             wave_lds_sync();
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = sh[lane + 64 * r];
+            for (int r = 0; r < 4; ++r) a[r] = sh[x(lane + 64 * r)];
             wave_lds_sync();
         }
     }
@@ -916,7 +924,7 @@ __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const 
 // Z_{N-k} through LDS) into P[L, 0:K, p].  Same arithmetic as k_sp_xmid with
 // a different radix order (4·4·4·4 instead of 8·8·4).  No workgroup
 // barrier after the twiddle staging.  One vmax partial per block.
-// LDS: the 256 transform and 257 real-transform twiddles, 4 × 256 complex.
+// LDS: the 256 transform and 257 real-transform twiddles, 4 × 320 complex.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
 {
@@ -924,7 +932,7 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *tw = reinterpret_cast<cplx<T> *>(smem_raw);
     cplx<T> *twx = tw + N;
-    cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * N;
+    cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * (N + N / 4); // padded exchange lines
     __shared__ double red[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = a.tw[t];
