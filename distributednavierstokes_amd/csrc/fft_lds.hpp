@@ -315,7 +315,22 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
 // PAD: exchange element i sits at sh[i + i/8] (sh then spans 575 elements):
 // the first pass writes element 8 t + r from lane t, which without the pad
 // puts every other lane on the same LDS banks.
-template <typename T, bool INV, bool PAD = false>
+// PTW: tw is the per-pass table of stage_ptw512 (w^(r k 64 / Ns) at
+// ptw[o_p + (r - 1) Ns + k]), read by the lanes of a pass at consecutive
+// entries; indexing w^t by r k 64 / Ns puts the lanes of the Ns = 8 pass on
+// one bank.
+constexpr int PTW512_LEN = 7 * (8 + 64);
+template <typename T> __device__ inline const cplx<T> *stage_ptw512(cplx<T> *dst, const cplx<T> *tw)
+{
+    for (int e = threadIdx.x; e < PTW512_LEN; e += blockDim.x) {
+        const int Ns = e < 56 ? 8 : 64;
+        const int j = e - (Ns == 8 ? 0 : 56), r = j / Ns + 1, k = j - (r - 1) * Ns;
+        dst[e] = tw[r * k * (64 / Ns)];
+    }
+    return dst;
+}
+
+template <typename T, bool INV, bool PAD = false, bool PTW = false>
 __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const cplx<T> *tw, int t)
 {
     auto x = [](int i) { return PAD ? i + (i >> 3) : i; };
@@ -326,7 +341,7 @@ __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const 
         if (p > 0) {
 #pragma unroll
             for (int r = 1; r < 8; ++r) {
-                const cplx<T> w = tw[r * k * (64 / Ns)];
+                const cplx<T> w = PTW ? tw[(Ns == 8 ? 0 : 56) + (r - 1) * Ns + k] : tw[r * k * (64 / Ns)];
                 a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
             }
         }
@@ -347,8 +362,9 @@ __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const 
 // by wavefront lane groups: G = N/4 lanes per line (64 for N = 512, 8 points
 // per lane), 64/G lines per wavefront at a time, the wavefronts of the block
 // taking lines in turn.  Needs nlines % (64/G) == 0.  Ends with a workgroup
-// barrier, as fft_lds.  PAD (N = 512): padded exchanges, needs ls >= 575.
-template <typename T, bool INV, int N, int NT = 0, bool PAD = false>
+// barrier, as fft_lds.  PAD (N = 512): padded exchanges, needs ls >= 575;
+// PTW (N = 512): tw is the per-pass table of stage_ptw512.
+template <typename T, bool INV, int N, int NT = 0, bool PAD = false, bool PTW = false>
 __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls, const cplx<T> *tw)
 {
     constexpr int G = N == 512 ? 64 : N / 4, LPW = 64 / G;
@@ -361,7 +377,7 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 #pragma unroll
             for (int r = 0; r < 8; ++r) a[r] = line[t + 64 * r];
             wave_lds_sync();
-            wave_fft512<T, INV, PAD>(a, line, tw, t);
+            wave_fft512<T, INV, PAD, PTW>(a, line, tw, t);
 #pragma unroll
             for (int r = 0; r < 8; ++r) line[t + 64 * r] = a[r];
         } else {
@@ -382,13 +398,14 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 // serve (16 ... 512, powers of two, nlines a multiple of the lines per
 // wavefront), otherwise fft_lds over sub-batches of at most sub lines.
 // PAD512: the 512-point lines use padded exchanges (the caller's line
-// stride ls is then at least 575).
-template <typename T, int EPT, bool INV, int NT = 0, bool PAD512 = false>
+// stride ls is then at least 575); PTW512: for n = 512, tw is the per-pass
+// table of stage_ptw512.
+template <typename T, int EPT, bool INV, int NT = 0, bool PAD512 = false, bool PTW512 = false>
 __device__ __forceinline__ void fft_lds_tile(cplx<T> *buf, int nlines, int ls, const FFTDesc &d,
                                              const cplx<T> *tw, int sub)
 {
     const int n = d.n;
-    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PAD512>(buf, nlines, ls, tw);
+    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PAD512, PTW512>(buf, nlines, ls, tw);
     if (n == 256) return fft_lds_groups<T, INV, 256, NT>(buf, nlines, ls, tw);
     if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128, NT>(buf, nlines, ls, tw);
     if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64, NT>(buf, nlines, ls, tw);

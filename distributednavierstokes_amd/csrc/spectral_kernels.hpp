@@ -46,6 +46,15 @@ template <typename T> __host__ __device__ constexpr int sp_line_stride(int n)
 }
 #define SP_PAD512(T) (sizeof(T) == 8)
 
+// The transform twiddles of a pencil tile in LDS: for 512-point lines
+// transformed by lane groups (GROUPS) the per-pass table of stage_ptw512,
+// otherwise w^t, t < n.
+template <typename T, bool GROUPS = true>
+__device__ inline const cplx<T> *sp_stage_twiddles(cplx<T> *dst, const cplx<T> *tw, int n)
+{
+    return GROUPS && n == 512 ? stage_ptw512<T>(dst, tw) : stage_twiddles<T>(dst, tw, n);
+}
+
 __device__ inline int sp_fine(int c, const SpecGeom &g) { return c <= g.kcut ? c : c + (g.n - g.m); }
 __device__ inline int sp_freq(int c, const SpecGeom &g) { return c <= g.kcut ? c : c - g.m; }
 // fine position → compact index, or -1 in the killed band
@@ -67,7 +76,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kz = byr, c = bzr;
@@ -85,7 +94,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
@@ -101,7 +110,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, c = bzr;
@@ -119,7 +128,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
@@ -135,7 +144,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, p = bzr;
@@ -150,7 +159,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int kc = fdiv(idx, ftb), b = idx - kc * tb, i = i0 + b;
         if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
@@ -167,7 +176,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kc = byr, p = bzr;
@@ -182,7 +191,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T)>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
     const T inv = T(1.0 / ((double)n * n * n));
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int jc = fdiv(idx, ftb), b = idx - jc * tb, i = i0 + b;
@@ -470,7 +479,8 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m, tb = s.tb;
     const FastDiv ftb = s.ftb;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
+    // FP32 transforms with the block-wide engine (natural table), FP64 with lane groups
+    const cplx<T> *twl = sp_stage_twiddles<T, sizeof(T) == 8>(buf + (size_t)tb * ls, s.tw, n);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int i0 = bxr * tb, kc = byr;
@@ -519,7 +529,7 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
         if constexpr (sizeof(T) == 4)
             fft_lds<T, EPT, false, NT>(buf, tb, ls, d, twl);
         else
-            fft_lds_tile<T, EPT, false, NT>(buf, tb, ls, d, twl, tb);
+            fft_lds_tile<T, EPT, false, NT, false, true>(buf, tb, ls, d, twl, tb);
         // the output indices are recomputed for each product rather than
         // kept live across the loop (registers)
         int tid = threadIdx.x;
@@ -876,12 +886,29 @@ __global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *s
 // (j div Ns) 4 Ns + k + r Ns) with j = l: pass 0 reads the registers as
 // loaded, passes 1-3 read the previous pass's outputs from the wavefront's
 // 256-entry LDS buffer sh, and the outputs of pass 3 land at l + 64 r, in
-// the registers.  tw[t] = exp(-2πi t / 256); unnormalised.  The exchanges
-// keep element i at sh[i + i/4] (sh spans 319 elements): the writes of
-// passes 0 and 1 (elements 4 l + r and 16 (l div 4) + l mod 4 + 4 r) would
-// otherwise put four lanes of every sixteen on the same LDS banks.
+// the registers; unnormalised.  The exchanges keep element i at sh[i + i/4]
+// (sh spans 319 elements): the writes of passes 0 and 1 (elements 4 l + r
+// and 16 (l div 4) + l mod 4 + 4 r) would otherwise put four lanes of every
+// sixteen on the same LDS banks.  The twiddles come per pass, ptw (from
+// stage_ptw256): w^(r k 64 / Ns) at ptw[o_p + (r - 1) Ns + k], so the lanes of
+// a pass read consecutive entries (the table w^t indexed by r k 64 / Ns
+// strides the lanes over the banks 4 to 8 ways).
+constexpr int PTW256_LEN = 3 * (4 + 16 + 64);
+__device__ __forceinline__ int ptw256_off(int Ns) { return Ns == 4 ? 0 : Ns == 16 ? 12 : 60; }
+
+// Stages the per-pass twiddle table of wave_fft256 into LDS from
+// tw[t] = exp(-2πi t / 256), t < 256 (the same values, rearranged).
+template <typename T> __device__ inline void stage_ptw256(cplx<T> *ptw, const cplx<T> *tw)
+{
+    for (int e = threadIdx.x; e < PTW256_LEN; e += blockDim.x) {
+        const int Ns = e < 12 ? 4 : e < 60 ? 16 : 64;
+        const int j = e - ptw256_off(Ns), r = j / Ns + 1, k = j - (r - 1) * Ns;
+        ptw[e] = tw[r * k * (64 / Ns)];
+    }
+}
+
 template <typename T, bool INV>
-__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw, int lane_)
+__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *ptw, int lane_)
 {
     auto x = [](int i) { return i + (i >> 2); };
     // the exchange addresses are recomputed at each call rather than kept
@@ -895,7 +922,7 @@ __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const 
         if (p > 0) {
 #pragma unroll
             for (int r = 1; r < 4; ++r) {
-                const cplx<T> w = tw[r * k * (64 / Ns)];
+                const cplx<T> w = ptw[ptw256_off(Ns) + (r - 1) * Ns + k];
                 a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
             }
         }
@@ -924,7 +951,8 @@ __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const 
 // Z_{N-k} through LDS) into P[L, 0:K, p].  Same arithmetic as k_sp_xmid with
 // a different radix order (4·4·4·4 instead of 8·8·4).  No workgroup
 // barrier after the twiddle staging.  One vmax partial per block.
-// LDS: the 256 transform and 257 real-transform twiddles, 4 × 320 complex.
+// LDS: the per-pass transform twiddles (252, in 256 slots), the 257
+// real-transform twiddles, 4 × 320 complex.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
 {
@@ -935,7 +963,7 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
     cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * (N + N / 4); // padded exchange lines
     __shared__ double red[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = a.tw[t];
+    stage_ptw256<T>(tw, a.tw);
     for (int t = threadIdx.x; t <= N; t += blockDim.x) twx[t] = a.twx[t];
     __syncthreads();
     const int K = g.K, n = g.n, nlines = n * n;
