@@ -312,9 +312,10 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
 // (r = 0..7) on entry and X[t + 64 r] on return; Stockham radix 8 in three
 // passes (Ns = 1, 8, 64) with the exchanges through the line's LDS storage
 // sh, as group_fft.  tw[t] = exp(-2πi t / 512); unnormalised.
-// PAD: exchange element i sits at sh[i + i/8] (sh then spans 575 elements):
-// the first pass writes element 8 t + r from lane t, which without the pad
-// puts every other lane on the same LDS banks.
+// The exchanges keep element i at sh[i ^ ((i / 8) mod 16)], a bijection of
+// [0, 512): the first pass writes element 8 t + r from lane t, which would
+// put every other lane on the same LDS banks; swizzled, every sixteen lanes
+// of each exchange write and read sixteen distinct 16-byte bank groups.
 // PTW: tw is the per-pass table of stage_ptw512 (w^(r k 64 / Ns) at
 // ptw[o_p + (r - 1) Ns + k]), read by the lanes of a pass at consecutive
 // entries; indexing w^t by r k 64 / Ns puts the lanes of the Ns = 8 pass on
@@ -330,10 +331,10 @@ template <typename T> __device__ inline const cplx<T> *stage_ptw512(cplx<T> *dst
     return dst;
 }
 
-template <typename T, bool INV, bool PAD = false, bool PTW = false>
+template <typename T, bool INV, bool PTW = false>
 __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const cplx<T> *tw, int t)
 {
-    auto x = [](int i) { return PAD ? i + (i >> 3) : i; };
+    auto x = [](int i) { return i ^ ((i >> 3) & 15); };
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const int Ns = 1 << (3 * p);
@@ -362,9 +363,9 @@ __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const 
 // by wavefront lane groups: G = N/4 lanes per line (64 for N = 512, 8 points
 // per lane), 64/G lines per wavefront at a time, the wavefronts of the block
 // taking lines in turn.  Needs nlines % (64/G) == 0.  Ends with a workgroup
-// barrier, as fft_lds.  PAD (N = 512): padded exchanges, needs ls >= 575;
-// PTW (N = 512): tw is the per-pass table of stage_ptw512.
-template <typename T, bool INV, int N, int NT = 0, bool PAD = false, bool PTW = false>
+// barrier, as fft_lds.  PTW (N = 512): tw is the per-pass table of
+// stage_ptw512.
+template <typename T, bool INV, int N, int NT = 0, bool PTW = false>
 __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls, const cplx<T> *tw)
 {
     constexpr int G = N == 512 ? 64 : N / 4, LPW = 64 / G;
@@ -377,7 +378,7 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 #pragma unroll
             for (int r = 0; r < 8; ++r) a[r] = line[t + 64 * r];
             wave_lds_sync();
-            wave_fft512<T, INV, PAD, PTW>(a, line, tw, t);
+            wave_fft512<T, INV, PTW>(a, line, tw, t);
 #pragma unroll
             for (int r = 0; r < 8; ++r) line[t + 64 * r] = a[r];
         } else {
@@ -397,15 +398,13 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 // The batched LDS transforms of a tile: lane groups for the lengths they
 // serve (16 ... 512, powers of two, nlines a multiple of the lines per
 // wavefront), otherwise fft_lds over sub-batches of at most sub lines.
-// PAD512: the 512-point lines use padded exchanges (the caller's line
-// stride ls is then at least 575); PTW512: for n = 512, tw is the per-pass
-// table of stage_ptw512.
-template <typename T, int EPT, bool INV, int NT = 0, bool PAD512 = false, bool PTW512 = false>
+// PTW512: for n = 512, tw is the per-pass table of stage_ptw512.
+template <typename T, int EPT, bool INV, int NT = 0, bool PTW512 = false>
 __device__ __forceinline__ void fft_lds_tile(cplx<T> *buf, int nlines, int ls, const FFTDesc &d,
                                              const cplx<T> *tw, int sub)
 {
     const int n = d.n;
-    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PAD512, PTW512>(buf, nlines, ls, tw);
+    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PTW512>(buf, nlines, ls, tw);
     if (n == 256) return fft_lds_groups<T, INV, 256, NT>(buf, nlines, ls, tw);
     if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128, NT>(buf, nlines, ls, tw);
     if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64, NT>(buf, nlines, ls, tw);

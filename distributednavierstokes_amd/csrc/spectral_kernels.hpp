@@ -36,15 +36,14 @@ struct SpecGeom {
     double kfac; // 2π / l
 };
 
-// LDS line stride (elements) of the y/z pencil tiles.  FP64 512-point lines
-// take 580: room for the padded transform exchanges (575), and a stride of
-// 16 banks between the lines of a tile, so the tile's row-wise stores and
-// reads (line fastest across lanes) fall on distinct banks.
+// LDS line stride (elements) of the y/z pencil tiles.  512-point lines take
+// 516 (a stride of 16 banks in FP64, 8 in FP32, between the lines of a tile),
+// so the tile's row-wise stores and reads (line fastest across the lanes)
+// fall on distinct banks; n + 1 otherwise.
 template <typename T> __host__ __device__ constexpr int sp_line_stride(int n)
 {
-    return sizeof(T) == 8 && n == 512 ? 580 : n + 1;
+    return n == 512 ? 516 : n + 1;
 }
-#define SP_PAD512(T) (sizeof(T) == 8)
 
 // The transform twiddles of a pencil tile in LDS: for 512-point lines
 // transformed by lane groups (GROUPS) the per-pass table of stage_ptw512,
@@ -94,7 +93,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
@@ -128,7 +127,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, true, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
         if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
@@ -159,7 +158,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int kc = fdiv(idx, ftb), b = idx - kc * tb, i = i0 + b;
         if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
@@ -191,7 +190,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false, 0, SP_PAD512(T), true>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
     const T inv = T(1.0 / ((double)n * n * n));
     for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
         int jc = fdiv(idx, ftb), b = idx - jc * tb, i = i0 + b;
@@ -529,7 +528,7 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
         if constexpr (sizeof(T) == 4)
             fft_lds<T, EPT, false, NT>(buf, tb, ls, d, twl);
         else
-            fft_lds_tile<T, EPT, false, NT, false, true>(buf, tb, ls, d, twl, tb);
+            fft_lds_tile<T, EPT, false, NT, true>(buf, tb, ls, d, twl, tb);
         // the output indices are recomputed for each product rather than
         // kept live across the loop (registers)
         int tid = threadIdx.x;
@@ -886,10 +885,12 @@ __global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *s
 // (j div Ns) 4 Ns + k + r Ns) with j = l: pass 0 reads the registers as
 // loaded, passes 1-3 read the previous pass's outputs from the wavefront's
 // 256-entry LDS buffer sh, and the outputs of pass 3 land at l + 64 r, in
-// the registers; unnormalised.  The exchanges keep element i at sh[i + i/4]
-// (sh spans 319 elements): the writes of passes 0 and 1 (elements 4 l + r
-// and 16 (l div 4) + l mod 4 + 4 r) would otherwise put four lanes of every
-// sixteen on the same LDS banks.  The twiddles come per pass, ptw (from
+// the registers; unnormalised.  The exchanges keep element i at
+// sh[i ^ ((i / 4) mod 16)], a bijection of [0, 256): the writes of passes 0
+// and 1 (elements 4 l + r and 16 (l div 4) + l mod 4 + 4 r) would otherwise
+// put four lanes of every sixteen on the same LDS banks; swizzled, every
+// sixteen lanes of each exchange access sixteen distinct 16-byte bank
+// groups.  The twiddles come per pass, ptw (from
 // stage_ptw256): w^(r k 64 / Ns) at ptw[o_p + (r - 1) Ns + k], so the lanes of
 // a pass read consecutive entries (the table w^t indexed by r k 64 / Ns
 // strides the lanes over the banks 4 to 8 ways).
@@ -910,7 +911,7 @@ template <typename T> __device__ inline void stage_ptw256(cplx<T> *ptw, const cp
 template <typename T, bool INV>
 __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *ptw, int lane_)
 {
-    auto x = [](int i) { return i + (i >> 2); };
+    auto x = [](int i) { return i ^ ((i >> 2) & 15); };
     // the exchange addresses are recomputed at each call rather than kept
     // live between the nine transforms of a line (registers)
     int lane = lane_;
@@ -952,7 +953,7 @@ __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const 
 // a different radix order (4·4·4·4 instead of 8·8·4).  No workgroup
 // barrier after the twiddle staging.  One vmax partial per block.
 // LDS: the per-pass transform twiddles (252, in 256 slots), the 257
-// real-transform twiddles, 4 × 320 complex.
+// real-transform twiddles, 4 × 256 complex.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
 {
@@ -960,7 +961,7 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *tw = reinterpret_cast<cplx<T> *>(smem_raw);
     cplx<T> *twx = tw + N;
-    cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * (N + N / 4); // padded exchange lines
+    cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * N;
     __shared__ double red[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     stage_ptw256<T>(tw, a.tw);
