@@ -263,14 +263,49 @@ __device__ __forceinline__ void wave_lds_sync()
 // is 2·4^a, whose two butterflies per lane (j = t, t + G) take a[0], a[2] and
 // a[1], a[3].  Between passes the outputs go through the group's N-entry LDS
 // line sh; the last pass's outputs land at t + r G, in the registers.
-// tw[t] = exp(-2πi t / N); unnormalised.
-template <typename T, bool INV, int N>
+// The exchanges keep element i at sh[i ^ ((i / 4) mod 16)], a bijection of
+// [0, N) under which every sixteen lanes of each exchange access sixteen
+// distinct 16-byte LDS bank groups (unswizzled, the writes of the first
+// passes put four lanes of every sixteen on the same banks).
+// tw[t] = exp(-2πi t / N), or with PTW the per-pass table of
+// stage_group_twiddles; unnormalised.
+template <int N> struct GroupFFT {
+    static constexpr int G = N / 4;
+    static constexpr int P4 = (N == 16 || N == 32) ? 2 : (N == 64 || N == 128) ? 3 : 4; // radix-4 passes
+    static constexpr bool R2 = (N == 32 || N == 128);
+    // per-pass table: w^(r k N / (4 Ns)) of pass p >= 1 (Ns = 4^p) at
+    // off(Ns) + (r - 1) Ns + k, then (R2) w^j, j < N/2, at r2off
+    static constexpr __host__ __device__ int off(int Ns) { return Ns == 4 ? 0 : Ns == 16 ? 12 : 60; }
+    static constexpr int r2off = (1 << (2 * P4)) - 4;
+    static constexpr int len = r2off + (R2 ? N / 2 : 0);
+};
+
+// Stages the per-pass twiddle table of group_fft<N> into LDS from
+// tw[t] = exp(-2πi t / N), t < N (the same values, rearranged so that the
+// lanes of a pass read consecutive entries).
+template <typename T, int N> __device__ inline const cplx<T> *stage_group_twiddles(cplx<T> *dst, const cplx<T> *tw)
+{
+    using F = GroupFFT<N>;
+    for (int e = threadIdx.x; e < F::len; e += blockDim.x) {
+        if (e >= F::r2off) {
+            dst[e] = tw[e - F::r2off];
+        } else {
+            const int Ns = e < 12 ? 4 : e < 60 ? 16 : 64;
+            const int j = e - F::off(Ns), r = j / Ns + 1, k = j - (r - 1) * Ns;
+            dst[e] = tw[r * k * (N / (4 * Ns))];
+        }
+    }
+    return dst;
+}
+
+template <typename T, bool INV, int N, bool PTW = false>
 __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw, int t)
 {
-    constexpr int G = N / 4;
-    constexpr int P4 = (N == 16 || N == 32) ? 2 : (N == 64 || N == 128) ? 3 : 4; // radix-4 passes
-    constexpr bool R2 = (N == 32 || N == 128 || N == 512);
+    using F = GroupFFT<N>;
+    constexpr int G = F::G, P4 = F::P4;
+    constexpr bool R2 = F::R2;
     static_assert(N == 16 || N == 32 || N == 64 || N == 128 || N == 256, "group_fft: N not supported");
+    auto x = [](int i) { return i ^ ((i >> 2) & 15); };
 #pragma unroll
     for (int p = 0; p < P4; ++p) {
         const int Ns = 1 << (2 * p);
@@ -278,7 +313,7 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
         if (p > 0) {
 #pragma unroll
             for (int r = 1; r < 4; ++r) {
-                const cplx<T> w = tw[r * k * (N / (4 * Ns))];
+                const cplx<T> w = PTW ? tw[F::off(Ns) + (r - 1) * Ns + k] : tw[r * k * (N / (4 * Ns))];
                 a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
             }
         }
@@ -286,10 +321,10 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
         if (p < P4 - 1 || R2) {
             const int base = (t - k) * 4 + k;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sh[base + r * Ns] = a[r];
+            for (int r = 0; r < 4; ++r) sh[x(base + r * Ns)] = a[r];
             wave_lds_sync();
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = sh[t + G * r];
+            for (int r = 0; r < 4; ++r) a[r] = sh[x(t + G * r)];
             wave_lds_sync();
         }
     }
@@ -299,7 +334,7 @@ __device__ __forceinline__ void group_fft(cplx<T> (&a)[4], cplx<T> *sh, const cp
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             cplx<T> b[2] = {a[h], a[h + 2]};
-            const cplx<T> w = tw[t + h * G];
+            const cplx<T> w = tw[(PTW ? F::r2off : 0) + t + h * G];
             b[1] = INV ? cmulc<T>(b[1], w) : cmul<T>(b[1], w);
             Dft<T, 2, INV>::run(b);
             a[h] = b[0];
@@ -363,8 +398,8 @@ __device__ __forceinline__ void wave_fft512(cplx<T> (&a)[8], cplx<T> *sh, const 
 // by wavefront lane groups: G = N/4 lanes per line (64 for N = 512, 8 points
 // per lane), 64/G lines per wavefront at a time, the wavefronts of the block
 // taking lines in turn.  Needs nlines % (64/G) == 0.  Ends with a workgroup
-// barrier, as fft_lds.  PTW (N = 512): tw is the per-pass table of
-// stage_ptw512.
+// barrier, as fft_lds.  PTW: tw is the per-pass table (stage_ptw512 for
+// N = 512, stage_group_twiddles otherwise).
 template <typename T, bool INV, int N, int NT = 0, bool PTW = false>
 __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls, const cplx<T> *tw)
 {
@@ -386,7 +421,7 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = line[t + G * r];
             wave_lds_sync();
-            group_fft<T, INV, N>(a, line, tw, t);
+            group_fft<T, INV, N, PTW>(a, line, tw, t);
 #pragma unroll
             for (int r = 0; r < 4; ++r) line[t + G * r] = a[r];
         }
@@ -398,16 +433,37 @@ __device__ __forceinline__ void fft_lds_groups(cplx<T> *buf, int nlines, int ls,
 // The batched LDS transforms of a tile: lane groups for the lengths they
 // serve (16 ... 512, powers of two, nlines a multiple of the lines per
 // wavefront), otherwise fft_lds over sub-batches of at most sub lines.
-// PTW512: for n = 512, tw is the per-pass table of stage_ptw512.
-template <typename T, int EPT, bool INV, int NT = 0, bool PTW512 = false>
+// Whether fft_lds_tile transforms nlines lines of n points by lane groups.
+__device__ inline bool fft_tile_groups(int n, int nlines)
+{
+    return n == 512 || n == 256 || (n == 128 && nlines % 2 == 0) || (n == 64 && nlines % 4 == 0);
+}
+
+// The twiddles fft_lds_tile<..., PTW = true> reads, staged into LDS at dst
+// from tw[t] = exp(-2πi t / n): the per-pass table of the lane-group
+// transform when it serves (n, nlines), otherwise tw itself (n entries).
+template <typename T> __device__ inline const cplx<T> *stage_tile_twiddles(cplx<T> *dst, const cplx<T> *tw, int n, int nlines)
+{
+    if (!fft_tile_groups(n, nlines)) return stage_twiddles<T>(dst, tw, n);
+    switch (n) {
+        case 512: return stage_ptw512<T>(dst, tw);
+        case 256: return stage_group_twiddles<T, 256>(dst, tw);
+        case 128: return stage_group_twiddles<T, 128>(dst, tw);
+        default: return stage_group_twiddles<T, 64>(dst, tw);
+    }
+}
+
+// PTW: tw comes from stage_tile_twiddles (per-pass tables for the lane-group
+// lengths), else it is exp(-2πi t / n), t < n.
+template <typename T, int EPT, bool INV, int NT = 0, bool PTW = false>
 __device__ __forceinline__ void fft_lds_tile(cplx<T> *buf, int nlines, int ls, const FFTDesc &d,
                                              const cplx<T> *tw, int sub)
 {
     const int n = d.n;
-    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PTW512>(buf, nlines, ls, tw);
-    if (n == 256) return fft_lds_groups<T, INV, 256, NT>(buf, nlines, ls, tw);
-    if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128, NT>(buf, nlines, ls, tw);
-    if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64, NT>(buf, nlines, ls, tw);
+    if (n == 512) return fft_lds_groups<T, INV, 512, NT, PTW>(buf, nlines, ls, tw);
+    if (n == 256) return fft_lds_groups<T, INV, 256, NT, PTW>(buf, nlines, ls, tw);
+    if (n == 128 && nlines % 2 == 0) return fft_lds_groups<T, INV, 128, NT, PTW>(buf, nlines, ls, tw);
+    if (n == 64 && nlines % 4 == 0) return fft_lds_groups<T, INV, 64, NT, PTW>(buf, nlines, ls, tw);
     for (int sb = 0; sb < nlines; sb += sub)
         fft_lds<T, EPT, INV, NT>(buf + (size_t)sb * ls, min(sub, nlines - sb), ls, d, tw);
 }

@@ -627,8 +627,9 @@ __global__ void __launch_bounds__(256) k_irfft_x(XInvArgs<T> a, FVGeom g, FFTDes
 // for the exchanges (group_fft), so no workgroup barrier follows the twiddle
 // staging.  Same arithmetic as k_rfft_x<MODE 0> / k_irfft_x with the radix
 // order 4·…·4(·2) instead of 8·…
-// LDS: the N transform and N+1 real-transform twiddles, then N complex per
-// line (256/G lines per block).
+// LDS: the per-pass transform twiddles (stage_group_twiddles, at most N) and
+// the N+1 real-transform twiddles, then N complex per line (256/G lines per
+// block).
 // ------------------------------------------------------------------------
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_rfft_xw(XPassArgs<T> a, FVGeom g)
@@ -639,7 +640,7 @@ __global__ void __launch_bounds__(256) k_rfft_xw(XPassArgs<T> a, FVGeom g)
     cplx<T> *twx = tw + N;
     const int t = threadIdx.x % G, lb = threadIdx.x / G;
     cplx<T> *sh = twx + (N + 1) + lb * N;
-    for (int q = threadIdx.x; q < N; q += blockDim.x) tw[q] = a.tw[q];
+    stage_group_twiddles<T, N>(tw, a.tw);
     for (int q = threadIdx.x; q <= N; q += blockDim.x) twx[q] = a.twx[q];
     const int nlines = g.ny * g.nz;
     const int L = xcd_block(blockIdx.x, gridDim.x) * LPB + lb;
@@ -669,7 +670,7 @@ __global__ void __launch_bounds__(256) k_rfft_xw(XPassArgs<T> a, FVGeom g)
         z[r].y = (X.y - X.x) * a.rdx + y1 * rdyj + z1 * a.rdz;
     }
     __syncthreads(); // the twiddles are staged
-    group_fft<T, false, N>(z, sh, tw, t);
+    group_fft<T, false, N, true>(z, sh, tw, t);
 #pragma unroll
     for (int r = 0; r < 4; ++r) sh[t + r * G] = z[r];
     wave_lds_sync();
@@ -700,7 +701,7 @@ __global__ void __launch_bounds__(256) k_irfft_xw(XInvArgs<T> a, FVGeom g)
     cplx<T> *twx = tw + N;
     const int t = threadIdx.x % G, lb = threadIdx.x / G;
     cplx<T> *sh = twx + (N + 1) + lb * N;
-    for (int q = threadIdx.x; q < N; q += blockDim.x) tw[q] = a.tw[q];
+    stage_group_twiddles<T, N>(tw, a.tw);
     for (int q = threadIdx.x; q <= N; q += blockDim.x) twx[q] = a.twx[q];
     const int nlines = g.ny * g.nz;
     const int L = xcd_block(blockIdx.x, gridDim.x) * LPB + lb;
@@ -728,7 +729,7 @@ __global__ void __launch_bounds__(256) k_irfft_xw(XInvArgs<T> a, FVGeom g)
         const cplx<T> od = cmulc<T>(csub<T>(Xk, Xc), twx[k]); // 2 Od = w^-k (Xk − Xc)
         z[r] = mk<T>(ev.x - od.y, ev.y + od.x);
     }
-    group_fft<T, true, N>(z, sh, tw, t);
+    group_fft<T, true, N, true>(z, sh, tw, t);
     if (live) {
         cplx<T> *out = reinterpret_cast<cplx<T> *>(a.out + (size_t)g.nx * L);
 #pragma unroll
@@ -747,7 +748,7 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int kx0 = bxr * tb, j = byr;
@@ -762,7 +763,7 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, INV>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, INV, 0, true>(buf, tb, ls, d, twl, tbf);
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb;
         int kx = kx0 + b;
@@ -785,7 +786,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = d.n, ls = n + 1;
-    const cplx<T> *twl = stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int kx0 = bxr * tb, kz = byr;
@@ -800,7 +801,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = v;
         });
     __syncthreads();
-    fft_lds_tile<T, EPT, false>(buf, tb, ls, d, twl, tbf);
+    fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
     if (DIVIDE) {
         for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
             int t = fdiv(idx, ftb), b = idx - t * tb;
@@ -812,7 +813,7 @@ __global__ void __launch_bounds__(512) k_fourier_y(cplx<T> *A, const cplx<T> *tw
             buf[b * ls + t] = mk<T>(T((double)v.x * f), T((double)v.y * f));
         }
         __syncthreads();
-        fft_lds_tile<T, EPT, true>(buf, tb, ls, d, twl, tbf);
+        fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
     }
     for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
         int t = fdiv(idx, ftb), b = idx - t * tb;

@@ -45,14 +45,7 @@ template <typename T> __host__ __device__ constexpr int sp_line_stride(int n)
     return n == 512 ? 516 : n + 1;
 }
 
-// The transform twiddles of a pencil tile in LDS: for 512-point lines
-// transformed by lane groups (GROUPS) the per-pass table of stage_ptw512,
-// otherwise w^t, t < n.
-template <typename T, bool GROUPS = true>
-__device__ inline const cplx<T> *sp_stage_twiddles(cplx<T> *dst, const cplx<T> *tw, int n)
-{
-    return GROUPS && n == 512 ? stage_ptw512<T>(dst, tw) : stage_twiddles<T>(dst, tw, n);
-}
+
 
 __device__ inline int sp_fine(int c, const SpecGeom &g) { return c <= g.kcut ? c : c + (g.n - g.m); }
 __device__ inline int sp_freq(int c, const SpecGeom &g) { return c <= g.kcut ? c : c - g.m; }
@@ -75,7 +68,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kz = byr, c = bzr;
@@ -109,7 +102,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, c = bzr;
@@ -143,7 +136,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, p = bzr;
@@ -175,7 +168,7 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m;
-    const cplx<T> *twl = sp_stage_twiddles<T>(buf + (size_t)tb * ls, tw, n);
+    const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kc = byr, p = bzr;
@@ -478,8 +471,9 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m, tb = s.tb;
     const FastDiv ftb = s.ftb;
-    // FP32 transforms with the block-wide engine (natural table), FP64 with lane groups
-    const cplx<T> *twl = sp_stage_twiddles<T, sizeof(T) == 8>(buf + (size_t)tb * ls, s.tw, n);
+    // FP64 512-point lines: the per-pass table of the wavefront transform
+    const cplx<T> *twl = sizeof(T) == 8 && n == 512 ? stage_ptw512<T>(buf + (size_t)tb * ls, s.tw)
+                                                    : stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int i0 = bxr * tb, kc = byr;
@@ -524,11 +518,13 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
         if (p < 5) load_tile(p + 1);
         // FP32: the block-wide engine (the lane-group transforms measured
         // slower here, 3.5 -> 4.3 ms per HIT 512 step, at 16 elements per
-        // thread); FP64: lane groups.
-        if constexpr (sizeof(T) == 4)
-            fft_lds<T, EPT, false, NT>(buf, tb, ls, d, twl);
+        // thread); FP64: wavefront transforms for 512-point lines (the
+        // other lane-group lengths are not instantiated here: their code
+        // would raise the register count of the kernel past its budget).
+        if (sizeof(T) == 8 && n == 512)
+            fft_lds_groups<T, false, 512, NT, true>(buf, tb, ls, twl);
         else
-            fft_lds_tile<T, EPT, false, NT, true>(buf, tb, ls, d, twl, tb);
+            fft_lds<T, EPT, false, NT>(buf, tb, ls, d, twl);
         // the output indices are recomputed for each product rather than
         // kept live across the loop (registers)
         int tid = threadIdx.x;
