@@ -163,10 +163,15 @@ template <typename T, bool INV> struct Dft<T, 5, INV> {
 
 // --------------------------------------------------------------- one pass
 
-template <typename T, int R, int EPT, bool INV>
+// O512 (512-point lines, radix 8): element i of a line sits at
+// i ^ ((i / 8) mod 32), under which the 32 lanes of each pass's stride-8
+// writes hit distinct LDS banks, and tw is the per-pass table of
+// stage_ptw512.
+template <typename T, int R, int EPT, bool INV, bool O512 = false>
 __device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns, FastDiv dm, FastDiv dns,
                                 const cplx<T> *__restrict__ tw, int tid, int nthr)
 {
+    auto x = [](int i) { return O512 ? i ^ ((i >> 3) & 31) : i; };
     constexpr int BPT = (EPT + R - 1) / R;
     const int m = n / R;
     const int total = nlines * m;
@@ -177,9 +182,9 @@ __device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns,
         if (t < total) {
             int line = fdiv(t, dm);
             int j = t - line * m;
-            const cplx<T> *src = buf + line * ls + j;
+            const cplx<T> *src = buf + line * ls;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[s][r] = src[r * m];
+            for (int r = 0; r < R; ++r) v[s][r] = src[x(j + r * m)];
         }
     }
     __syncthreads();
@@ -195,14 +200,15 @@ __device__ inline void fft_pass(cplx<T> *buf, int nlines, int ls, int n, int Ns,
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    cplx<T> w = tw[r * k * tws];
+                    cplx<T> w = O512 ? tw[(Ns == 8 ? 0 : 56) + (r - 1) * Ns + k] : tw[r * k * tws];
                     v[s][r] = INV ? cmulc<T>(v[s][r], w) : cmul<T>(v[s][r], w);
                 }
             }
             Dft<T, R, INV>::run(v[s]);
-            cplx<T> *dst = buf + line * ls + jd * Ns * R + k;
+            cplx<T> *dst = buf + line * ls;
+            const int base = jd * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * Ns] = v[s][r];
+            for (int r = 0; r < R; ++r) dst[x(base + r * Ns)] = v[s][r];
         }
     }
     __syncthreads();
@@ -222,16 +228,24 @@ __device__ inline const cplx<T> *stage_twiddles(cplx<T> *dst, const cplx<T> *tw,
 // tw[t] = exp(-2πi t / n) for t in [0, n).  Unnormalised in both directions.
 // Must be called by every thread of the block (contains barriers).
 // NT > 0: the block size is known at compile time (blockDim.x is then not
-// read, which would be a memory load).
-template <typename T, int EPT, bool INV, int NT = 0>
+// read, which would be a memory load).  O512: when d.n = 512 the lines are
+// stored swizzled (element i at i ^ ((i / 8) mod 32)) and tw is the per-pass
+// table of stage_ptw512 (the caller stages and addresses the lines so).
+template <typename T, int EPT, bool INV, int NT = 0, bool O512 = false>
 __device__ inline void fft_lds(cplx<T> *buf, int nlines, int ls, const FFTDesc &d, const cplx<T> *__restrict__ tw)
 {
     const int tid = threadIdx.x, nthr = NT > 0 ? NT : (int)blockDim.x;
+    const bool o512 = O512 && d.n == 512;
     int Ns = 1;
     for (int p = 0; p < d.npass; ++p) {
         int R = d.radix[p];
         switch (R) {
-            case 8: fft_pass<T, 8, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
+            case 8:
+                if (o512)
+                    fft_pass<T, 8, EPT, INV, O512>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr);
+                else
+                    fft_pass<T, 8, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr);
+                break;
             case 4: fft_pass<T, 4, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
             case 2: fft_pass<T, 2, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;
             case 3: fft_pass<T, 3, EPT, INV>(buf, nlines, ls, d.n, Ns, d.dm[p], d.dns[p], tw, tid, nthr); break;

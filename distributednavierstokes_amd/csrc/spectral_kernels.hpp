@@ -471,9 +471,11 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
     const int n = g.n, ls = sp_line_stride<T>(n), K = g.K, m = g.m, tb = s.tb;
     const FastDiv ftb = s.ftb;
-    // FP64 512-point lines: the per-pass table of the wavefront transform
-    const cplx<T> *twl = sizeof(T) == 8 && n == 512 ? stage_ptw512<T>(buf + (size_t)tb * ls, s.tw)
-                                                    : stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
+    // 512-point lines: the per-pass twiddle table (both transform engines
+    // read it); FP32 512-point lines are also stored swizzled (fft_lds O512)
+    const cplx<T> *twl = n == 512 ? stage_ptw512<T>(buf + (size_t)tb * ls, s.tw)
+                                  : stage_twiddles<T>(buf + (size_t)tb * ls, s.tw, n);
+    auto lsw = [n](int t) { return sizeof(T) == 4 && n == 512 ? t ^ ((t >> 3) & 31) : t; };
     int bxr, byr;
     xcd_block2(bxr, byr);
     const int i0 = bxr * tb, kc = byr;
@@ -511,7 +513,7 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
             const int idx = threadIdx.x + q * NT;
             if (idx < tb * n) {
                 int t = fdiv(idx, ftb), b = idx - t * tb;
-                buf[b * ls + t] = pre[q];
+                buf[b * ls + lsw(t)] = pre[q];
             }
         }
         __syncthreads();
@@ -524,7 +526,7 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
         if (sizeof(T) == 8 && n == 512)
             fft_lds_groups<T, false, 512, NT, true>(buf, tb, ls, twl);
         else
-            fft_lds<T, EPT, false, NT>(buf, tb, ls, d, twl);
+            fft_lds<T, EPT, false, NT, sizeof(T) == 4>(buf, tb, ls, d, twl);
         // the output indices are recomputed for each product rather than
         // kept live across the loop (registers)
         int tid = threadIdx.x;
@@ -536,9 +538,9 @@ __global__ void __launch_bounds__(NT, 2) k_sp_ystage(StageArgs<T> s, SpecGeom g,
                 int jc = fdiv(oi, ftb), b = oi - jc * tb, i = i0 + b;
                 if (i < K && MODE == 2) {
                     s.du[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))] =
-                        cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
+                        cscale<T>(buf[b * ls + lsw(sp_fine(jc, g))], inv);
                 } else if (i < K) {
-                    cplx<T> sg = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
+                    cplx<T> sg = cscale<T>(buf[b * ls + lsw(sp_fine(jc, g))], inv);
                     if (s.extra) sg = cadd<T>(sg, s.extra[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))]);
                     T kx = T(g.kfac * i), ky = T(g.kfac * sp_freq(jc, g));
                     // −i k σ  =  (k σ.y, −k σ.x)
