@@ -269,7 +269,7 @@ template <typename T> struct SpecImpl : SpecBase {
         timer.begin(SFAM_X);
         if (MODE == XMID_RHS && xr_wave)
             hipLaunchKernelGGL((k_sp_xrhs256<T>), dim3(ceil_div((long)g.n * g.n, 4)), dim3(SP_NTHR),
-                               (size_t)(256 + 257 + 4 * 256) * sizeof(cplx<T>), st, a, g);
+                               (size_t)(TW4S_LEN + 257 + 4 * 256) * sizeof(cplx<T>), st, a, g);
         else if (MODE == XMID_RHS && xr_lpi)
             hipLaunchKernelGGL((k_sp_xrhs<T, SP_XEPT>), dim3(xr_blocks), dim3(SP_NTHR), ldsxr, st, a, g, fdh, xr_lpi);
         else

@@ -877,65 +877,97 @@ __global__ void __launch_bounds__(256) k_les_tau(cplx<T> *tauL, const cplx<T> *s
 // ------------------------------------------- wavefront x pass (n = 512)
 
 // 256-point complex FFT by one wavefront: lane l holds a[r] = x[l + 64 r] on
-// entry and X[l + 64 r] on return.  Stockham radix-4 in four passes, the
-// same schedule as fft_lds (pass with sub-transform size Ns: inputs
-// x[j + 64 r] twiddled by w^(r k 64 / Ns), k = j mod Ns, outputs at
-// (j div Ns) 4 Ns + k + r Ns) with j = l: pass 0 reads the registers as
-// loaded, passes 1-3 read the previous pass's outputs from the wavefront's
-// 256-entry LDS buffer sh, and the outputs of pass 3 land at l + 64 r, in
-// the registers; unnormalised.  The exchanges keep element i at
-// sh[i ^ ((i / 4) mod 16)], a bijection of [0, 256): the writes of passes 0
-// and 1 (elements 4 l + r and 16 (l div 4) + l mod 4 + 4 r) would otherwise
-// put four lanes of every sixteen on the same LDS banks; swizzled, every
-// sixteen lanes of each exchange access sixteen distinct 16-byte bank
-// groups.  The twiddles come per pass, ptw (from
-// stage_ptw256): w^(r k 64 / Ns) at ptw[o_p + (r - 1) Ns + k], so the lanes of
-// a pass read consecutive entries (the table w^t indexed by r k 64 / Ns
-// strides the lanes over the banks 4 to 8 ways).
-constexpr int PTW256_LEN = 3 * (4 + 16 + 64);
-__device__ __forceinline__ int ptw256_off(int Ns) { return Ns == 4 ? 0 : Ns == 16 ? 12 : 60; }
-
-// Stages the per-pass twiddle table of wave_fft256 into LDS from
-// tw[t] = exp(-2πi t / 256), t < 256 (the same values, rearranged).
-template <typename T> __device__ inline void stage_ptw256(cplx<T> *ptw, const cplx<T> *tw)
+// entry and X[l + 64 r] on return; unnormalised (w = exp(∓2πi/256)).  A
+// four-step 16 x 16 transform: element n = l + 64 r splits as n = na + 16 nb with
+// na = l mod 16 and nb = l div 16 + 4 r, and X[kb + 16 ka] =
+// Σ_na w16^(na ka) w^(na kb) Σ_nb x[na + 16 nb] w16^(nb kb).  Each 16-point
+// DFT (fft16_rows) is a radix-4 over the registers, the twiddles
+// w16^(h k) (h = the lane's row l div 16), a transpose of the four lane rows
+// with the four registers (permlane32_swap on register pairs (0, 2), (1, 3),
+// then permlane16_swap on (0, 1), (2, 3): no LDS), and a radix-4 over the
+// registers again; between the two DFT stages the twiddles w^(na kb) and one
+// LDS exchange (element (kb, na) at sh[16 kb + (na ^ kb)], so that sixteen
+// lanes of each write and read hit sixteen distinct 16-byte bank groups)
+// bring the sixteen kb columns to the lanes.  One LDS exchange per
+// transform (a radix-4 Stockham schedule needs three; measured 5.28 → 4.57 ms
+// per HIT 512 FP64 step for the x pass).  tw4 (stage_tw4s): w^(na (q + 4 t))
+// at t 64 + l (q = l div 16), then w16^(h k) at 256 + 4 h + k, both from
+// tw[t] = exp(-2πi t / 256).
+constexpr int TW4S_LEN = 256 + 16;
+template <typename T> __device__ inline void stage_tw4s(cplx<T> *tw4, const cplx<T> *tw)
 {
-    for (int e = threadIdx.x; e < PTW256_LEN; e += blockDim.x) {
-        const int Ns = e < 12 ? 4 : e < 60 ? 16 : 64;
-        const int j = e - ptw256_off(Ns), r = j / Ns + 1, k = j - (r - 1) * Ns;
-        ptw[e] = tw[r * k * (64 / Ns)];
+    for (int e = threadIdx.x; e < TW4S_LEN; e += blockDim.x) {
+        if (e < 256) {
+            const int t = e >> 6, l = e & 63;
+            tw4[e] = tw[(l & 15) * ((l >> 4) + 4 * t)];
+        } else {
+            const int h = (e - 256) >> 2, k = (e - 256) & 3;
+            tw4[e] = tw[16 * h * k];
+        }
     }
 }
 
-template <typename T, bool INV>
-__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *ptw, int lane_)
+// Swaps of 32-bit words between the lane halves (X = 32) or between the odd
+// rows of a and the even rows of b (X = 16), word by word.
+template <int X, typename T> __device__ __forceinline__ void permlane_swap(cplx<T> &a, cplx<T> &b)
 {
-    auto x = [](int i) { return i ^ ((i >> 2) & 15); };
-    // the exchange addresses are recomputed at each call rather than kept
-    // live between the nine transforms of a line (registers)
+    constexpr int W = (int)(sizeof(cplx<T>) / 4);
+    using UV = unsigned __attribute__((ext_vector_type(W)));
+    UV ua = __builtin_bit_cast(UV, a), ub = __builtin_bit_cast(UV, b);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        auto r = X == 32 ? __builtin_amdgcn_permlane32_swap(ua[i], ub[i], false, false)
+                         : __builtin_amdgcn_permlane16_swap(ua[i], ub[i], false, false);
+        ua[i] = r[0];
+        ub[i] = r[1];
+    }
+    a = __builtin_bit_cast(cplx<T>, ua);
+    b = __builtin_bit_cast(cplx<T>, ub);
+}
+
+// 16-point DFTs down the columns of the wavefront: column c = l mod 16 holds
+// element h + 4 s in lane row h = l div 16, register s; on return lane row q,
+// register t holds X_c[q + 4 t].
+template <typename T, bool INV>
+__device__ __forceinline__ void fft16_rows(cplx<T> (&a)[4], const cplx<T> *tw16, int h)
+{
+    Dft<T, 4, INV>::run(a);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        const cplx<T> w = tw16[4 * h + k];
+        a[k] = INV ? cmulc<T>(a[k], w) : cmul<T>(a[k], w);
+    }
+    permlane_swap<32, T>(a[0], a[2]);
+    permlane_swap<32, T>(a[1], a[3]);
+    permlane_swap<16, T>(a[0], a[1]);
+    permlane_swap<16, T>(a[2], a[3]);
+    Dft<T, 4, INV>::run(a);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw4, int lane_)
+{
     int lane = lane_;
     asm volatile("" : "+v"(lane));
+    const int c = lane & 15, h = lane >> 4;
+    fft16_rows<T, INV>(a, tw4 + 256, h);
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const int Ns = 1 << (2 * p);
-        const int k = lane & (Ns - 1);
-        if (p > 0) {
-#pragma unroll
-            for (int r = 1; r < 4; ++r) {
-                const cplx<T> w = ptw[ptw256_off(Ns) + (r - 1) * Ns + k];
-                a[r] = INV ? cmulc<T>(a[r], w) : cmul<T>(a[r], w);
-            }
-        }
-        Dft<T, 4, INV>::run(a);
-        if (p < 3) {
-            const int base = (lane - k) * 4 + k;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) sh[x(base + r * Ns)] = a[r];
-            wave_lds_sync();
-#pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = sh[x(lane + 64 * r)];
-            wave_lds_sync();
-        }
+    for (int t = 0; t < 4; ++t) {
+        const cplx<T> w = tw4[64 * t + lane];
+        a[t] = INV ? cmulc<T>(a[t], w) : cmul<T>(a[t], w);
     }
+    // lane (na = c, row q = h) register t holds column kb = h + 4 t, row na
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int kb = h + 4 * t;
+        sh[16 * kb + (c ^ kb)] = a[t];
+    }
+    wave_lds_sync();
+    // lane (kb = c, row h) register s takes na = h + 4 s
+#pragma unroll
+    for (int s = 0; s < 4; ++s) a[s] = sh[16 * c + ((h + 4 * s) ^ c)];
+    wave_lds_sync();
+    fft16_rows<T, INV>(a, tw4 + 256, h);
 }
 
 // The RHS x pass (k_sp_xmid, XMID_RHS) for n = 512: one wavefront per
@@ -948,21 +980,21 @@ __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const 
 // packed as z_t = (p_2t, p_2t+1), transformed forward and unpacked
 // (P_k = ½[(Z_k + conj Z_{N-k}) − i w^k (Z_k − conj Z_{N-k})], the partner
 // Z_{N-k} through LDS) into P[L, 0:K, p].  Same arithmetic as k_sp_xmid with
-// a different radix order (4·4·4·4 instead of 8·8·4).  No workgroup
-// barrier after the twiddle staging.  One vmax partial per block.
-// LDS: the per-pass transform twiddles (252, in 256 slots), the 257
-// real-transform twiddles, 4 × 256 complex.
+// a different factorisation of the 256-point transforms (four-step 16 x 16
+// instead of 8·8·4).  No workgroup barrier after the twiddle staging.  One
+// vmax partial per block.  LDS: the transform twiddles (stage_tw4s, 272), the
+// 257 real-transform twiddles, 4 × 256 complex.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
 {
     constexpr int N = 256;
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *tw = reinterpret_cast<cplx<T> *>(smem_raw);
-    cplx<T> *twx = tw + N;
+    cplx<T> *twx = tw + TW4S_LEN;
     cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * N;
     __shared__ double red[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    stage_ptw256<T>(tw, a.tw);
+    stage_tw4s<T>(tw, a.tw);
     for (int t = threadIdx.x; t <= N; t += blockDim.x) twx[t] = a.twx[t];
     __syncthreads();
     const int K = g.K, n = g.n, nlines = n * n;
