@@ -125,6 +125,7 @@ template <typename T> struct FVImpl : FVBase {
     size_t ldst = 0;
     int cn_tl = 0, cn_nthr = 256; // x-lines and threads per block of the LDS CN kernel (0: sequential)
     int cn_seg = 0;               // rows per segment of the equal-segment CN kernel (0: not used)
+    int cn_fused = 0;             // the three components in one launch (k_cn_seg3): 1 pressure rows re-read, 2 staged in LDS
     int cn_nl = 1;                // x-lines per thread of the equal-segment CN kernel
     size_t ldscn = 0;
     double *Dcn = nullptr;
@@ -489,6 +490,19 @@ template <typename T> struct FVImpl : FVBase {
                         cn_nl = 2;
                         cn_tl *= 2;
                     }
+                }
+                // DNS_CN_FUSED=1/2: the three components in one launch (k_cn_seg3),
+                // the pressure rows re-read through L2 / staged once in LDS.  Measured
+                // on the 512x384x512 channel: FP64 +1-2 % steps/s with 2 (17 % fewer
+                // bytes at a lower bandwidth: two blocks per CU), mixed 4-5 % slower;
+                // one launch per component stays the default
+                {
+                    const char *ef = getenv("DNS_CN_FUSED");
+                    cn_fused = ef ? atoi(ef) : 0;
+                    if (!cn_seg || cn_nl != 1 || cn_fused < 0 || cn_fused > 2) cn_fused = 0;
+                    if (cn_fused == 2 &&
+                        (size_t)(3 + cn_seg + 1) * cn_nthr * sizeof(double) + (size_t)7 * ny * sizeof(double) > (size_t)maxlds)
+                        cn_fused = 1;
                 }
                 if (cn_tl && !cn_seg && (ny + cn_nthr / cn_tl - 1) / (cn_nthr / cn_tl) > CN_RMAX) {
                     set_error("CN tile %d:%d leaves more than %d rows per thread", cn_tl, cn_nthr, CN_RMAX);
@@ -862,6 +876,49 @@ template <typename T> struct FVImpl : FVBase {
         const double theta = alpha * dt * visc / 2;
         hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(NTHR), (size_t)6 * g.ny * sizeof(double), st, ydc[0], ydc[1], ydc[2],
                            ydc[3], ydc[4], ydc[5], theta, g.ny, cncoef);
+        if (cn_fused) {
+            CN3Args<T> a;
+            for (int c = 0; c < 3; ++c) {
+                a.u[c] = u[c];
+                a.q[c] = q[c];
+            }
+            a.p = p;
+            for (int set = 0; set < 2; ++set) {
+                a.ya[set] = ydc[3 * set + 0];
+                a.yb[set] = ydc[3 * set + 1];
+                a.yc[set] = ydc[3 * set + 2];
+                a.cp[set] = cncoef + set * 2 * g.ny;
+                a.ip[set] = a.cp[set] + g.ny;
+            }
+            a.rdyu = rdyu_d;
+            a.B = B;
+            a.adt = alpha * dt;
+            a.theta = theta;
+            a.rdx = 1.0 / dx;
+            a.rdz = 1.0 / dz;
+            const dim3 grd(ceil_div(g.nx, cn_tl), g.nz), blk(cn_nthr);
+            const size_t lds = (size_t)(3 + (cn_fused == 2 ? cn_seg + 1 : 0)) * cn_nthr * sizeof(double) +
+                               (size_t)7 * g.ny * sizeof(double);
+            timer.begin(FAM_CN);
+            if (cn_fused == 2) {
+                switch (cn_seg) {
+                    case 8: hipLaunchKernelGGL((k_cn_seg3<T, 8, true>), grd, blk, lds, st, a, g, cn_tl); break;
+                    case 16: hipLaunchKernelGGL((k_cn_seg3<T, 16, true>), grd, blk, lds, st, a, g, cn_tl); break;
+                    case 24: hipLaunchKernelGGL((k_cn_seg3<T, 24, true>), grd, blk, lds, st, a, g, cn_tl); break;
+                    default: hipLaunchKernelGGL((k_cn_seg3<T, 32, true>), grd, blk, lds, st, a, g, cn_tl); break;
+                }
+            } else {
+                switch (cn_seg) {
+                    case 8: hipLaunchKernelGGL((k_cn_seg3<T, 8, false>), grd, blk, lds, st, a, g, cn_tl); break;
+                    case 16: hipLaunchKernelGGL((k_cn_seg3<T, 16, false>), grd, blk, lds, st, a, g, cn_tl); break;
+                    case 24: hipLaunchKernelGGL((k_cn_seg3<T, 24, false>), grd, blk, lds, st, a, g, cn_tl); break;
+                    default: hipLaunchKernelGGL((k_cn_seg3<T, 32, false>), grd, blk, lds, st, a, g, cn_tl); break;
+                }
+            }
+            timer.end(FAM_CN);
+            DNS_HIP_CHECK(hipGetLastError());
+            return DNS_OK;
+        }
         for (int c = 0; c < 3; ++c) {
             CNArgs<T> a;
             a.u = u[c];
@@ -1114,6 +1171,7 @@ template <typename T> struct FVImpl : FVBase {
         else if (thomas_tl) snprintf(ys, sizeof ys, "tile tl=%d thr=%d", thomas_tl, thomas_nthr);
         else snprintf(ys, sizeof ys, "sequential");
         if (!imexy) snprintf(cs, sizeof cs, "none");
+        else if (cn_fused) snprintf(cs, sizeof cs, "seg3 R=%d tl=%d thr=%d p=%s", cn_seg, cn_tl, cn_nthr, cn_fused == 2 ? "lds" : "l2");
         else if (cn_tl && cn_seg) snprintf(cs, sizeof cs, "seg R=%d tl=%d thr=%d nl=%d", cn_seg, cn_tl, cn_nthr, cn_nl);
         else if (cn_tl) snprintf(cs, sizeof cs, "tile tl=%d thr=%d", cn_tl, cn_nthr);
         else snprintf(cs, sizeof cs, "sequential");

@@ -1723,6 +1723,210 @@ __global__ void __launch_bounds__(256, 2) k_cn_seg(CNArgs<T> a, FVGeom g, int TL
     }
 }
 
+// The three velocity components of the CN y-diffusion in one launch
+// (k_cn_seg with one x-line per thread, R rows per segment): the pressure
+// rows of the segment, p(k) for rows j0 .. j0 + R, are loaded once and kept
+// in LDS for all three components (the per-component launches read the
+// plane three times; in registers they would spill); each component then
+// loads its u and q rows, the neighbour pressure it needs (x + 1 for u, the
+// next row — already staged — for v, the k + 1 plane for w), and runs the same segmented
+// sweeps and chains as k_cn_seg, one after the other through the block's
+// LDS.  The y component uses the normal-row coefficient set, x and z the
+// tangential one: the LDS coefficient rows are restaged between components.
+// Same arithmetic in the same order as k_cn_seg for each component.
+// grid (ceil(nx / TL), nz); LDS 7 ny + (R + 4) blockDim doubles.
+template <typename T> struct CN3Args {
+    T *u[3];
+    const T *q[3];
+    const T *p;
+    const double *ya[2], *yb[2], *yc[2]; // Dy rows: set 0 (x, z), set 1 (y)
+    const double *cp[2], *ip[2];         // c'_j and 1/β_j of each set
+    const double *rdyu;                  // 1/Δyu (ghosted index)
+    double B, adt, theta, rdx, rdz;
+};
+
+template <typename T, int COMP, int R, bool PKL>
+__device__ __forceinline__ void cn3_component(const CN3Args<T> &a, const FVGeom &g, const double *pk, int TL,
+                                              double *As, double *Bs, double *Xin, const double *ca, const double *cb,
+                                              const double *cc, const double *cf, const double *ci, const double *cq,
+                                              const double *cr)
+{
+    constexpr int CH = R % 8 == 0 ? 8 : R;
+    const int ny = g.ny, S = blockDim.x / TL;
+    const int l = threadIdx.x % TL, sg = threadIdx.x / TL;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int i = bxr * TL + l, k = byr;
+    const bool valid = i < g.nx;
+    const int ii = valid ? i : 0;
+    const int j0 = sg * R;
+    const size_t plane = (size_t)g.nx * ny;
+    const uint32_t pbytes = (uint32_t)(plane * sizeof(T));
+    const int rowb = g.nx * (int)sizeof(T);
+    const bufres ru = make_bufres(a.u[COMP] + plane * k, pbytes), rq = make_bufres(a.q[COMP] + plane * k, pbytes);
+    // neighbour pressure: x + 1 (same plane), y + 1 (next row) or the k + 1 plane
+    const bufres rp = make_bufres(a.p + plane * k, pbytes);
+    const bufres rpn = make_bufres(a.p + plane * (COMP == 2 ? wrapp(k, g.nz) : k), pbytes);
+    const int vo = (ii + g.nx * j0) * (int)sizeof(T);
+    const int von = COMP == 0 ? (wrapp(ii, g.nx) + g.nx * j0) * (int)sizeof(T) : vo;
+    const int jact = (COMP == 1 && g.wall) ? ny - 1 : ny;
+    const double rdn = COMP == 0 ? a.rdx : a.rdz;
+    double x[R];
+    double um = (double)bload<T>(ru, sg > 0 ? vo - rowb : vo, 0); // coefficient ca[0] = 0 cuts row −1
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+        // the chunk's global offsets and LDS rows pass through an empty asm
+        // that depends on the previous chunk's result: neither its loads nor
+        // its coefficient reads are hoisted above that chunk (registers)
+        int vc = vo, vn = von, jb = j0 + c0;
+        asm volatile("" : "+v"(vc), "+v"(vn), "+v"(jb) : "v"(x[c0 > 0 ? c0 - 1 : 0]));
+        T ur[CH + 1], qr[CH], pn[CH], pr[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int so = rowb * (c0 + q);
+            ur[q] = bload<T>(ru, vc, so);
+            qr[q] = bload<T>(rq, vc, so);
+            if (!PKL) pr[q] = bload<T>(rp, vc, so);
+            if (COMP != 1) pn[q] = bload<T>(rpn, vn, so);
+            else if (!PKL) pn[q] = bload<T>(rp, vc, so + rowb);
+        }
+        ur[CH] = bload<T>(ru, vc, rowb * (c0 + CH));
+        if (c0 == 0) __syncthreads(); // (as k_cn_seg: keeps the coefficient reads after the first loads)
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int j = jb + q;
+            const double rd = COMP == 1 ? cr[j] : rdn;
+            const double pc = PKL ? pk[c0 + q] : (double)pr[q];
+            const double pnb = COMP == 1 && PKL ? pk[c0 + q + 1] : (double)pn[q];
+            const double dpdc = (pnb - pc) * rd;
+            const double v = j < jact ? a.B * (double)qr[q] - a.adt * dpdc : 0.0;
+            const double u0 = (double)ur[q];
+            double lap = cb[j] * u0;
+            lap += ca[j] * um;
+            lap += cc[j] * (double)ur[q + 1];
+            x[c0 + q] = u0 + v + a.theta * lap;
+            um = u0;
+        }
+    }
+    // forward sweep, local to the segment (its coefficient rows are read
+    // after the right-hand side is complete, as above)
+    int js = j0;
+    asm volatile("" : "+v"(js) : "v"(x[R - 1]));
+    {
+        double P = 1.0, y = 0.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int j = js + q;
+            y = cf[j] * y + ci[j] * x[q];
+            x[q] = y;
+            P *= cf[j];
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double xi = 0.0;
+        for (int q = 0; q < S; ++q) {
+            const int t = q * TL + l;
+            Xin[t] = xi;
+            xi = As[t] * xi + Bs[t];
+        }
+    }
+    __syncthreads();
+    {
+        const double xi = Xin[threadIdx.x];
+        double P = 1.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            P *= cf[js + q];
+            x[q] += P * xi;
+        }
+    }
+    __syncthreads();
+    // back substitution, local to the segment
+    int jq = j0;
+    asm volatile("" : "+v"(jq) : "v"(x[R - 1]));
+    {
+        double P = 1.0, y = 0.0;
+#pragma unroll
+        for (int q = R - 1; q >= 0; --q) {
+            const int j = jq + q;
+            y = cq[j] * y + x[q];
+            x[q] = y;
+            P *= cq[j];
+        }
+        As[threadIdx.x] = P;
+        Bs[threadIdx.x] = y;
+    }
+    __syncthreads();
+    if (sg == 0) {
+        double xi = 0.0;
+        for (int q = S - 1; q >= 0; --q) {
+            const int t = q * TL + l;
+            Xin[t] = xi;
+            xi = As[t] * xi + Bs[t];
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        const double xi = Xin[threadIdx.x];
+        double P = 1.0;
+#pragma unroll
+        for (int q = R - 1; q >= 0; --q) {
+            P *= cq[jq + q];
+            bstore(ru, vo, rowb * q, T(x[q] + P * xi));
+        }
+    }
+}
+
+template <typename T, int R, bool PKL>
+__global__ void __launch_bounds__(256, 2) k_cn_seg3(CN3Args<T> a, FVGeom g, int TL)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int ny = g.ny;
+    double *As = reinterpret_cast<double *>(smem_raw), *Bs = As + blockDim.x, *Xin = Bs + blockDim.x;
+    double *ca = Xin + blockDim.x, *cb = ca + ny, *cc = cb + ny, *cf = cc + ny, *ci = cf + ny, *cq = ci + ny;
+    double *cr = cq + ny;            // 1/Δyu of the normal rows
+    double *pks = cr + ny;           // the threads' pressure rows, R + 1 each
+    auto stage = [&](int set) {
+        for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+            ca[j] = j == 0 ? 0.0 : a.ya[set][j]; // Dy rows with the couplings past the line ends cut
+            cb[j] = a.yb[set][j];
+            cc[j] = j == ny - 1 ? 0.0 : a.yc[set][j];
+            cf[j] = j == 0 ? 0.0 : a.theta * a.ya[set][j] * a.ip[set][j];
+            ci[j] = a.ip[set][j];
+            cq[j] = j == ny - 1 ? 0.0 : -a.cp[set][j];
+        }
+    };
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) cr[j] = a.rdyu[j + 1];
+    stage(0);
+    // this thread's pressure rows j0 .. j0 + R of plane k (past the plane: 0)
+    const int l = threadIdx.x % TL, sg = threadIdx.x / TL;
+    int bxr, byr;
+    xcd_block2(bxr, byr);
+    const int i = bxr * TL + l, k = byr;
+    const int ii = i < g.nx ? i : 0;
+    const size_t plane = (size_t)g.nx * ny;
+    const bufres rp = make_bufres(a.p + plane * k, (uint32_t)(plane * sizeof(T)));
+    const int vo = (ii + g.nx * sg * R) * (int)sizeof(T), rowb = g.nx * (int)sizeof(T);
+    double *pk = pks + (size_t)threadIdx.x * (R + 1); // odd stride: the lanes' rows on distinct banks
+    if (PKL) {
+#pragma unroll
+        for (int q = 0; q <= R; ++q) pk[q] = (double)bload<T>(rp, vo, rowb * q);
+    }
+    __syncthreads(); // the coefficient rows are staged
+    cn3_component<T, 0, R, PKL>(a, g, pk, TL, As, Bs, Xin, ca, cb, cc, cf, ci, cq, cr);
+    __syncthreads(); // the x component's last reads of the rows are done
+    stage(1);
+    __syncthreads();
+    cn3_component<T, 1, R, PKL>(a, g, pk, TL, As, Bs, Xin, ca, cb, cc, cf, ci, cq, cr);
+    __syncthreads();
+    stage(0);
+    __syncthreads();
+    cn3_component<T, 2, R, PKL>(a, g, pk, TL, As, Bs, Xin, ca, cb, cc, cf, ci, cq, cr);
+}
+
 // Pivots of I − θ tridiag(a, b, c) for the tangential (set 0) and normal
 // (set 1) rows: out[set][0][j] = c'_j, out[set][1][j] = 1/β_j.
 __global__ void __launch_bounds__(256) k_cn_coeffs(const double *__restrict__ at, const double *__restrict__ bt,

@@ -124,6 +124,20 @@ def test_c3_y_solves(precision, full_handles, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("fused", ["1", "2"])
+def test_c3_cn_three_components_per_launch(precision, fused, monkeypatch):
+    """The same CN segments with the three velocity components in one launch
+    (k_cn_seg3, DNS_CN_FUSED=1: pressure rows re-read per component; 2: staged
+    once in LDS)."""
+    monkeypatch.setenv("DNS_CN_FUSED", fused)
+    ug, u, pg, po, s = _run_pair((64, 384, 64), "implicit", precision)
+    d = s.describe()
+    monkeypatch.delenv("DNS_CN_FUSED")
+    assert d["cn"].startswith("seg3 R=24") and d["cn"].endswith("p=lds" if fused == "2" else "p=l2")
+    _check(ug, u, pg, po, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_c3_x_and_z_lines(precision, full_handles):
     """(512, 8, 512) CN: 256-point half-length x lines and 512-point z lines
     (in the mixed form the 16-line FP32 z tile)."""
