@@ -7,7 +7,8 @@ configurations run:
   the mixed form: Thomas segments of 12 (FP64) / 24 (FP32 pencil) rows, CN
   segments of 24 rows, 256-point half-length x lines, 512-point z lines
   (the FP32 16-line z tile);
-- HIT 512³, kcut 170 (C4): one IF-RK3 step of the production pipeline.
+- HIT 512³, kcut 170 (C4): one IF-RK3 step of the production pipeline, and
+  the same step at 128³ and 256³ (their lane-group pencil FFT lengths).
 
 The FV cases are thin slabs that keep one production dimension each, so the
 FP64 oracle finishes in seconds; `dns_fv_describe` shows that a slab runs the
@@ -147,17 +148,21 @@ def test_c3_x_and_z_lines(precision, full_handles):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_hit512_step_matches_band_limited_oracle(precision):
-    """One IF-RK3 step (src/spectral/step.jl:68-92) at n = 512, kcut = 170 on a
-    field with |k_i| <= 2, against the oracle's step of the same modes at
-    n = 48, kcut = 16 (no truncation or aliasing differs between the two)."""
+@pytest.mark.parametrize("n", [128, 256, 512])
+def test_hit_step_matches_band_limited_oracle(n, precision):
+    """One IF-RK3 step (src/spectral/step.jl:68-92) at n = 128, 256, 512 with
+    the 2/3 cut kcut = n/3 on a field with |k_i| <= 2, against the oracle's
+    step of the same modes at n = 48, kcut = 16 (no truncation or aliasing
+    differs between the two).  n = 512 is the production pipeline (the
+    four-step 256-point x lines, the 512-point lane-group pencils); 128 and
+    256 run the 128- and 256-point lane-group pencil FFTs."""
     visc, dt = 0.01, 0.02
     small = OracleSpectral(SpectralProblem(n=48, kcut=16, visc=visc))
     uh = small.velocityfield(lambda x, y, z: np.sin(x + 2 * y) * np.cos(z) + 0.4 * np.cos(2 * z - y),
                              lambda x, y, z: np.cos(2 * x) * np.sin(z - y) + 0.3 * np.sin(x),
                              lambda x, y, z: np.sin(y) * np.cos(2 * x + z) - 0.2 * np.cos(2 * y))
     assert np.abs(uh).max() > 0.1
-    big = dns.spectral_setup(n=512, kcut=170, visc=visc, T=np.float32 if precision == "mixed" else np.float64)
+    big = dns.spectral_setup(n=n, kcut=n // 3, visc=visc, T=np.float32 if precision == "mixed" else np.float64)
     K, m = big.K, big.m
 
     def to_big(a):
@@ -187,7 +192,7 @@ def test_hit512_step_matches_band_limited_oracle(precision):
     kept = list(range(small.kcut + 1)) + list(range(m - small.kcut, m))
     rest[np.ix_(range(small.K), kept, kept)] = 0
     leak = float(np.abs(rest).max() / np.abs(uh).max())
-    print(f"HIT 512 {precision}: rel err {err:.3e}, modes outside |k| <= 16 {leak:.3e}", flush=True)
+    print(f"HIT {n} {precision}: rel err {err:.3e}, modes outside |k| <= 16 {leak:.3e}", flush=True)
     tol = 1e-12 if precision == "fp64" else MIXED_TOL
     assert err < tol
     assert leak < tol
