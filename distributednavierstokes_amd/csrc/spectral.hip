@@ -240,11 +240,11 @@ template <typename T> struct SpecImpl : SpecBase {
     int backward_yz(const cplx<T> *src = nullptr)
     {
         timer.begin(SFAM_Y);
-        hipLaunchKernelGGL((k_sp_yback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st,
+        hipLaunchKernelGGL((k_sp_yback<T, SP_EPT, SP_NTHR>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st,
                            src ? src : uh, Q, twn, tb, tbf, make_fastdiv(tb), g, fdn);
         timer.end(SFAM_Y);
         timer.begin(SFAM_Z);
-        hipLaunchKernelGGL((k_sp_zback<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, 3), dim3(SP_NTHR), lds, st, Q, P,
+        hipLaunchKernelGGL((k_sp_zback<T, SP_EPT, SP_NTHR>), dim3(ceil_div(g.K, tb), g.n, 3), dim3(SP_NTHR), lds, st, Q, P,
                            twn, tb, tbf, make_fastdiv(tb), g, fdn);
         timer.end(SFAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
@@ -283,7 +283,7 @@ template <typename T> struct SpecImpl : SpecBase {
     int zfwd(int ncomp)
     {
         timer.begin(SFAM_Z);
-        hipLaunchKernelGGL((k_sp_zfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.n, ncomp), dim3(SP_NTHR), lds, st, P, Q,
+        hipLaunchKernelGGL((k_sp_zfwd<T, SP_EPT, SP_NTHR>), dim3(ceil_div(g.K, tb), g.n, ncomp), dim3(SP_NTHR), lds, st, P, Q,
                            twn, tb, tbf, make_fastdiv(tb), g, fdn);
         timer.end(SFAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
@@ -378,7 +378,7 @@ template <typename T> struct SpecImpl : SpecBase {
         DNS_TRY(ensure_phys());
         DNS_TRY(xpass<XMID_R2C>(red));
         DNS_TRY(zfwd(3));
-        hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, uh, twn,
+        hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT, SP_NTHR>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, uh, twn,
                            tb, tbf, make_fastdiv(tb), g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;
@@ -390,7 +390,7 @@ template <typename T> struct SpecImpl : SpecBase {
     {
         DNS_TRY(xpass<XMID_R2C>(red, src));
         DNS_TRY(zfwd(3));
-        hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, dst, twn,
+        hipLaunchKernelGGL((k_sp_yfwd<T, SP_EPT, SP_NTHR>), dim3(ceil_div(g.K, tb), g.m, 3), dim3(SP_NTHR), lds, st, Q, dst, twn,
                            tb, tbf, make_fastdiv(tb), g, fdn);
         DNS_HIP_CHECK(hipGetLastError());
         return DNS_OK;

@@ -59,11 +59,125 @@ __device__ inline int sp_compact(int f, const SpecGeom &g)
 
 // ------------------------------------------------------------ y / z passes
 
+// Tile walk of the y/z pencil passes.  A block holds tb lines (line b is
+// kx = i0 + b, b < nb valid) of one pencil; point t of line b lives at
+// base + b + stride·s(t) in HBM and at buf[b ls + t] in LDS.  When tb
+// divides NT and NT divides tb·n (power-of-two n and tile width), element e
+// of thread x is always line b = x mod tb, point t = x / tb + e·NT/tb: the
+// global byte offset is a per-lane part fixed for the thread plus a
+// block-uniform step (an SGPR), and the LDS offset the thread's base plus a
+// uniform step, so no element pays an index division or a 64-bit address
+// product (half the VALU work of an FP32 512-point pencil pass went there).
+// Other tiles take the per-element division.  Elements of invalid lines
+// and of the killed band take the byte offset 0x80000000, beyond the buffer
+// range: they read zero and their writes are dropped.
+constexpr int SP_OOB = (int)0x80000000u;
+
+template <typename T, int NT>
+__device__ inline bool pencil_walk(int tb, FastDiv ftb, int n, int stride, int &b, int &t0)
+{
+    static_assert((NT & (NT - 1)) == 0, "pencil_walk: NT must be a power of two");
+    const bool fast = (tb & (tb - 1)) == 0 && tb <= NT && ((tb * n) & (NT - 1)) == 0 &&
+                      ((size_t)stride * n + tb) * sizeof(cplx<T>) < (size_t)0x7fffffff;
+    t0 = fdiv((int)threadIdx.x, ftb);
+    b = (int)threadIdx.x - t0 * tb;
+    return fast;
+}
+
+// Loads the tile: buf[b ls + t] = base[b + stride·s(t)], t < n, with
+// s(t) = t, or (COMPACT) the compact index of fine position t (zero in the
+// killed band kcut < t < n − kcut).  `src` counts the positions along the
+// line in HBM (n, or m when COMPACT).
+template <typename T, int EPT, int NT, bool COMPACT>
+__device__ inline void pencil_load(cplx<T> *buf, int ls, const cplx<T> *base, int stride, int nb, int tb, FastDiv ftb,
+                                   const SpecGeom &g)
+{
+    const int n = g.n, src = COMPACT ? g.m : n, sz = (int)sizeof(cplx<T>);
+    int b, t0;
+    if (pencil_walk<T, NT>(tb, ftb, n, stride, b, t0)) {
+        const int S = NT / tb, E = tb * n / NT;
+        const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (src - 1) + tb) * sz));
+        const int v0 = (b + stride * t0) * sz, shift = (n - g.m) * stride * sz;
+        cplx<T> pre[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            if (e < E) {
+                if (COMPACT) {
+                    // the whole offset in the VGPR: a shifted lane offset may
+                    // be negative alone
+                    const int t = t0 + e * S, vs = v0 + e * S * stride * sz;
+                    const int voff = b >= nb ? SP_OOB : t <= g.kcut ? vs : t >= n - g.kcut ? vs - shift : SP_OOB;
+                    pre[e] = bload<cplx<T>>(r, voff, 0);
+                } else {
+                    pre[e] = bload<cplx<T>>(r, b < nb ? v0 : SP_OOB, e * S * stride * sz);
+                }
+            }
+        }
+        cplx<T> *row = buf + b * ls + t0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e)
+            if (e < E) row[e * S] = pre[e];
+        return;
+    }
+    batched_for<SP_LOAD_U(EPT), cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            const int t = fdiv(idx, ftb), bb = idx - t * tb;
+            const int sc = COMPACT ? sp_compact(t, g) : t;
+            cplx<T> v = mk<T>(T(0), T(0));
+            if (bb < nb && sc >= 0) v = base[bb + (size_t)stride * sc];
+            return v;
+        },
+        [&](int idx, cplx<T> v) {
+            const int t = fdiv(idx, ftb), bb = idx - t * tb;
+            buf[bb * ls + t] = v;
+        });
+}
+
+// Stores points c < cnt of the tile: base[b + stride c] = sc · buf[b ls + f(c)],
+// f(c) = c, or (FINE) the fine position of compact index c.
+template <typename T, int EPT, int NT, bool FINE, bool SCALE>
+__device__ inline void pencil_store(const cplx<T> *buf, int ls, cplx<T> *base, int stride, int nb, int tb, FastDiv ftb,
+                                    int cnt, const SpecGeom &g, T sc)
+{
+    const int sz = (int)sizeof(cplx<T>);
+    int b, t0;
+    if (pencil_walk<T, NT>(tb, ftb, g.n, stride, b, t0)) {
+        const int S = NT / tb;
+        const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (cnt - 1) + tb) * sz));
+        const int v0 = b < nb ? (b + stride * t0) * sz : SP_OOB;
+        const cplx<T> *row = buf + b * ls;
+        // all LDS reads first (positions past the line clamped), then the
+        // stores, those of points c >= cnt sent out of range: no branch
+        // between them, so the reads overlap
+        cplx<T> v[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int c = t0 + e * S;
+            v[e] = row[min(FINE ? sp_fine(c, g) : c, g.n - 1)];
+        }
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int c = t0 + e * S;
+            bstore(r, c < cnt ? v0 : SP_OOB, e * S * stride * sz, SCALE ? cscale<T>(v[e], sc) : v[e]);
+        }
+        return;
+    }
+    for (int idx = threadIdx.x; idx < tb * cnt; idx += NT) {
+        const int c = fdiv(idx, ftb), bb = idx - c * tb;
+        if (bb < nb) {
+            cplx<T> v = buf[bb * ls + (FINE ? sp_fine(c, g) : c)];
+            if (SCALE) v = cscale<T>(v, sc);
+            base[bb + (size_t)stride * c] = v;
+        }
+    }
+}
+
 // Backward y pass: B1[i, y, kz, c] = Σ_ky uh[i, ky, kz, c] e^{+i ky y}.
-// grid (ceil(K/tb), m, 3); LDS tb·(n+1) + n (the twiddles).
-template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb, int tbf, FastDiv ftb,
-                                                  SpecGeom g, FFTDesc d)
+// grid (ceil(K/tb), m, 3); LDS tb·ls + the twiddles.
+template <typename T, int EPT, int NT>
+__global__ void __launch_bounds__(NT, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *B1, const cplx<T> *tw, int tb, int tbf, FastDiv ftb,
+                                                     SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -71,33 +185,19 @@ __global__ void __launch_bounds__(256, 2) k_sp_yback(const cplx<T> *uh, cplx<T> 
     const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
-    const int i0 = bxr * tb, kz = byr, c = bzr;
-    batched_for<SP_LOAD_U(EPT), cplx<T>>(
-        tb * n,
-        [&](int idx) {
-            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
-            int jc = sp_compact(t, g);
-            cplx<T> v = mk<T>(T(0), T(0));
-            if (i < K && jc >= 0) v = uh[i + (size_t)K * (jc + (size_t)m * (kz + (size_t)m * c))];
-            return v;
-        },
-        [&](int idx, cplx<T> v) {
-            int t = fdiv(idx, ftb), b = idx - t * tb;
-            buf[b * ls + t] = v;
-        });
+    const int i0 = bxr * tb, kz = byr, c = bzr, nb = min(tb, K - i0);
+    pencil_load<T, EPT, NT, true>(buf, ls, uh + i0 + (size_t)K * m * (kz + (size_t)m * c), K, nb, tb, ftb, g);
     __syncthreads();
     fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
-        if (i < K) B1[i + (size_t)K * (t + (size_t)n * (kz + (size_t)m * c))] = buf[b * ls + t];
-    }
+    pencil_store<T, EPT, NT, false, false>(buf, ls, B1 + i0 + (size_t)K * n * (kz + (size_t)m * c), K, nb, tb, ftb, n, g,
+                                           T(1));
 }
 
 // Backward z pass: P[i, y, z, c] = Σ_kz B1[i, y, kz, c] e^{+i kz z}.
 // grid (ceil(K/tb), n, 3).
-template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, SpecGeom g,
-                                                  FFTDesc d)
+template <typename T, int EPT, int NT>
+__global__ void __launch_bounds__(NT, 2) k_sp_zback(const cplx<T> *B1, cplx<T> *P, const cplx<T> *tw, int tb, int tbf, FastDiv ftb,
+                                                     SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -105,33 +205,19 @@ __global__ void __launch_bounds__(256, 2) k_sp_zback(const cplx<T> *B1, cplx<T> 
     const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
-    const int i0 = bxr * tb, y = byr, c = bzr;
-    batched_for<SP_LOAD_U(EPT), cplx<T>>(
-        tb * n,
-        [&](int idx) {
-            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
-            int kc = sp_compact(t, g);
-            cplx<T> v = mk<T>(T(0), T(0));
-            if (i < K && kc >= 0) v = B1[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * c))];
-            return v;
-        },
-        [&](int idx, cplx<T> v) {
-            int t = fdiv(idx, ftb), b = idx - t * tb;
-            buf[b * ls + t] = v;
-        });
+    const int i0 = bxr * tb, y = byr, c = bzr, nb = min(tb, K - i0);
+    pencil_load<T, EPT, NT, true>(buf, ls, B1 + i0 + (size_t)K * (y + (size_t)n * m * c), K * n, nb, tb, ftb, g);
     __syncthreads();
     fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
-        if (i < K) P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * c))] = buf[b * ls + t];
-    }
+    pencil_store<T, EPT, NT, false, false>(buf, ls, P + i0 + (size_t)K * (y + (size_t)n * n * c), K * n, nb, tb, ftb, n, g,
+                                           T(1));
 }
 
 // Forward z pass: Q[i, y, kc, p] = Σ_z P[i, y, z, p] e^{−i kz z} at the
 // retained kz.  grid (ceil(K/tb), n, ncomp).
-template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, SpecGeom g,
-                                                 FFTDesc d)
+template <typename T, int EPT, int NT>
+__global__ void __launch_bounds__(NT, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q, const cplx<T> *tw, int tb, int tbf, FastDiv ftb,
+                                                    SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -139,31 +225,20 @@ __global__ void __launch_bounds__(256, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q
     const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
-    const int i0 = bxr * tb, y = byr, p = bzr;
-    batched_for<SP_LOAD_U(EPT), cplx<T>>(
-        tb * n,
-        [&](int idx) {
-            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
-            return i < K ? P[i + (size_t)K * (y + (size_t)n * (t + (size_t)n * p))] : mk<T>(T(0), T(0));
-        },
-        [&](int idx, cplx<T> v) {
-            int t = fdiv(idx, ftb), b = idx - t * tb;
-            buf[b * ls + t] = v;
-        });
+    const int i0 = bxr * tb, y = byr, p = bzr, nb = min(tb, K - i0);
+    pencil_load<T, EPT, NT, false>(buf, ls, P + i0 + (size_t)K * (y + (size_t)n * n * p), K * n, nb, tb, ftb, g);
     __syncthreads();
     fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
-    for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
-        int kc = fdiv(idx, ftb), b = idx - kc * tb, i = i0 + b;
-        if (i < K) Q[i + (size_t)K * (y + (size_t)n * (kc + (size_t)m * p))] = buf[b * ls + sp_fine(kc, g)];
-    }
+    pencil_store<T, EPT, NT, true, false>(buf, ls, Q + i0 + (size_t)K * (y + (size_t)n * m * p), K * n, nb, tb, ftb, m, g,
+                                          T(1));
 }
 
 // Forward y pass with truncation and 1/n³ (truncate_y_kernel!,
 // src/spectral/fft.jl:190-193): out[i, jc, kc, p] (K, m, m, ncomp).
 // grid (ceil(K/tb), m, ncomp).
-template <typename T, int EPT>
-__global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, SpecGeom g,
-                                                 FFTDesc d)
+template <typename T, int EPT, int NT>
+__global__ void __launch_bounds__(NT, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *out, const cplx<T> *tw, int tb, int tbf, FastDiv ftb,
+                                                    SpecGeom g, FFTDesc d)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -171,24 +246,13 @@ __global__ void __launch_bounds__(256, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *o
     const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
-    const int i0 = bxr * tb, kc = byr, p = bzr;
-    batched_for<SP_LOAD_U(EPT), cplx<T>>(
-        tb * n,
-        [&](int idx) {
-            int t = fdiv(idx, ftb), b = idx - t * tb, i = i0 + b;
-            return i < K ? Q[i + (size_t)K * (t + (size_t)n * (kc + (size_t)m * p))] : mk<T>(T(0), T(0));
-        },
-        [&](int idx, cplx<T> v) {
-            int t = fdiv(idx, ftb), b = idx - t * tb;
-            buf[b * ls + t] = v;
-        });
+    const int i0 = bxr * tb, kc = byr, p = bzr, nb = min(tb, K - i0);
+    pencil_load<T, EPT, NT, false>(buf, ls, Q + i0 + (size_t)K * n * (kc + (size_t)m * p), K, nb, tb, ftb, g);
     __syncthreads();
     fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
     const T inv = T(1.0 / ((double)n * n * n));
-    for (int idx = threadIdx.x; idx < tb * m; idx += blockDim.x) {
-        int jc = fdiv(idx, ftb), b = idx - jc * tb, i = i0 + b;
-        if (i < K) out[i + (size_t)K * (jc + (size_t)m * (kc + (size_t)m * p))] = cscale<T>(buf[b * ls + sp_fine(jc, g)], inv);
-    }
+    pencil_store<T, EPT, NT, true, true>(buf, ls, out + i0 + (size_t)K * m * (kc + (size_t)m * p), K, nb, tb, ftb, m, g,
+                                         inv);
 }
 
 // ------------------------------------------------------------- x passes
