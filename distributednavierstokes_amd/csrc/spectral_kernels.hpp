@@ -1012,7 +1012,7 @@ template <typename T, bool INV>
 __device__ __forceinline__ void wave_fft256(cplx<T> (&a)[4], cplx<T> *sh, const cplx<T> *tw4, int lane_)
 {
     int lane = lane_;
-    asm volatile("" : "+v"(lane));
+    if (sizeof(T) == 8) asm volatile("" : "+v"(lane));
     const int c = lane & 15, h = lane >> 4;
     fft16_rows<T, INV>(a, tw4 + 256, h);
 #pragma unroll
@@ -1057,7 +1057,7 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
     cplx<T> *twx = tw + TW4S_LEN;
     cplx<T> *sh = twx + (N + 1) + (threadIdx.x >> 6) * N;
     __shared__ double red[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     stage_tw4s<T>(tw, a.tw);
     for (int t = threadIdx.x; t <= N; t += blockDim.x) twx[t] = a.twx[t];
     __syncthreads();
@@ -1065,17 +1065,31 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
     const int L = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
     const bool live = L < nlines;
     const size_t pstride = (size_t)K * n * n;
-    const cplx<T> zero = mk<T>(T(0), T(0));
+    // Rows through buffer resources of K modes (none past the last line):
+    // the byte offsets of the lane's modes k = lane + 64 r and N − k are
+    // fixed, those at or beyond K are out of range (read as zero, writes
+    // dropped), so no branch surrounds the loads and stores.
+    const int sz = (int)sizeof(cplx<T>);
+    // (wave-uniform values, kept in SGPRs for the buffer resources)
+    const uint32_t rowbytes = __builtin_amdgcn_readfirstlane(live ? K * sz : 0);
+    const int Lrow = __builtin_amdgcn_readfirstlane(live ? L : 0);
+    int ok[4], on[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = lane + 64 * r;
+        ok[r] = k < K ? k * sz : SP_OOB;
+        on[r] = N - k < K ? (N - k) * sz : SP_OOB;
+    }
     T v[3][8];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const cplx<T> *row = a.P + (size_t)(live ? L : 0) * K + c * pstride;
+        const bufres row = make_bufres(a.P + (size_t)Lrow * K + c * pstride, rowbytes);
         cplx<T> z[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int k = lane + 64 * r;
-            cplx<T> Xk = (live && k < K) ? row[k] : zero;
-            cplx<T> Xn = (live && N - k < K) ? row[N - k] : zero;
+            cplx<T> Xk = bload<cplx<T>>(row, ok[r], 0);
+            cplx<T> Xn = bload<cplx<T>>(row, on[r], 0);
             if (k == 0) {
                 Xk.y = T(0);
                 Xn.y = T(0);
@@ -1111,17 +1125,15 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
 #pragma unroll
         for (int r = 0; r < 4; ++r) sh[lane + 64 * r] = z[r];
         wave_lds_sync();
-        cplx<T> *out = a.P + (size_t)L * K + p * pstride;
+        const bufres out = make_bufres(a.P + (size_t)Lrow * K + p * pstride, rowbytes);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int kx = lane + 64 * r;
-            if (live && kx < K) {
-                const cplx<T> Zk = z[r];
-                const cplx<T> Zc = cconj<T>(sh[(N - kx) & (N - 1)]);
-                const cplx<T> sm = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
-                const cplx<T> wd = cmul<T>(twx[kx], df);
-                out[kx] = mk<T>(T(0.5) * (sm.x + wd.y), T(0.5) * (sm.y - wd.x));
-            }
+            const cplx<T> Zk = z[r];
+            const cplx<T> Zc = cconj<T>(sh[(N - kx) & (N - 1)]);
+            const cplx<T> sm = cadd<T>(Zk, Zc), df = csub<T>(Zk, Zc);
+            const cplx<T> wd = cmul<T>(twx[kx], df);
+            bstore(out, ok[r], 0, mk<T>(T(0.5) * (sm.x + wd.y), T(0.5) * (sm.y - wd.x)));
         }
         wave_lds_sync();
     }
