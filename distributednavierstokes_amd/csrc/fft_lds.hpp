@@ -482,4 +482,90 @@ __device__ __forceinline__ void fft_lds_tile(cplx<T> *buf, int nlines, int ls, c
         fft_lds<T, EPT, INV, NT>(buf + (size_t)sb * ls, min(sub, nlines - sb), ls, d, tw);
 }
 
+// Line tiles in HBM: a block holds tb lines (line b valid for b < nb) of n
+// points; point t of line b lives at base + b + stride·t and at
+// buf[b ls + t] in LDS.  When tb divides NT and NT divides tb·n
+// (power-of-two n and tile width), element e of thread x is always line
+// b = x mod tb, point t = x / tb + e·NT/tb (tile_walk): the global byte
+// offset is a per-lane part fixed for the thread plus a block-uniform step
+// (an SGPR), and the LDS offset the thread's base plus a uniform step, so no
+// element pays an index division or a 64-bit address product (half the VALU
+// work of an FP32 512-point pencil pass went there).  Other tiles take the
+// per-element division.  Invalid lines take the byte offset 0x80000000,
+// beyond the buffer range: they read zero and their writes are dropped.
+constexpr int TILE_OOB = (int)0x80000000u;
+
+template <typename T, int NT>
+__device__ inline bool tile_walk(int tb, FastDiv ftb, int n, int stride, int &b, int &t0)
+{
+    static_assert((NT & (NT - 1)) == 0, "tile_walk: NT must be a power of two");
+    const bool fast = (tb & (tb - 1)) == 0 && tb <= NT && ((tb * n) & (NT - 1)) == 0 &&
+                      ((size_t)stride * n + tb) * sizeof(cplx<T>) < (size_t)0x7fffffff;
+    t0 = fdiv((int)threadIdx.x, ftb);
+    b = (int)threadIdx.x - t0 * tb;
+    return fast;
+}
+
+// buf[b ls + t] = base[b + stride t], t < n (U loads in flight per thread on
+// the division path).
+template <typename T, int EPT, int NT, int U>
+__device__ inline void tile_load(cplx<T> *buf, int ls, const cplx<T> *base, int stride, int nb, int tb, FastDiv ftb,
+                                 int n)
+{
+    const int sz = (int)sizeof(cplx<T>);
+    int b, t0;
+    if (tile_walk<T, NT>(tb, ftb, n, stride, b, t0)) {
+        const int S = NT / tb, E = tb * n / NT;
+        const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (n - 1) + tb) * sz));
+        const int v0 = b < nb ? (b + stride * t0) * sz : TILE_OOB;
+        cplx<T> pre[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e)
+            if (e < E) pre[e] = bload<cplx<T>>(r, v0, e * S * stride * sz);
+        cplx<T> *row = buf + b * ls + t0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e)
+            if (e < E) row[e * S] = pre[e];
+        return;
+    }
+    batched_for<U, cplx<T>>(
+        tb * n,
+        [&](int idx) {
+            const int t = fdiv(idx, ftb), bb = idx - t * tb;
+            return bb < nb ? base[bb + (size_t)stride * t] : mk<T>(T(0), T(0));
+        },
+        [&](int idx, cplx<T> v) {
+            const int t = fdiv(idx, ftb), bb = idx - t * tb;
+            buf[bb * ls + t] = v;
+        });
+}
+
+// base[b + stride t] = buf[b ls + t], t < n; the LDS reads all issued before
+// the stores (no branch between them).
+template <typename T, int EPT, int NT>
+__device__ inline void tile_store(const cplx<T> *buf, int ls, cplx<T> *base, int stride, int nb, int tb, FastDiv ftb,
+                                  int n)
+{
+    const int sz = (int)sizeof(cplx<T>);
+    int b, t0;
+    if (tile_walk<T, NT>(tb, ftb, n, stride, b, t0)) {
+        const int S = NT / tb, E = tb * n / NT;
+        const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (n - 1) + tb) * sz));
+        const int v0 = b < nb ? (b + stride * t0) * sz : TILE_OOB;
+        const cplx<T> *row = buf + b * ls + t0;
+        cplx<T> v[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e)
+            if (e < E) v[e] = row[e * S];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e)
+            if (e < E) bstore(r, v0, e * S * stride * sz, v[e]);
+        return;
+    }
+    for (int idx = threadIdx.x; idx < tb * n; idx += NT) {
+        const int t = fdiv(idx, ftb), bb = idx - t * tb;
+        if (bb < nb) base[bb + (size_t)stride * t] = buf[bb * ls + t];
+    }
+}
+
 }  // namespace dns

@@ -712,6 +712,19 @@ template <typename T> struct FVImpl : FVBase {
 
     // poisson! of the divergence of uu, writing the pressure into psi
     // (src/poisson.jl:147-161), without transposes: x pass, z pass, y solve,
+    // in-place z FFT of the (nxc, ny, nz) pencil A (k_fft_z, 256- or
+    // 512-thread tiles)
+    template <bool INV> void launch_fft_z(cplx<T> *A_)
+    {
+        const dim3 grd(ceil_div(g.nxc, tbz), g.ny);
+        if (nthrz == 2 * NTHR)
+            hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, INV, 2 * NTHR>), grd, dim3(nthrz), ldsz, st, A_, twz, tbz, tbzf,
+                               make_fastdiv(tbz), g, fdz);
+        else
+            hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, INV, NTHR>), grd, dim3(nthrz), ldsz, st, A_, twz, tbz, tbzf,
+                               make_fastdiv(tbz), g, fdz);
+    }
+
     // z pass, x pass.
     int launch_poisson(T **uu, T *psi, int mode /*0 = divergence of uu*/)
     {
@@ -748,8 +761,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.end(FAM_X);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
-        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
-                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
+        launch_fft_z<false>(A);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Y);
@@ -794,8 +806,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.end(FAM_Y);
         DNS_HIP_CHECK(hipGetLastError());
         timer.begin(FAM_Z);
-        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
-                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
+        launch_fft_z<true>(A);
         timer.end(FAM_Z);
         DNS_HIP_CHECK(hipGetLastError());
         XInvArgs<T> xi{};
@@ -1043,8 +1054,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.end(FAM_X);
         timer.begin(FAM_Z);
         plain_copy_pencil(pB, A);
-        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, pB,
-                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
+        launch_fft_z<false>(pB);
         plain_copy_pencil(pC, pB);
         timer.end(FAM_Z);
         timer.begin(FAM_Y);
@@ -1059,8 +1069,7 @@ template <typename T> struct FVImpl : FVBase {
         timer.end(FAM_Y);
         timer.begin(FAM_Z);
         plain_copy_pencil(pB, pC);
-        hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, true>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, pB,
-                           twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
+        launch_fft_z<true>(pB);
         plain_copy_pencil(A, pB);
         timer.end(FAM_Z);
         timer.begin(FAM_X);
@@ -1582,8 +1591,7 @@ template <typename T> struct FVImpl : FVBase {
         xa.fny = make_fastdiv(g.ny);
             hipLaunchKernelGGL((k_rfft_x<T, X_EPT, 1>), dim3(ceil_div(g.ny * g.nz, lpb)), dim3(NTHR), ldsx, st, xa, g,
                                fdx);
-            hipLaunchKernelGGL((k_fft_z<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tbz), g.ny), dim3(nthrz), ldsz, st, A,
-                               twz, tbz, tbzf, make_fastdiv(tbz), g, fdz);
+            launch_fft_z<false>(A);
             hipLaunchKernelGGL((k_fourier_y<T, ZY_EPT, false>), dim3(ceil_div(g.nxc, tby), g.nz), dim3(nthry), ldsy,
                                st, A, twy, lam1, lam2, lam3, 1.0, tby, tbyf, make_fastdiv(tby), g, fdy);
             if (nbins <= 2048)

@@ -741,7 +741,9 @@ __global__ void __launch_bounds__(256) k_irfft_xw(XInvArgs<T> a, FVGeom g)
 // z pass: in-place complex FFT along z of A(kx, j, kz) for a tile of `tb`
 // consecutive kx at fixed j (coalesced tb·16-byte row segments).
 // ------------------------------------------------------------------------
-template <typename T, int EPT, bool INV>
+// NT: the block size (256 or 512).  The line tile walks of fft_lds.hpp
+// (tile_load / tile_store).
+template <typename T, int EPT, bool INV, int NT>
 __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, int tb, int tbf, FastDiv ftb, FVGeom g,
                                                   FFTDesc d)
 {
@@ -751,24 +753,12 @@ __global__ void __launch_bounds__(512) k_fft_z(cplx<T> *A, const cplx<T> *tw, in
     const cplx<T> *twl = stage_tile_twiddles<T>(buf + (size_t)tb * ls, tw, n, tb);
     int bxr, byr;
     xcd_block2(bxr, byr);
-    const int kx0 = bxr * tb, j = byr;
-    batched_for<FV_LOAD_U(EPT), cplx<T>>(
-        tb * n,
-        [&](int idx) {
-            int t = fdiv(idx, ftb), kx = kx0 + (idx - t * tb);
-            return kx < g.nxc ? A[CX(kx, j, t, g)] : mk<T>(T(0), T(0));
-        },
-        [&](int idx, cplx<T> v) {
-            int t = fdiv(idx, ftb), b = idx - t * tb;
-            buf[b * ls + t] = v;
-        });
+    const int kx0 = bxr * tb, j = byr, nb = min(tb, g.nxc - kx0), stride = g.ldc * g.ny;
+    cplx<T> *line0 = A + CX(kx0, j, 0, g);
+    tile_load<T, EPT, NT, FV_LOAD_U(EPT)>(buf, ls, line0, stride, nb, tb, ftb, n);
     __syncthreads();
     fft_lds_tile<T, EPT, INV, 0, true>(buf, tb, ls, d, twl, tbf);
-    for (int idx = threadIdx.x; idx < tb * n; idx += blockDim.x) {
-        int t = fdiv(idx, ftb), b = idx - t * tb;
-        int kx = kx0 + b;
-        if (kx < g.nxc) A[CX(kx, j, t, g)] = buf[b * ls + t];
-    }
+    tile_store<T, EPT, NT>(buf, ls, line0, stride, nb, tb, ftb, n);
 }
 
 // ------------------------------------------------------------------------

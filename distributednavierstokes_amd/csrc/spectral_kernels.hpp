@@ -59,58 +59,32 @@ __device__ inline int sp_compact(int f, const SpecGeom &g)
 
 // ------------------------------------------------------------ y / z passes
 
-// Tile walk of the y/z pencil passes.  A block holds tb lines (line b is
-// kx = i0 + b, b < nb valid) of one pencil; point t of line b lives at
-// base + b + stride·s(t) in HBM and at buf[b ls + t] in LDS.  When tb
-// divides NT and NT divides tb·n (power-of-two n and tile width), element e
-// of thread x is always line b = x mod tb, point t = x / tb + e·NT/tb: the
-// global byte offset is a per-lane part fixed for the thread plus a
-// block-uniform step (an SGPR), and the LDS offset the thread's base plus a
-// uniform step, so no element pays an index division or a 64-bit address
-// product (half the VALU work of an FP32 512-point pencil pass went there).
-// Other tiles take the per-element division.  Elements of invalid lines
-// and of the killed band take the byte offset 0x80000000, beyond the buffer
-// range: they read zero and their writes are dropped.
-constexpr int SP_OOB = (int)0x80000000u;
+// Pencil tiles (line tiles of fft_lds.hpp: line b is kx = i0 + b) whose
+// HBM side is truncated: the backward passes read the compact index of each
+// fine position (zero in the killed band kcut < t < n − kcut), the forward
+// passes write the retained positions only.  Same walk as tile_load /
+// tile_store.
 
-template <typename T, int NT>
-__device__ inline bool pencil_walk(int tb, FastDiv ftb, int n, int stride, int &b, int &t0)
+// buf[b ls + t] = base[b + stride·c(t)], t < n, c(t) the compact index of t.
+template <typename T, int EPT, int NT>
+__device__ inline void pencil_load_compact(cplx<T> *buf, int ls, const cplx<T> *base, int stride, int nb, int tb,
+                                           FastDiv ftb, const SpecGeom &g)
 {
-    static_assert((NT & (NT - 1)) == 0, "pencil_walk: NT must be a power of two");
-    const bool fast = (tb & (tb - 1)) == 0 && tb <= NT && ((tb * n) & (NT - 1)) == 0 &&
-                      ((size_t)stride * n + tb) * sizeof(cplx<T>) < (size_t)0x7fffffff;
-    t0 = fdiv((int)threadIdx.x, ftb);
-    b = (int)threadIdx.x - t0 * tb;
-    return fast;
-}
-
-// Loads the tile: buf[b ls + t] = base[b + stride·s(t)], t < n, with
-// s(t) = t, or (COMPACT) the compact index of fine position t (zero in the
-// killed band kcut < t < n − kcut).  `src` counts the positions along the
-// line in HBM (n, or m when COMPACT).
-template <typename T, int EPT, int NT, bool COMPACT>
-__device__ inline void pencil_load(cplx<T> *buf, int ls, const cplx<T> *base, int stride, int nb, int tb, FastDiv ftb,
-                                   const SpecGeom &g)
-{
-    const int n = g.n, src = COMPACT ? g.m : n, sz = (int)sizeof(cplx<T>);
+    const int n = g.n, sz = (int)sizeof(cplx<T>);
     int b, t0;
-    if (pencil_walk<T, NT>(tb, ftb, n, stride, b, t0)) {
+    if (tile_walk<T, NT>(tb, ftb, n, stride, b, t0)) {
         const int S = NT / tb, E = tb * n / NT;
-        const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (src - 1) + tb) * sz));
+        const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (g.m - 1) + tb) * sz));
         const int v0 = (b + stride * t0) * sz, shift = (n - g.m) * stride * sz;
         cplx<T> pre[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             if (e < E) {
-                if (COMPACT) {
-                    // the whole offset in the VGPR: a shifted lane offset may
-                    // be negative alone
-                    const int t = t0 + e * S, vs = v0 + e * S * stride * sz;
-                    const int voff = b >= nb ? SP_OOB : t <= g.kcut ? vs : t >= n - g.kcut ? vs - shift : SP_OOB;
-                    pre[e] = bload<cplx<T>>(r, voff, 0);
-                } else {
-                    pre[e] = bload<cplx<T>>(r, b < nb ? v0 : SP_OOB, e * S * stride * sz);
-                }
+                // the whole offset in the VGPR: a shifted lane offset may be
+                // negative alone
+                const int t = t0 + e * S, vs = v0 + e * S * stride * sz;
+                const int voff = b >= nb ? TILE_OOB : t <= g.kcut ? vs : t >= n - g.kcut ? vs - shift : TILE_OOB;
+                pre[e] = bload<cplx<T>>(r, voff, 0);
             }
         }
         cplx<T> *row = buf + b * ls + t0;
@@ -122,8 +96,7 @@ __device__ inline void pencil_load(cplx<T> *buf, int ls, const cplx<T> *base, in
     batched_for<SP_LOAD_U(EPT), cplx<T>>(
         tb * n,
         [&](int idx) {
-            const int t = fdiv(idx, ftb), bb = idx - t * tb;
-            const int sc = COMPACT ? sp_compact(t, g) : t;
+            const int t = fdiv(idx, ftb), bb = idx - t * tb, sc = sp_compact(t, g);
             cplx<T> v = mk<T>(T(0), T(0));
             if (bb < nb && sc >= 0) v = base[bb + (size_t)stride * sc];
             return v;
@@ -134,41 +107,37 @@ __device__ inline void pencil_load(cplx<T> *buf, int ls, const cplx<T> *base, in
         });
 }
 
-// Stores points c < cnt of the tile: base[b + stride c] = sc · buf[b ls + f(c)],
-// f(c) = c, or (FINE) the fine position of compact index c.
-template <typename T, int EPT, int NT, bool FINE, bool SCALE>
-__device__ inline void pencil_store(const cplx<T> *buf, int ls, cplx<T> *base, int stride, int nb, int tb, FastDiv ftb,
-                                    int cnt, const SpecGeom &g, T sc)
+// base[b + stride c] = sc · buf[b ls + f(c)], c < m, f(c) the fine position
+// of compact index c.
+template <typename T, int EPT, int NT, bool SCALE>
+__device__ inline void pencil_store_fine(const cplx<T> *buf, int ls, cplx<T> *base, int stride, int nb, int tb,
+                                         FastDiv ftb, const SpecGeom &g, T sc)
 {
-    const int sz = (int)sizeof(cplx<T>);
+    const int sz = (int)sizeof(cplx<T>), cnt = g.m;
     int b, t0;
-    if (pencil_walk<T, NT>(tb, ftb, g.n, stride, b, t0)) {
+    if (tile_walk<T, NT>(tb, ftb, g.n, stride, b, t0)) {
         const int S = NT / tb;
         const bufres r = make_bufres(base, (uint32_t)(((size_t)stride * (cnt - 1) + tb) * sz));
-        const int v0 = b < nb ? (b + stride * t0) * sz : SP_OOB;
+        const int v0 = b < nb ? (b + stride * t0) * sz : TILE_OOB;
         const cplx<T> *row = buf + b * ls;
         // all LDS reads first (positions past the line clamped), then the
-        // stores, those of points c >= cnt sent out of range: no branch
+        // stores, those of indices c >= m sent out of range: no branch
         // between them, so the reads overlap
         cplx<T> v[EPT];
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) {
-            const int c = t0 + e * S;
-            v[e] = row[min(FINE ? sp_fine(c, g) : c, g.n - 1)];
-        }
+        for (int e = 0; e < EPT; ++e) v[e] = row[min(sp_fine(t0 + e * S, g), g.n - 1)];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             const int c = t0 + e * S;
-            bstore(r, c < cnt ? v0 : SP_OOB, e * S * stride * sz, SCALE ? cscale<T>(v[e], sc) : v[e]);
+            bstore(r, c < cnt ? v0 : TILE_OOB, e * S * stride * sz, SCALE ? cscale<T>(v[e], sc) : v[e]);
         }
         return;
     }
     for (int idx = threadIdx.x; idx < tb * cnt; idx += NT) {
         const int c = fdiv(idx, ftb), bb = idx - c * tb;
         if (bb < nb) {
-            cplx<T> v = buf[bb * ls + (FINE ? sp_fine(c, g) : c)];
-            if (SCALE) v = cscale<T>(v, sc);
-            base[bb + (size_t)stride * c] = v;
+            cplx<T> v = buf[bb * ls + sp_fine(c, g)];
+            base[bb + (size_t)stride * c] = SCALE ? cscale<T>(v, sc) : v;
         }
     }
 }
@@ -186,11 +155,10 @@ __global__ void __launch_bounds__(NT, 2) k_sp_yback(const cplx<T> *uh, cplx<T> *
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kz = byr, c = bzr, nb = min(tb, K - i0);
-    pencil_load<T, EPT, NT, true>(buf, ls, uh + i0 + (size_t)K * m * (kz + (size_t)m * c), K, nb, tb, ftb, g);
+    pencil_load_compact<T, EPT, NT>(buf, ls, uh + i0 + (size_t)K * m * (kz + (size_t)m * c), K, nb, tb, ftb, g);
     __syncthreads();
     fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
-    pencil_store<T, EPT, NT, false, false>(buf, ls, B1 + i0 + (size_t)K * n * (kz + (size_t)m * c), K, nb, tb, ftb, n, g,
-                                           T(1));
+    tile_store<T, EPT, NT>(buf, ls, B1 + i0 + (size_t)K * n * (kz + (size_t)m * c), K, nb, tb, ftb, n);
 }
 
 // Backward z pass: P[i, y, z, c] = Σ_kz B1[i, y, kz, c] e^{+i kz z}.
@@ -206,11 +174,10 @@ __global__ void __launch_bounds__(NT, 2) k_sp_zback(const cplx<T> *B1, cplx<T> *
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, c = bzr, nb = min(tb, K - i0);
-    pencil_load<T, EPT, NT, true>(buf, ls, B1 + i0 + (size_t)K * (y + (size_t)n * m * c), K * n, nb, tb, ftb, g);
+    pencil_load_compact<T, EPT, NT>(buf, ls, B1 + i0 + (size_t)K * (y + (size_t)n * m * c), K * n, nb, tb, ftb, g);
     __syncthreads();
     fft_lds_tile<T, EPT, true, 0, true>(buf, tb, ls, d, twl, tbf);
-    pencil_store<T, EPT, NT, false, false>(buf, ls, P + i0 + (size_t)K * (y + (size_t)n * n * c), K * n, nb, tb, ftb, n, g,
-                                           T(1));
+    tile_store<T, EPT, NT>(buf, ls, P + i0 + (size_t)K * (y + (size_t)n * n * c), K * n, nb, tb, ftb, n);
 }
 
 // Forward z pass: Q[i, y, kc, p] = Σ_z P[i, y, z, p] e^{−i kz z} at the
@@ -226,11 +193,11 @@ __global__ void __launch_bounds__(NT, 2) k_sp_zfwd(const cplx<T> *P, cplx<T> *Q,
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, y = byr, p = bzr, nb = min(tb, K - i0);
-    pencil_load<T, EPT, NT, false>(buf, ls, P + i0 + (size_t)K * (y + (size_t)n * n * p), K * n, nb, tb, ftb, g);
+    tile_load<T, EPT, NT, SP_LOAD_U(EPT)>(buf, ls, P + i0 + (size_t)K * (y + (size_t)n * n * p), K * n, nb, tb, ftb, n);
     __syncthreads();
     fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
-    pencil_store<T, EPT, NT, true, false>(buf, ls, Q + i0 + (size_t)K * (y + (size_t)n * m * p), K * n, nb, tb, ftb, m, g,
-                                          T(1));
+    pencil_store_fine<T, EPT, NT, false>(buf, ls, Q + i0 + (size_t)K * (y + (size_t)n * m * p), K * n, nb, tb, ftb, g,
+                                         T(1));
 }
 
 // Forward y pass with truncation and 1/n³ (truncate_y_kernel!,
@@ -247,12 +214,11 @@ __global__ void __launch_bounds__(NT, 2) k_sp_yfwd(const cplx<T> *Q, cplx<T> *ou
     int bxr, byr, bzr;
     xcd_block3(bxr, byr, bzr);
     const int i0 = bxr * tb, kc = byr, p = bzr, nb = min(tb, K - i0);
-    pencil_load<T, EPT, NT, false>(buf, ls, Q + i0 + (size_t)K * n * (kc + (size_t)m * p), K, nb, tb, ftb, g);
+    tile_load<T, EPT, NT, SP_LOAD_U(EPT)>(buf, ls, Q + i0 + (size_t)K * n * (kc + (size_t)m * p), K, nb, tb, ftb, n);
     __syncthreads();
     fft_lds_tile<T, EPT, false, 0, true>(buf, tb, ls, d, twl, tbf);
     const T inv = T(1.0 / ((double)n * n * n));
-    pencil_store<T, EPT, NT, true, true>(buf, ls, out + i0 + (size_t)K * m * (kc + (size_t)m * p), K, nb, tb, ftb, m, g,
-                                         inv);
+    pencil_store_fine<T, EPT, NT, true>(buf, ls, out + i0 + (size_t)K * m * (kc + (size_t)m * p), K, nb, tb, ftb, g, inv);
 }
 
 // ------------------------------------------------------------- x passes
@@ -1077,8 +1043,8 @@ __global__ void __launch_bounds__(256) k_sp_xrhs256(XMidArgs<T> a, SpecGeom g)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int k = lane + 64 * r;
-        ok[r] = k < K ? k * sz : SP_OOB;
-        on[r] = N - k < K ? (N - k) * sz : SP_OOB;
+        ok[r] = k < K ? k * sz : TILE_OOB;
+        on[r] = N - k < K ? (N - k) * sz : TILE_OOB;
     }
     T v[3][8];
 #pragma unroll
