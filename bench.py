@@ -382,8 +382,9 @@ def mmin_units_per_step(prob, cfl=True):
     return units + (3 if cfl else 0)
 
 
-def family_bytes_per_launch(prob, fam, esize):
-    """Algorithmic bytes of one launch of a kernel family (per launch average)."""
+def family_bytes_per_launch(prob, fam, esize, cn3=False):
+    """Algorithmic bytes of one launch of a kernel family (per launch average).
+    cn3: the CN family runs the three components per launch (k_cn_seg3)."""
     nx, ny, nz = prob.n
     N = nx * ny * nz
     NC = (nx // 2 + 1) * ny * nz
@@ -398,8 +399,8 @@ def family_bytes_per_launch(prob, fam, esize):
         return 2 * NC * 2 * esize
     if fam == 4:  # pgrad: p + u read, u written (CN: + p read/write)
         return (7 if not prob.implicit else 9) * N * esize
-    if fam == 5:  # CN, one component per launch: u_c, q_c, p read, u_c written
-        return 4 * N * esize
+    if fam == 5:  # CN per component: u_c, q_c, p read, u_c written; three per launch: p read once
+        return (10 if cn3 else 4) * N * esize
     return 0.0
 
 
@@ -466,7 +467,7 @@ def run_gpu(prob, steps, warmup, cfl=0.5, fixed_dt=None, ic=None, events=True):
     _lib.check(L.dns_fv_maxdiv(s.handle, _lib.dptr(md)))
     ke = np.zeros(1)
     _lib.check(L.dns_fv_kinetic_energy(s.handle, _lib.dptr(ke)))
-    return dict(seconds=t1 - t0, fam=fam, maxdiv=float(md[0]), ke=float(ke[0]), t=t)
+    return dict(seconds=t1 - t0, fam=fam, maxdiv=float(md[0]), ke=float(ke[0]), t=t, describe=s.describe())
 
 
 def cpu_model():
@@ -571,10 +572,12 @@ def _stream_copy_child(nbytes, reps):
     return 2 * nbytes / sec / 1e9
 
 
-def measured_traffic(prob, family):
+def measured_traffic(prob, family, form=None):
     """DRAM bytes per launch of `family` on this workload from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated), or None when
-    no profile of this exact workload and kernel family is committed."""
+    no profile of this exact workload, kernel family and kernel form (the
+    first word of the handle's description of the family, when given) is
+    committed."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"{family}_traffic_channel_{prob.n[0]}x{prob.n[1]}x{prob.n[2]}_{prob.precision}.json")
     try:
@@ -583,7 +586,8 @@ def measured_traffic(prob, family):
     except (OSError, ValueError):
         return None
     if (list(rec.get("n", [])) != list(prob.n) or rec.get("ydiffusion") != prob.ydiffusion
-            or rec.get("precision") != prob.precision or rec.get("family") != family):
+            or rec.get("precision") != prob.precision or rec.get("family") != family
+            or (form is not None and rec.get("form") != form)):
         return None
     return rec["bytes_per_launch"]
 
@@ -705,11 +709,13 @@ def main():
     dom = max(r["fam"], key=lambda f: r["fam"][f][0])
     ms_tot, cnt = r["fam"][dom]
     avg_s = ms_tot / cnt / 1e3
-    fam_bytes = family_bytes_per_launch(prob, dom, esize)
+    cn_form = r["describe"].get("cn", "").split(" ")[0] or None
+    cn3 = cn_form == "seg3"
+    fam_bytes = family_bytes_per_launch(prob, dom, esize, cn3)
     achieved = fam_bytes / avg_s / 1e9
     fam_summary = {FAMILIES[f]: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps,
                                  "avg_us": 1e3 * v[0] / v[1],
-                                 "gbs": family_bytes_per_launch(prob, f, esize) / (v[0] / v[1] / 1e3) / 1e9}
+                                 "gbs": family_bytes_per_launch(prob, f, esize, cn3) / (v[0] / v[1] / 1e3) / 1e9}
                    for f, v in r["fam"].items()}
     # the other precision forms of the same workload and their speed ratios
     precision_forms = {prob.precision: {"steps_per_s": sps, "frac": step_bytes * sps / 1e9 / HBM_PEAK_GBS}}
@@ -767,7 +773,7 @@ def main():
         "mcell_updates_per_s": N * sps / 1e6,
         "roofline": {"bound": "hbm", "kernel": FAMILIES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(prob, FAMILIES[dom]),
+                     "traffic": measured_traffic(prob, FAMILIES[dom], cn_form if dom == 5 else None),
                      "algorithmic_bytes": fam_bytes, "attainable_gbs": copy_gbs},
         "step_roofline": {"model": "minimal-pass", "units_per_step": units, "bytes_per_step": step_bytes,
                           "achieved_gbs": step_bytes * sps / 1e9, "frac": step_bytes * sps / 1e9 / HBM_PEAK_GBS,

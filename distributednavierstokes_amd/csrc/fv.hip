@@ -491,18 +491,21 @@ template <typename T> struct FVImpl : FVBase {
                         cn_tl *= 2;
                     }
                 }
-                // DNS_CN_FUSED=1/2: the three components in one launch (k_cn_seg3),
-                // the pressure rows re-read through L2 / staged once in LDS.  Measured
-                // on the 512x384x512 channel: FP64 +1-2 % steps/s with 2 (17 % fewer
-                // bytes at a lower bandwidth: two blocks per CU), mixed 4-5 % slower;
-                // one launch per component stays the default
+                // The three components in one launch (k_cn_seg3) with the pressure
+                // rows staged once in LDS (2) or re-read through L2 (1), or one
+                // launch per component (0).  Measured on the 512x384x512 channel:
+                // FP64 2 is 2.4 % faster in steps/s than 0 (17 % fewer bytes at a
+                // lower bandwidth: two blocks per CU) and 1 no faster; mixed 2 is
+                // 4-5 % slower.  Default: 2 in FP64 where its LDS fits, else 0;
+                // DNS_CN_FUSED=0|1|2 overrides (1 falls back from 2 when the LDS
+                // does not fit)
                 {
                     const char *ef = getenv("DNS_CN_FUSED");
-                    cn_fused = ef ? atoi(ef) : 0;
+                    cn_fused = ef ? atoi(ef) : (sizeof(T) == 8 ? 2 : 0);
                     if (!cn_seg || cn_nl != 1 || cn_fused < 0 || cn_fused > 2) cn_fused = 0;
                     if (cn_fused == 2 &&
                         (size_t)(3 + cn_seg + 1) * cn_nthr * sizeof(double) + (size_t)7 * ny * sizeof(double) > (size_t)maxlds)
-                        cn_fused = 1;
+                        cn_fused = ef ? 1 : 0;
                 }
                 if (cn_tl && !cn_seg && (ny + cn_nthr / cn_tl - 1) / (cn_nthr / cn_tl) > CN_RMAX) {
                     set_error("CN tile %d:%d leaves more than %d rows per thread", cn_tl, cn_nthr, CN_RMAX);

@@ -119,22 +119,27 @@ def test_c3_y_solves(precision, full_handles, monkeypatch):
     d = s.describe()
     monkeypatch.delenv("DNS_MOM_ZC")
     assert d["ysolve"].startswith("seg R=12" if precision == "fp64" else "seg R=24")
-    assert d["cn"].startswith("seg R=24")
+    # FP64: the three components per launch by default (k_cn_seg3, pressure in LDS)
+    assert d["cn"].startswith("seg3 R=24" if precision == "fp64" else "seg R=24")
     _same_instances(d, full_handles["c3_" + precision], ["ysolve", "cn", "momentum"])
     _check(ug, u, pg, po, precision)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("fused", ["1", "2"])
-def test_c3_cn_three_components_per_launch(precision, fused, monkeypatch):
-    """The same CN segments with the three velocity components in one launch
+@pytest.mark.parametrize("fused", ["0", "1", "2"])
+def test_c3_cn_launch_forms(precision, fused, monkeypatch):
+    """The same CN segments one component per launch (k_cn_seg,
+    DNS_CN_FUSED=0) and with the three velocity components in one launch
     (k_cn_seg3, DNS_CN_FUSED=1: pressure rows re-read per component; 2: staged
     once in LDS)."""
     monkeypatch.setenv("DNS_CN_FUSED", fused)
     ug, u, pg, po, s = _run_pair((64, 384, 64), "implicit", precision)
     d = s.describe()
     monkeypatch.delenv("DNS_CN_FUSED")
-    assert d["cn"].startswith("seg3 R=24") and d["cn"].endswith("p=lds" if fused == "2" else "p=l2")
+    if fused == "0":
+        assert d["cn"].startswith("seg R=24")
+    else:
+        assert d["cn"].startswith("seg3 R=24") and d["cn"].endswith("p=lds" if fused == "2" else "p=l2")
     _check(ug, u, pg, po, precision)
 
 
