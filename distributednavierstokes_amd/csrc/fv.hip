@@ -360,7 +360,7 @@ template <typename T> struct FVImpl : FVBase {
             }
             std::vector<double> *sets[6] = {&at, &bt, &ct, &an, &bn, &cn};
             for (int s = 0; s < 6; ++s) DNS_TRY(dupload(&ydc[s], *sets[s]));
-            DNS_TRY(dalloc(&cncoef, 4 * (size_t)ny));
+            DNS_TRY(dalloc(&cncoef, 12 * (size_t)ny)); // (stage, set): c', 1/β
         }
 
         // FFT tiles
@@ -885,11 +885,20 @@ template <typename T> struct FVImpl : FVBase {
         else cn_seg_launch_nl<R, 1>(c, grd, blk, a);
     }
 
-    int launch_cn(double B, double alpha, double dt)
+    // the CN pivots of the three stages of a step at Δt (k_cn_coeffs3), read
+    // by launch_cn(stage)
+    void launch_cn_coeffs(double dt)
     {
+        hipLaunchKernelGGL(k_cn_coeffs3, dim3(1), dim3(NTHR), (size_t)6 * g.ny * sizeof(double), st, ydc[0], ydc[1],
+                           ydc[2], ydc[3], ydc[4], ydc[5], RK_ALPHA[0] * dt * visc / 2, RK_ALPHA[1] * dt * visc / 2,
+                           RK_ALPHA[2] * dt * visc / 2, g.ny, cncoef);
+    }
+
+    int launch_cn(int stage, double dt)
+    {
+        const double B = RK_B[stage], alpha = RK_ALPHA[stage];
         const double theta = alpha * dt * visc / 2;
-        hipLaunchKernelGGL(k_cn_coeffs, dim3(1), dim3(NTHR), (size_t)6 * g.ny * sizeof(double), st, ydc[0], ydc[1], ydc[2],
-                           ydc[3], ydc[4], ydc[5], theta, g.ny, cncoef);
+        const double *coef = cncoef + (size_t)stage * 4 * g.ny; // this stage's (set 0, set 1) pivots
         if (cn_fused) {
             CN3Args<T> a;
             for (int c = 0; c < 3; ++c) {
@@ -901,7 +910,7 @@ template <typename T> struct FVImpl : FVBase {
                 a.ya[set] = ydc[3 * set + 0];
                 a.yb[set] = ydc[3 * set + 1];
                 a.yc[set] = ydc[3 * set + 2];
-                a.cp[set] = cncoef + set * 2 * g.ny;
+                a.cp[set] = coef + set * 2 * g.ny;
                 a.ip[set] = a.cp[set] + g.ny;
             }
             a.rdyu = rdyu_d;
@@ -943,7 +952,7 @@ template <typename T> struct FVImpl : FVBase {
             a.a = ydc[3 * set + 0];
             a.b = ydc[3 * set + 1];
             a.c = ydc[3 * set + 2];
-            a.cp = cncoef + set * 2 * g.ny;
+            a.cp = coef + set * 2 * g.ny;
             a.ip = a.cp + g.ny;
             a.rdyu = rdyu_d;
             a.B = B;
@@ -1259,6 +1268,7 @@ template <typename T> struct FVImpl : FVBase {
 
     int step_launch(double dt)
     {
+        if (imexy) launch_cn_coeffs(dt);
         for (int s = 0; s < 3; ++s) {
             if (!imexy) {
                 DNS_TRY(launch_momentum(u, v, RK_A[s], dt, RK_B[s], true));
@@ -1267,7 +1277,7 @@ template <typename T> struct FVImpl : FVBase {
                 for (int c = 0; c < 3; ++c) std::swap(u[c], v[c]);
             } else {
                 DNS_TRY(launch_momentum(u, nullptr, RK_A[s], dt, 0.0, false));
-                DNS_TRY(launch_cn(RK_B[s], RK_ALPHA[s], dt));
+                DNS_TRY(launch_cn(s, dt));
                 DNS_TRY(launch_poisson(u, delta, 0));
                 DNS_TRY(launch_pgrad(u, delta, true, 1.0 / (RK_ALPHA[s] * dt)));
             }

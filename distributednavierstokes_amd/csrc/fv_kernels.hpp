@@ -1960,6 +1960,46 @@ __global__ void __launch_bounds__(256) k_cn_coeffs(const double *__restrict__ at
     }
 }
 
+// The pivots of k_cn_coeffs for the three RK stages of a step at once
+// (theta_s = α_s Δt ν / 2): the six recurrences (stage, set) run on six
+// threads side by side, the rows staged in LDS, the results written straight
+// to out[(2 s + set) · 2 ny ...] (c'_j, then 1/β_j), the same arithmetic per
+// recurrence as k_cn_coeffs.  One launch per step instead of three.
+__global__ void __launch_bounds__(256) k_cn_coeffs3(const double *__restrict__ at, const double *__restrict__ bt,
+                                                    const double *__restrict__ ct, const double *__restrict__ an,
+                                                    const double *__restrict__ bn, const double *__restrict__ cn,
+                                                    double theta0, double theta1, double theta2, int ny,
+                                                    double *__restrict__ out)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *rows = reinterpret_cast<double *>(smem_raw); // [set][a, b, c][ny]
+    for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+        rows[0 * ny + j] = at[j];
+        rows[1 * ny + j] = bt[j];
+        rows[2 * ny + j] = ct[j];
+        rows[3 * ny + j] = an[j];
+        rows[4 * ny + j] = bn[j];
+        rows[5 * ny + j] = cn[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int stage = threadIdx.x >> 1, set = threadIdx.x & 1;
+        const double theta = stage == 0 ? theta0 : stage == 1 ? theta1 : theta2;
+        const double *a = rows + 3 * set * ny, *b = a + ny, *c = b + ny;
+        double *cpo = out + (size_t)(2 * stage + set) * 2 * ny, *ipo = cpo + ny;
+        double beta = 1.0 - theta * b[0];
+        double cp = -theta * c[0] / beta;
+        cpo[0] = cp;
+        ipo[0] = 1.0 / beta;
+        for (int j = 1; j < ny; ++j) {
+            beta = (1.0 - theta * b[j]) + theta * a[j] * cp;
+            cp = -theta * c[j] / beta;
+            cpo[j] = cp;
+            ipo[j] = 1.0 / beta;
+        }
+    }
+}
+
 // ------------------------------------------------------------ reductions
 
 // Per-block maxima of |u_c| over the interior (CFL, src/timestep.jl:69-80).
