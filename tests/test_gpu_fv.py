@@ -133,7 +133,7 @@ def test_step_parity(name):
         dns.step_(ug, pg, qg, dg, s, 0.004)
     for c in range(3):
         assert relmax(ug[c], u[c]) < 1e-11, c
-    assert relmax(pg, po) < 1e-10
+    assert relmax(pg, po) < 1e-11
     assert dns.maxdiv(ug, s) < 1e-12
 
 
@@ -157,7 +157,7 @@ def test_pgrad_kernels_step_parity(name, pgrad, monkeypatch):
         dns.step_(ug, pg, qg, dg, s, 0.004)
     for c in range(3):
         assert relmax(ug[c], u[c]) < 1e-11, c
-    assert relmax(pg, po) < 1e-10
+    assert relmax(pg, po) < 1e-11
     assert math.isclose(dns.cfl_timestep(ug, s, 0.5), o.cfl_timestep(ug, 0.5), rel_tol=1e-14)
 
 
@@ -183,7 +183,7 @@ def test_cn_variants_step_parity(n, variant, monkeypatch):
         dns.step_(ug, pg, qg, dg, s, 0.004)
     for c in range(3):
         assert relmax(ug[c], u[c]) < 1e-11, c
-    assert relmax(pg, po) < 1e-10
+    assert relmax(pg, po) < 1e-11
 
 
 @pytest.mark.parametrize("graph", ["", "0"])
@@ -220,7 +220,7 @@ def test_fixed_dt_graph_replay_parity(name, graph, monkeypatch):
     s.download("p", pg)
     for c in range(3):
         assert relmax(ug[c], u[c]) < 1e-11, c
-    assert relmax(pg, po) < 1e-10
+    assert relmax(pg, po) < 1e-11
     assert math.isclose(float(out[0]), o.cfl_timestep(u, 0.5), rel_tol=1e-13)
 
 
@@ -243,7 +243,7 @@ def test_plain_step_parity(name):
         dns.step_(ug, pg, qg, dg, s, 0.004)
     for c in range(3):
         assert relmax(ug[c], u[c]) < 1e-11, c
-    assert relmax(pg, po) < 1e-10
+    assert relmax(pg, po) < 1e-11
     assert dns.maxdiv(ug, s) < 1e-12
 
 
